@@ -1,0 +1,204 @@
+"""Band-solve benchmark on one MI355X.
+
+A step is one complete band solve of the 64-atom workload (cubic a = 20.52
+bohr, 96^3 box, Gamma point, G = 16135 plane waves, 256 bands + 26 extra,
+512 projectors) in the mixed form: complex64 H*psi and subspace algebra,
+FP64 Cholesky/eigh, FP64 Rayleigh-Ritz finish. Each step starts from the
+same seeded random bands (randomize + orthonormalize are inside the step);
+the cell, potential and projectors are resident in HBM before timing.
+
+Printed (one JSON line): band solves/s of the mixed form, the plain FP64 form
+timed in the same run, the eigenvalue error of mixed vs the FP64 path, the
+roofline of the dominant HIP kernel (live HIP-event timing of every library
+launch on its stream, with its compulsory bytes) and the FP64 CPU oracle
+timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("cfg1: 64-atom cubic cell a=20.52 bohr, 96^3 box, Gamma only, G=16135, "
+            "256 bands (+26 extra), 512 projectors, Davidson")
+MIXED_THRESHOLD = 1e-7  # complex64 phase stops at |deig| < 1e-7, then FP64 Rayleigh-Ritz
+PLAIN_THRESHOLD = 1e-8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
+    ap.add_argument("--cfg", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(spec, n_iterations_plain: int) -> dict:
+    """FP64 oracle (numpy/scipy) on this host: 2 Davidson iterations of the
+    same cell, extrapolated to the iteration count of the converged FP64 solve."""
+    import psutil
+
+    cores = psutil.cpu_count(logical=False) or os.cpu_count() or 1
+    os.environ.setdefault("ORACLE_THREADS", str(cores))
+    from threadpoolctl import threadpool_limits
+
+    from oracle.basis import Basis as OBasis
+    from oracle.davidson import davidson, initial_state
+    from oracle.hamiltonian import Hamiltonian
+
+    with threadpool_limits(limits=cores):
+        ob = OBasis(spec)
+        H = Hamiltonian(ob)
+        C = initial_state(ob, spec.n_bands)
+        t0 = time.perf_counter()
+        davidson(ob, H, C, spec.n_bands, spec.n_bands_extra, 2, 0.0)
+        t = time.perf_counter() - t0
+    per_iter = t / 3.0  # initial H*psi + eigh counts as one more iteration of work
+    t_solve = per_iter * (n_iterations_plain + 1)
+    return {
+        "value": 1.0 / t_solve, "unit": "band solves/s", "cores": int(cores), "kind": "port",
+        "sample": (f"FP64 numpy/scipy oracle, 2 Davidson iterations of the same cell "
+                   f"({t:.1f} s on {cores} threads of {cpu_model()}), extrapolated to the "
+                   f"{n_iterations_plain} iterations of the FP64 GPU solve"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(requests, cells and k-points would shard across GPUs with no collectives)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from qimpy_amd import _lib, rc, systems
+    from qimpy_amd.electrons import electrons_from_spec
+
+    rc.init()
+    rc.require_gpu()
+    spec = systems.make_cell(args.cfg)
+
+    def make(dtype, thr):
+        el = electrons_from_spec(spec, dtype=dtype, n_iterations=200, eig_threshold=thr)
+        return el
+
+    el_mixed = make(torch.complex64, MIXED_THRESHOLD)
+    el_plain = make(torch.complex128, PLAIN_THRESHOLD)
+
+    def solve(el):
+        el.initialize_wavefunctions(seed=0)
+        return el.solve()
+
+    # Warm-up (also JIT-free: all kernels are precompiled):
+    for _ in range(args.warmup):
+        solve(el_mixed)
+    torch.cuda.synchronize()
+
+    # Timed mixed solves, with every library launch bracketed by HIP events:
+    _lib.profile_enable(True)
+    _lib.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [solve(el_mixed) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    _lib.profile_enable(False)
+    prof = _lib.profile_read()
+
+    # Plain FP64 form, same run:
+    for _ in range(min(args.warmup, 1)):
+        pass
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = [solve(el_plain) for _ in range(args.plain_steps)]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+
+    eig_m = reps[-1].eig.cpu().numpy()
+    eig_p = plain[-1].eig.cpu().numpy()
+    err = float(np.abs(eig_m - eig_p).max())
+    eband_err = float(abs(reps[-1].Eband - plain[-1].Eband))
+    value = args.steps / t_mixed
+    plain_value = args.plain_steps / t_plain
+
+    # Roofline of the dominant HIP kernel (largest total time):
+    kid = max(prof, key=lambda i: prof[i][0])
+    ms, by, cnt = prof[kid]
+    avg_s = ms / cnt * 1e-3
+    achieved = by / cnt / avg_s / 1e9
+    kernels = {
+        _lib.KERNEL_NAMES.get(i, str(i)): {
+            "launches": c, "avg_us": round(m / c * 1e3, 2),
+            "GBps": round(b / c / (m / c * 1e-3) / 1e9, 1), "share_of_step": round(m * 1e-3 / t_mixed, 3),
+        }
+        for i, (m, b, c) in prof.items()
+    }
+    nb = spec.n_bands + spec.n_bands_extra
+    N = int(np.prod(spec.shape))
+    b_loc = 8 * (2 * el_mixed.basis.n_tot + 4 * N) + 4 * N / nb  # bytes/band, c64
+
+    out = {
+        "metric": "band solves/s (mixed c64/FP64 Davidson, <=1e-6 Ha vs FP64)",
+        "value": round(value, 4),
+        "unit": "band solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mixed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+f64",
+        "data": "synthetic (seeded cell, potential, projectors and start bands)",
+        "config": {"workload": WORKLOAD, "G": el_mixed.basis.n_tot, "box": list(spec.shape),
+                   "n_bands": spec.n_bands, "n_bands_extra": spec.n_bands_extra,
+                   "projectors": spec.n_projectors, "mixed_eig_threshold": MIXED_THRESHOLD,
+                   "plain_eig_threshold": PLAIN_THRESHOLD},
+        "plain_fp64": {"value": round(plain_value, 4), "unit": "band solves/s",
+                       "ms_per_solve": round(1e3 * t_plain / args.plain_steps, 2),
+                       "iterations": plain[-1].n_iterations},
+        "mixed_over_plain": round(value / plain_value, 3),
+        "mixed_iterations": reps[-1].n_iterations,
+        "eig_err_vs_fp64_Ha": err,
+        "Eband_err_vs_fp64_Ha": eband_err,
+        "max_residual_mixed": float(reps[-1].residual_norm.max().item()),
+        "roofline": {
+            "kernel": _lib.KERNEL_NAMES.get(kid, str(kid)),
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": round(by / cnt),
+        },
+        "kernels": kernels,
+        "b_loc_convention": {"bytes_per_band": round(b_loc)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(spec, plain[-1].n_iterations)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

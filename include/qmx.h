@@ -1,0 +1,165 @@
+/* qmx — C ABI of the MI355X (gfx950) plane-wave band-solve kernels.
+ *
+ * Plain pointers, explicit int64 sizes and strides, an explicit HIP stream
+ * (passed as void*, i.e. a hipStream_t), and an int status (0 = OK) on every
+ * entry point. No C++ exception crosses this boundary; on failure
+ * qmx_last_error() returns a message for the calling thread. The library
+ * never frees caller memory; workspace is caller-provided and sized by
+ * qmx_plan_workspace_bytes(). Kernels are re-entrant; one plan must not be
+ * used by two streams at once.
+ *
+ * Layout conventions (those of QimPy's Wavefunction and Basis):
+ *   - a wavefunction block is complex (interleaved re, im), element
+ *     (k, b, g) at ptr[k * ldk + b * ldb + g], g < n_tot;
+ *   - the FFT box is C-ordered (n0, n1, n2) with the last axis fastest;
+ *   - fft_index[k * n_tot + g] is the linear box index of plane wave g of
+ *     k-point k (dft/electrons/_basis.py:186-193); entries g >= n[k] are
+ *     padding and are ignored on input and zeroed on output.
+ */
+#ifndef QMX_H
+#define QMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types of wavefunction data. */
+enum { QMX_C64 = 0, QMX_C128 = 1 };
+
+/* Status codes. */
+enum {
+  QMX_OK = 0,
+  QMX_ERR_ARG = 1,    /* invalid argument (shape, null pointer, size) */
+  QMX_ERR_HIP = 2,    /* HIP runtime error */
+  QMX_ERR_NOMEM = 3,  /* workspace too small / allocation failed */
+  QMX_ERR_UNSUPPORTED = 4
+};
+
+/* Geometry of one plane-wave basis (all host pointers). */
+typedef struct qmx_geom {
+  int32_t shape[3];          /* FFT box; each a product of 2, 3 and 5, <= 512 */
+  int32_t nk;                /* number of k-points sharing the box */
+  int32_t n_tot;             /* padded basis length */
+  const int32_t* n;          /* [nk] in-sphere counts */
+  const int64_t* fft_index;  /* [nk * n_tot] linear box indices */
+} qmx_geom;
+
+typedef struct qmx_plan qmx_plan;
+
+/* Host-only: the sparse-FFT index tables a plan uploads, for inspection
+ * and CPU checks (no device calls). sizes[0] = nxa (active x-planes),
+ * sizes[1] = ncol (active (x, y) columns). Any table pointer may be NULL;
+ * call once with NULL tables to learn the sizes. Tables:
+ *   x_list[nxa]            box x of each active plane (ascending)
+ *   pcol_start[nxa + 1]    first column of each plane
+ *   col_iy[ncol]           box y of each column
+ *   cstart[nk * (ncol+1)]  first basis index of each column, per k
+ *   t1idx[nk * n_tot]      column * shape[2] + z of each in-sphere point
+ * The column grouping relies on fft_index increasing within each k, which
+ * the reference's argsort construction guarantees (_basis.py:186-193). */
+int qmx_geom_tables(const qmx_geom* geom, int64_t* sizes, int32_t* x_list,
+                    int32_t* pcol_start, int32_t* col_iy, int32_t* cstart,
+                    int32_t* t1idx);
+
+/* Build the sparse-FFT index tables of a basis on the current device.
+ * Replaces the index set-up of _KernelCommon.__init__
+ * (dft/electrons/_basis_ops.py:119-141) and Basis.fft_index/pad_index
+ * (_basis.py:186-204). */
+int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan);
+int qmx_plan_destroy(qmx_plan* plan);
+
+/* Workspace bytes needed by qmx_apply_local for n_bands bands per k.
+ * Any smaller (nonzero) workspace is accepted as long as it holds one band
+ * of every k; the bands are then processed in blocks. */
+int64_t qmx_plan_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands);
+
+/* Query plan geometry: [0] active x-planes, [1] active (x, y) columns. */
+int qmx_plan_info(const qmx_plan* plan, int64_t* info);
+
+/* Local-potential apply, optionally with the kinetic term:
+ *   out(k, b, G) = [accumulate ? out : 0] + KE(k, G) C(k, b, G)
+ *                  + (1/N) FFT[ V(r) IFFT[C](r) ](G)
+ * i.e. _apply_potential (dft/electrons/_basis_ops.py:39-113, scatter,
+ * ifftn(norm="forward"), *= V, fftn(norm="forward"), gather, constrain)
+ * fused with _apply_ke (_basis_ops.py:29-37) when ke != NULL.
+ * V: real box (float for QMX_C64, double for QMX_C128), shared by all k.
+ * ke: [nk * ke_ldk] real of the same precision, or NULL. out may equal C. */
+int qmx_apply_local(const qmx_plan* plan, int dtype, const void* C, int64_t ldb,
+                    int64_t ldk, void* out, int64_t out_ldb, int64_t out_ldk,
+                    const void* V, const void* ke, int64_t ke_ldk, int64_t n_bands,
+                    int accumulate, void* workspace, int64_t workspace_bytes,
+                    void* stream);
+
+/* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
+ * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
+ * The result is multiplied by `scale` (1/N for QimPy's forward transform,
+ * 1 for its inverse: grid/_grid.py:220-268, grid/_fft.py:228-337). */
+int qmx_fft3d(int dtype, const int32_t* shape, void* data, int64_t batch,
+              int direction, double scale, void* stream);
+
+/* Per-band reductions norm2[k, b] = sum_g |C|^2 and (if ke != NULL)
+ * kin[k, b] = sum_g |C|^2 KE(k, g), accumulated in double
+ * (Wavefunction.band_norm / band_ke, _wavefunction.py:226-247).
+ * ke is double [nk * ke_ldk]. Outputs are double, [nk * nb] row-major. */
+int qmx_band_reduce(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t nk,
+                    int64_t nb, int64_t n_tot, const double* ke, int64_t ke_ldk,
+                    double* norm2, double* kin, void* stream);
+
+/* Davidson residual + inverse-kinetic preconditioner (_davidson.py:93-99,
+ * :182-184):  out = (HC - eig C) / x,  x = KE/KEref,  x <- x + exp(-x);
+ * also writes norm2[k, b] = sum_g |out|^2 (double). eig, KEref: double
+ * [nk * nb]; ke double [nk * ke_ldk]. */
+int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb,
+                         int64_t ldk, const double* eig, const double* KEref,
+                         const double* ke, int64_t ke_ldk, void* out, int64_t out_ldb,
+                         int64_t out_ldk, int64_t nk, int64_t nb, int64_t n_tot,
+                         double* norm2, void* stream);
+
+/* Scale each band by scale[k, b] (double, or 1 if NULL) and zero the
+ * padding g >= n[k] (n: device int32 [nk]) — Wavefunction.constrain
+ * (_wavefunction.py:264-275) fused with the band normalisation of
+ * _davidson.py:186-187. */
+int qmx_scale_constrain(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk,
+                        int64_t nb, int64_t n_tot, const double* scale,
+                        const int32_t* n, void* stream);
+
+/* Bandwidth-limited xorshift randomisation of bands [b_start, b_stop)
+ * (Wavefunction.randomize, _wavefunction_init.py:27-125): state
+ * 1 + seed + g + n_max (k_offset + k), b_start + 1 discarded draws, then
+ * one complex normal per band, zero padding, times 1/(1+((4/3)KE)^6).
+ * n: device int32 [nk]; ke: device double [nk * ke_ldk]. */
+int qmx_randomize(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk,
+                  int64_t n_tot, int64_t n_max, int64_t k_offset, const int32_t* n,
+                  const double* ke, int64_t ke_ldk, int64_t seed, int64_t b_start,
+                  int64_t b_stop, void* stream);
+
+/* Kernel ids reported by the profiler. */
+enum {
+  QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */
+  QMX_KERNEL_PASS_B = 1, /* columns -> planes, inverse y */
+  QMX_KERNEL_PASS_C = 2, /* x pass: inverse x, * V, forward x */
+  QMX_KERNEL_PASS_D = 3, /* planes -> columns, forward y */
+  QMX_KERNEL_PASS_E = 4, /* columns -> sphere, forward z, KE, constrain */
+  QMX_KERNEL_FFT3D = 5,
+  QMX_KERNEL_GEMM = 6,
+  QMX_N_KERNEL_IDS = 16
+};
+
+/* Event-based profiling of this library's launches (no counterpart in the
+ * reference, whose StopWatch (src/qimpy/profiler.py:15-117) times whole
+ * Python calls). When enabled, each launch is bracketed by HIP events on its
+ * own stream; qmx_profile_read() waits for them and returns, per kernel id,
+ * the summed milliseconds, the summed compulsory HBM bytes and the launch
+ * count, then clears the records. */
+int qmx_profile_enable(int on);
+int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids);
+
+const char* qmx_last_error(void);
+const char* qmx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMX_H */

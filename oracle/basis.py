@@ -1,0 +1,44 @@
+"""Oracle plane-wave basis (test infrastructure; see oracle/__init__.py).
+
+Restates dft/electrons/_basis.py:167-204 and get_ke (:256-270) for one
+process: the full signed-frequency box mesh in C order, the sphere mask
+``KE < ke_cutoff``, ``n_tot = n_max``, and ``fft_index`` = the in-sphere
+linear box indices in increasing order followed by the first out-of-sphere
+indices as padding (argsort of ``where(mask, 0, N) + arange(N)``).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from qimpy_amd.systems import CellSpec, signed_freqs
+
+
+class Basis:
+    def __init__(self, spec: CellSpec):
+        self.spec = spec
+        self.shape = tuple(spec.shape)
+        self.N = int(np.prod(self.shape))
+        self.Gbasis = spec.Gbasis
+        self.k = spec.k
+        mesh = np.stack(
+            np.meshgrid(*[signed_freqs(s) for s in self.shape], indexing="ij"), -1
+        ).reshape(-1, 3)
+        ke_full = self._ke(mesh[None], self.k)  # (K, N)  _basis.py:171
+        within = ke_full < spec.ke_cutoff
+        self.n = within.sum(axis=1)  # :173
+        self.n_min, self.n_max = int(self.n.min()), int(self.n.max())
+        self.n_tot = self.n_max  # :177 with one basis process
+        keys = np.where(within, 0, self.N) + np.arange(self.N)[None]
+        self.fft_index = np.argsort(keys, axis=1, kind="stable")[:, : self.n_tot]
+        self.iG = mesh[self.fft_index]  # (K, n_tot, 3)  :194
+        self.pad_mask = np.arange(self.n_tot)[None] >= self.n[:, None]  # :195-197
+        self.KE = self._ke(self.iG, self.k)  # (K, n_tot)
+        self.w_sk = 2.0 * spec.wk  # integer spin weight 2 // (S X)  :137-138
+
+    def _ke(self, iG: np.ndarray, k: np.ndarray) -> np.ndarray:
+        G = (iG + k[:, None, :]) @ self.Gbasis.T
+        return 0.5 * (G**2).sum(-1)
+
+    def constrain(self, C: np.ndarray) -> None:
+        """Zero padded coefficients in place (_wavefunction.py:264-272)."""
+        C[np.broadcast_to(self.pad_mask[:, None, :], C.shape)] = 0.0

@@ -1,0 +1,152 @@
+"""ctypes binding of libqmx.so, the C ABI declared in include/qmx.h.
+
+The library is loaded once, from ``qimpy_amd/_lib/libqmx.so`` (built in-tree
+by ``qimpy_amd.build``). There is no fallback: if the library is missing or a
+call fails, a ``RuntimeError`` carrying ``qmx_last_error()`` is raised.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libqmx.so"
+
+QMX_C64, QMX_C128 = 0, 1
+
+c_i32p = ct.POINTER(ct.c_int32)
+c_i64p = ct.POINTER(ct.c_int64)
+c_f64p = ct.POINTER(ct.c_double)
+vp = ct.c_void_p
+i64 = ct.c_int64
+i32 = ct.c_int
+
+
+class QmxGeom(ct.Structure):
+    _fields_ = [
+        ("shape", ct.c_int32 * 3),
+        ("nk", ct.c_int32),
+        ("n_tot", ct.c_int32),
+        ("n", c_i32p),
+        ("fft_index", c_i64p),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "qmx_geom_tables": (i32, [ct.POINTER(QmxGeom), c_i64p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p]),
+    "qmx_plan_create": (i32, [ct.POINTER(QmxGeom), ct.POINTER(vp)]),
+    "qmx_plan_destroy": (i32, [vp]),
+    "qmx_plan_workspace_bytes": (i64, [vp, i32, i64]),
+    "qmx_plan_info": (i32, [vp, c_i64p]),
+    "qmx_apply_local": (
+        i32,
+        [vp, i32, vp, i64, i64, vp, i64, i64, vp, vp, i64, i64, i32, vp, i64, vp],
+    ),
+    "qmx_fft3d": (i32, [i32, c_i32p, vp, i64, i32, ct.c_double, vp]),
+    "qmx_band_reduce": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, i64, vp, vp, vp]),
+    "qmx_precond_residual": (
+        i32,
+        [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp],
+    ),
+    "qmx_scale_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
+    "qmx_randomize": (
+        i32,
+        [i32, vp, i64, i64, i64, i64, i64, i64, vp, vp, i64, i64, i64, i64, vp],
+    ),
+    "qmx_profile_enable": (i32, [i32]),
+    "qmx_profile_read": (i32, [vp, vp, vp, i32]),
+    "qmx_last_error": (ct.c_char_p, []),
+    "qmx_version": (ct.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    """The loaded library (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m qimpy_amd.build`"
+            )
+        handle = ct.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().qmx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")
+
+
+def call(name: str, *args) -> int:
+    status = getattr(lib(), name)(*args)
+    check(status, name)
+    return status
+
+
+def make_geom(shape, n: np.ndarray, fft_index: np.ndarray) -> tuple[QmxGeom, tuple]:
+    """Geometry struct over host arrays (returned tuple keeps them alive)."""
+    n32 = np.ascontiguousarray(n, dtype=np.int32)
+    fi = np.ascontiguousarray(fft_index, dtype=np.int64)
+    nk, n_tot = fi.shape
+    g = QmxGeom()
+    for i in range(3):
+        g.shape[i] = int(shape[i])
+    g.nk = nk
+    g.n_tot = n_tot
+    g.n = n32.ctypes.data_as(c_i32p)
+    g.fft_index = fi.ctypes.data_as(c_i64p)
+    return g, (n32, fi)
+
+
+def geom_tables(shape, n: np.ndarray, fft_index: np.ndarray) -> dict:
+    """Host-side sparse-FFT tables (qmx_geom_tables) as numpy arrays."""
+    g, keep = make_geom(shape, n, fft_index)
+    sizes = np.zeros(2, dtype=np.int64)
+    null = ct.cast(None, c_i32p)
+    call("qmx_geom_tables", ct.byref(g), sizes.ctypes.data_as(c_i64p), null, null, null, null, null)
+    nxa, ncol = int(sizes[0]), int(sizes[1])
+    nk, n_tot = keep[1].shape
+    out = {
+        "x_list": np.zeros(nxa, np.int32),
+        "pcol_start": np.zeros(nxa + 1, np.int32),
+        "col_iy": np.zeros(ncol, np.int32),
+        "cstart": np.zeros((nk, ncol + 1), np.int32),
+        "t1idx": np.zeros((nk, n_tot), np.int32),
+    }
+    p = lambda a: a.ctypes.data_as(c_i32p)
+    call(
+        "qmx_geom_tables", ct.byref(g), sizes.ctypes.data_as(c_i64p), p(out["x_list"]),
+        p(out["pcol_start"]), p(out["col_iy"]), p(out["cstart"]), p(out["t1idx"]),
+    )
+    out["nxa"], out["ncol"] = nxa, ncol
+    return out
+
+
+KERNEL_NAMES = {
+    0: "k_sphere_to_cols", 1: "k_cols_to_planes", 2: "k_xpass_potential",
+    3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
+}
+N_KERNEL_IDS = 16
+
+
+def profile_enable(on: bool) -> None:
+    call("qmx_profile_enable", int(on))
+
+
+def profile_read() -> dict:
+    """{kernel id: (total ms, total compulsory bytes, launches)} since the last read."""
+    ms = np.zeros(N_KERNEL_IDS)
+    by = np.zeros(N_KERNEL_IDS)
+    cnt = np.zeros(N_KERNEL_IDS, dtype=np.int64)
+    call("qmx_profile_read", ms.ctypes.data, by.ctypes.data, cnt.ctypes.data, N_KERNEL_IDS)
+    return {i: (float(ms[i]), float(by[i]), int(cnt[i])) for i in range(N_KERNEL_IDS) if cnt[i]}

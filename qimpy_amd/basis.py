@@ -1,0 +1,206 @@
+"""Plane-wave basis and its operators (reference: dft/electrons/_basis.py,
+_basis_ops.py).
+
+The sphere is the set of box plane waves with ``KE(G+k) < ke_cutoff``
+(_basis.py:167-177); ``fft_index`` lists their linear box indices in
+increasing order, padded with the first out-of-sphere indices to the common
+length ``n_tot = n_max`` (_basis.py:183-204, one basis process so no extra
+padding). ``apply_potential`` and ``apply_ke`` run on the sparse-FFT HIP
+kernels through ``qmx_apply_local``; nothing here has a CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib, rc
+from .grid import Grid
+from .systems import signed_freqs
+
+DTYPE_CODE = {torch.complex64: _lib.QMX_C64, torch.complex128: _lib.QMX_C128}
+REAL_OF = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+
+#: Largest workspace the local apply allocates; bands beyond it are blocked.
+WORKSPACE_CAP = 1 << 30
+
+
+class Basis:
+    """Plane waves of one cell at a set of k-points, on one GPU."""
+
+    def __init__(
+        self,
+        *,
+        Rbasis: np.ndarray,
+        k: np.ndarray,
+        wk: np.ndarray,
+        ke_cutoff: float,
+        shape,
+        ions=None,
+        n_spins: int = 1,
+        n_spinor: int = 1,
+    ):
+        rc.require_gpu()
+        if n_spinor != 1:
+            raise NotImplementedError("spinorial wavefunctions are not on this path")
+        self.Rbasis = np.asarray(Rbasis, dtype=np.float64)
+        self.Gbasis = 2 * np.pi * np.linalg.inv(self.Rbasis.T)  # lattice/_lattice.py:217
+        self.volume = float(abs(np.linalg.det(self.Rbasis)))
+        self.k_np = np.asarray(k, dtype=np.float64).reshape(-1, 3)
+        self.wk_np = np.asarray(wk, dtype=np.float64)
+        self.n_spins, self.n_spinor = n_spins, n_spinor
+        self.ke_cutoff = float(ke_cutoff)
+        self.ions = ions
+        self.grid = Grid(shape, self.Rbasis)
+        self.real_wavefunctions = False
+        dev = rc.device
+        self.k = torch.tensor(self.k_np, device=dev)
+        self.wk = torch.tensor(self.wk_np, device=dev)
+        w_spin = 2 // (n_spins * n_spinor)  # _basis.py:137
+        self.w_sk = w_spin * self.wk.view(1, -1, 1)
+
+        # Sphere selection and ordering on the host in float64:
+        mesh = np.stack(
+            np.meshgrid(*[signed_freqs(s) for s in self.grid.shape], indexing="ij"), -1
+        ).reshape(-1, 3)
+        N = mesh.shape[0]
+        nk = self.k_np.shape[0]
+        within = np.empty((nk, N), dtype=bool)
+        for ik in range(nk):
+            Gk = (mesh + self.k_np[ik]) @ self.Gbasis.T
+            within[ik] = 0.5 * np.einsum("gi,gi->g", Gk, Gk) < self.ke_cutoff
+        self.n_np = within.sum(axis=1).astype(np.int64)
+        self.n_min, self.n_max = int(self.n_np.min()), int(self.n_np.max())
+        self.n_tot = self.n_max
+        self.n_avg = float((self.n_np * self.wk_np).sum())
+        assert self.n_tot <= N
+        keys = np.where(within, 0, N) + np.arange(N)[None]
+        self.fft_index_np = np.argsort(keys, axis=1, kind="stable")[:, : self.n_tot]
+        iG = mesh[self.fft_index_np]  # (nk, n_tot, 3)
+        self.iG = torch.tensor(iG, device=dev)
+        self.fft_index = torch.tensor(self.fft_index_np, device=dev)
+        self.n = torch.tensor(self.n_np, device=dev)
+        self.n_i32 = self.n.to(torch.int32)
+        g_range = torch.arange(self.n_tot, device=dev)
+        pad = torch.where(g_range[None, :] >= self.n[:, None])
+        self.pad_index = (slice(None), pad[0], slice(None), slice(None), pad[1])
+        self.pad_index_mine = self.pad_index
+        self.mine = slice(0, self.n_tot)
+        Gk = (self.iG.to(torch.float64) + self.k[:, None, :]) @ torch.tensor(
+            self.Gbasis.T, device=dev
+        )
+        self._ke = {torch.float64: 0.5 * Gk.square().sum(dim=-1)}  # get_ke :256-270
+        self._ke[torch.float32] = self._ke[torch.float64].to(torch.float32)
+
+        # Sparse-FFT plan (index tables on the device):
+        geom, keep = _lib.make_geom(self.grid.shape, self.n_np, self.fft_index_np)
+        handle = ct.c_void_p()
+        _lib.call("qmx_plan_create", ct.byref(geom), ct.byref(handle))
+        self._plan = handle
+        info = np.zeros(2, dtype=np.int64)
+        _lib.call("qmx_plan_info", self._plan, info.ctypes.data_as(_lib.c_i64p))
+        self.n_active_planes, self.n_active_columns = int(info[0]), int(info[1])
+        self._workspace: Optional[torch.Tensor] = None
+        self._V_cache: dict = {}
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            try:
+                _lib.lib().qmx_plan_destroy(plan)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+            self._plan = None
+
+    @property
+    def nk(self) -> int:
+        return int(self.k_np.shape[0])
+
+    @property
+    def n_ideal(self) -> float:
+        return ((2.0 * self.ke_cutoff) ** 1.5) * self.volume / (6 * np.pi**2)
+
+    def get_ke(self, basis_slice: slice = slice(None), dtype=torch.float64) -> torch.Tensor:
+        """KE of each plane wave (nk x len(slice)), cached per precision."""
+        return self._ke[dtype][:, basis_slice]
+
+    # ---------------------------------------------------------------- operators
+    def workspace(self, dtype: torch.dtype, n_bands: int) -> tuple[torch.Tensor, int]:
+        need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dtype], n_bands))
+        one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dtype], 1))
+        size = max(one, min(need, WORKSPACE_CAP))
+        if self._workspace is None or self._workspace.numel() < size:
+            self._workspace = torch.empty(size, dtype=torch.uint8, device=rc.device)
+        return self._workspace, int(self._workspace.numel())
+
+    def potential_box(self, V, dtype: torch.dtype) -> torch.Tensor:
+        """V(r) on the box, contiguous, in the real type matching `dtype`."""
+        data = V.data if hasattr(V, "data") and not isinstance(V, torch.Tensor) else V
+        if not isinstance(data, torch.Tensor):
+            data = torch.as_tensor(np.asarray(data), device=rc.device)
+        if data.dim() == 4 and data.shape[0] == 1:
+            data = data[0]
+        if tuple(data.shape) != self.grid.shape:
+            raise ValueError(f"potential shape {tuple(data.shape)} != box {self.grid.shape}")
+        rtype = REAL_OF[dtype]
+        key = (data.data_ptr(), data._version, rtype)
+        cached = self._V_cache.get(key)
+        if cached is None:
+            cached = data.to(device=rc.device, dtype=rtype).contiguous()
+            self._V_cache = {key: cached}  # keep only the latest potential
+        return cached
+
+    def apply_local(self, coeff: torch.Tensor, V, with_ke: bool, out: Optional[torch.Tensor] = None,
+                    accumulate: bool = False) -> torch.Tensor:
+        """out = [KE C] + V C on (S, K, B, X, G) coefficients via qmx_apply_local."""
+        if coeff.dtype not in DTYPE_CODE:
+            raise TypeError(f"unsupported wavefunction dtype {coeff.dtype}")
+        S, K, B, X, G = coeff.shape
+        if K != self.nk or G != self.n_tot or X != 1:
+            raise ValueError(f"coefficients {tuple(coeff.shape)} do not match the basis")
+        if coeff.stride(-1) != 1 or coeff.stride(2) < G or (K > 1 and coeff.stride(1) < G):
+            coeff = coeff.contiguous()
+        if out is None:
+            out = torch.empty((S, K, B, X, G), dtype=coeff.dtype, device=coeff.device)
+        Vbox = self.potential_box(V, coeff.dtype)
+        ke = self.get_ke(dtype=REAL_OF[coeff.dtype]) if with_ke else None
+        ws, ws_bytes = self.workspace(coeff.dtype, B)
+        code = DTYPE_CODE[coeff.dtype]
+        stream = rc.stream_ptr()
+        esize = coeff.element_size()
+        for s in range(S):
+            cs, os_ = coeff[s], out[s]
+            _lib.call(
+                "qmx_apply_local", self._plan, code, cs.data_ptr(), cs.stride(1), cs.stride(0),
+                os_.data_ptr(), os_.stride(1), os_.stride(0), Vbox.data_ptr(),
+                ke.data_ptr() if ke is not None else None, ke.stride(0) if ke is not None else 0,
+                B, int(accumulate), ws.data_ptr(), ws_bytes, stream,
+            )
+        del esize
+        return out
+
+    def apply_ke(self, C):
+        from .wavefunction import Wavefunction
+
+        coeff = C.coeff * self.get_ke(dtype=REAL_OF[C.coeff.dtype])[None, :, None, None, :]
+        return Wavefunction(self, coeff=coeff, band_division=C.band_division)
+
+    def apply_potential(self, V, C):
+        from .wavefunction import Wavefunction
+
+        out = self.apply_local(C.coeff, V, with_ke=False)
+        return Wavefunction(self, coeff=out, band_division=C.band_division)
+
+    def apply_ke_potential(self, V, C):
+        """Fused T C + V C (one pass over the sphere, KE folded into the gather)."""
+        from .wavefunction import Wavefunction
+
+        out = self.apply_local(C.coeff, V, with_ke=True)
+        return Wavefunction(self, coeff=out, band_division=C.band_division)
+
+    @classmethod
+    def from_spec(cls, spec, ions=None) -> "Basis":
+        return cls(Rbasis=spec.Rbasis, k=spec.k, wk=spec.wk, ke_cutoff=spec.ke_cutoff,
+                   shape=spec.shape, ions=ions)

@@ -1,0 +1,859 @@
+// Sphere-sparse local-potential apply and dense 3D FFT for gfx950.
+//
+// The plane-wave sphere occupies ~1/15 of the FFT box (4x Ecut density
+// box), so the apply never materialises a full box. With the box (n0, n1,
+// n2) C-ordered (x, y, z; z fastest), one band goes through five LDS-staged
+// passes, each a batch of 1D Stockham transforms:
+//   A  sphere -> T1[col][z]      scatter in-sphere points of each active
+//                                (x, y) column, inverse z-FFT
+//   B  T1 -> T2[xa][y][z]        inverse y-FFT on active x-planes only
+//   C  T2 -> T2 (in place)       inverse x-FFT, x V(r), forward x-FFT,
+//                                keep the active x-planes only
+//   D  T2 -> T1                  forward y-FFT, keep the active columns
+//   E  T1 -> sphere              forward z-FFT, gather the sphere, x 1/N,
+//                                + KE C, zero the padding
+// Intermediates are read and written coalesced: z is the unit-stride axis
+// of every tile. Bands are processed in blocks sized to the workspace.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qmx_common.h"
+#include "qmx_fft_core.h"
+
+namespace qmx {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------ profiling
+// When enabled, every kernel launch of the library is bracketed by a pair of
+// HIP events on its own stream and tagged with a kernel id and the
+// compulsory bytes it moves; qmx_profile_read() resolves the events.
+struct ProfRec {
+  int id;
+  hipEvent_t a, b;
+  double bytes;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+bool prof_on() { return g_prof_on; }
+
+void prof_begin(hipStream_t st, hipEvent_t* a) {
+  *a = nullptr;
+  if (!g_prof_on) return;
+  if (hipEventCreate(a) != hipSuccess || hipEventRecord(*a, st) != hipSuccess) *a = nullptr;
+}
+
+void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes) {
+  if (!a) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, st) != hipSuccess) return;
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof.push_back({id, a, b, bytes});
+}
+
+bool factorize(int n, Radices* out) {
+  if (n < 1) return false;
+  out->n = n;
+  out->nr = 0;
+  int m = n, ns = 1;
+  const int order[4] = {4, 2, 3, 5};
+  for (int r : order) {
+    while (m % r == 0) {
+      if (out->nr >= kMaxRadices) return false;
+      out->r[out->nr] = r;
+      out->ns[out->nr] = make_fastdiv(ns);
+      ++out->nr;
+      ns *= r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+constexpr int kThreads = 256;
+constexpr int kLdsBudget = 60 * 1024;
+
+// Largest line count M for lines of length n with two ping-pong buffers,
+// the twiddle table and `extra_per_line` bytes per line within the budget.
+static int max_lines(int n, int csize, int extra_per_line, int cap) {
+  int m = cap;
+  while (m > 1 && (2 * (m + 1) * n * csize + n * csize + m * extra_per_line) > kLdsBudget) --m;
+  return m;
+}
+
+// Split `extent` into the fewest tiles of at most `mmax` and return the tile.
+static int even_tile(int extent, int mmax) {
+  int ntile = (extent + mmax - 1) / mmax;
+  return (extent + ntile - 1) / ntile;
+}
+
+__host__ __device__ inline int odd_ld(int m) { return m | 1; }
+
+}  // namespace qmx
+
+using namespace qmx;
+
+struct qmx_plan {
+  int device;
+  int shape[3];
+  int nk, n_tot;
+  int64_t N;
+  int nxa, ncol;
+  int64_t n_sum;  // sum of n[k]
+  Radices rad[3];
+  int* d_x_list;      // [nxa] box x of each active plane
+  int* d_pcol_start;  // [nxa + 1] first column of each active plane
+  int* d_col_iy;      // [ncol] y of each column
+  int* d_cstart;      // [nk][ncol + 1] first sphere point of each column
+  int* d_t1idx;       // [nk][n_tot] col * n2 + z of each sphere point
+  int* d_n;           // [nk]
+  float2* d_tw32[3];
+  double2* d_tw64[3];
+};
+
+namespace {
+
+// ---------------------------------------------------------------- pass A
+// One workgroup: M consecutive columns of one band of one k.
+template <typename C2, typename R>
+__global__ void __launch_bounds__(kThreads)
+    k_sphere_to_cols(const C2* __restrict__ C, int64_t ldb, int64_t ldk,
+                     C2* __restrict__ T1, const int* __restrict__ t1idx,
+                     const int* __restrict__ cstart, const C2* __restrict__ tw_g, Radices rz,
+                     FastDiv fm, int ncol, int nb, int n_tot, int b0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nz = rz.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + nz;
+  C2* b = a + nz * ld;
+  const int col0 = blockIdx.x * M, ib = blockIdx.y, k = blockIdx.z;
+  const int mcols = min(M, ncol - col0);
+  load_twiddles(tw, tw_g, nz);
+  zero_lds(a, nz * ld);
+  __syncthreads();
+  const int* cs = cstart + (int64_t)k * (ncol + 1);
+  const int gs = cs[col0], ge = cs[col0 + mcols];
+  const C2* Cb = C + k * ldk + (int64_t)(b0 + ib) * ldb;
+  const int* ti = t1idx + (int64_t)k * n_tot;
+  const int base = col0 * nz;
+  for (int g = gs + threadIdx.x; g < ge; g += blockDim.x) {
+    const int t = ti[g] - base;
+    const int l = t / nz;
+    a[(t - l * nz) * ld + l] = Cb[g];
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, +1>(a, b, tw, rz, fm, ld);
+  C2* out = T1 + ((int64_t)k * nb + ib) * ncol * nz + (int64_t)col0 * nz;
+  for (int i = threadIdx.x; i < mcols * nz; i += blockDim.x) {
+    const int l = i / nz;
+    const int p = i - l * nz;
+    out[i] = res[p * ld + l];
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+// One workgroup: one active x-plane, one z-tile, one (k, band).
+template <typename C2, typename R>
+__global__ void __launch_bounds__(kThreads)
+    k_cols_to_planes(const C2* __restrict__ T1, C2* __restrict__ T2,
+                     const int* __restrict__ pcol_start, const int* __restrict__ col_iy,
+                     const C2* __restrict__ tw_g, Radices ry, FastDiv fm, int ncol, int nxa,
+                     int nz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ny = ry.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + ny;
+  C2* b = a + ny * ld;
+  const int z0 = blockIdx.x * M, xa = blockIdx.y;
+  const int64_t bk = blockIdx.z;
+  const int mz = min(M, nz - z0);
+  load_twiddles(tw, tw_g, ny);
+  zero_lds(a, ny * ld);
+  __syncthreads();
+  const int c0 = pcol_start[xa], nc = pcol_start[xa + 1] - c0;
+  const C2* src = T1 + (bk * ncol + c0) * nz + z0;
+  for (int i = threadIdx.x; i < nc * M; i += blockDim.x) {
+    const int c = fdiv(i, fm);
+    const int l = i - c * M;
+    if (l < mz) a[col_iy[c0 + c] * ld + l] = src[(int64_t)c * nz + l];
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, +1>(a, b, tw, ry, fm, ld);
+  C2* dst = T2 + (bk * nxa + xa) * (int64_t)ny * nz + z0;
+  for (int i = threadIdx.x; i < ny * M; i += blockDim.x) {
+    const int y = fdiv(i, fm);
+    const int l = i - y * M;
+    if (l < mz) dst[(int64_t)y * nz + l] = res[y * ld + l];
+  }
+}
+
+// ---------------------------------------------------------------- pass C
+// One workgroup: one y, one z-tile, one k, a chunk of bands (V reused).
+template <typename C2, typename R>
+__global__ void __launch_bounds__(kThreads)
+    k_xpass_potential(C2* __restrict__ T2, const R* __restrict__ V,
+                      const int* __restrict__ x_list, const C2* __restrict__ tw_g, Radices rx,
+                      FastDiv fm, int ny, int nz, int nxa, int nb, int bands_per_chunk,
+                      int n_chunks) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nx = rx.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + nx;
+  C2* b = a + nx * ld;
+  R* vt = reinterpret_cast<R*>(b + nx * ld);
+  const int z0 = blockIdx.x * M, y = blockIdx.y;
+  const int k = blockIdx.z / n_chunks, chunk = blockIdx.z - k * n_chunks;
+  const int mz = min(M, nz - z0);
+  const int bstart = chunk * bands_per_chunk;
+  const int bstop = min(nb, bstart + bands_per_chunk);
+  load_twiddles(tw, tw_g, nx);
+  for (int i = threadIdx.x; i < nx * M; i += blockDim.x) {
+    const int x = fdiv(i, fm);
+    const int l = i - x * M;
+    vt[i] = l < mz ? V[((int64_t)x * ny + y) * nz + z0 + l] : R(0);
+  }
+  const int64_t plane = (int64_t)ny * nz;
+  for (int ib = bstart; ib < bstop; ++ib) {
+    zero_lds(a, nx * ld);
+    __syncthreads();
+    C2* base = T2 + ((int64_t)k * nb + ib) * nxa * plane + (int64_t)y * nz + z0;
+    for (int i = threadIdx.x; i < nxa * M; i += blockDim.x) {
+      const int xa = fdiv(i, fm);
+      const int l = i - xa * M;
+      if (l < mz) a[x_list[xa] * ld + l] = base[xa * plane + l];
+    }
+    __syncthreads();
+    C2* r1 = fft_lds<C2, +1>(a, b, tw, rx, fm, ld);
+    C2* other = (r1 == a) ? b : a;
+    for (int i = threadIdx.x; i < nx * M; i += blockDim.x) {
+      const int x = fdiv(i, fm);
+      const int l = i - x * M;
+      r1[x * ld + l] = cscale(r1[x * ld + l], vt[i]);
+    }
+    __syncthreads();
+    C2* r2 = fft_lds<C2, -1>(r1, other, tw, rx, fm, ld);
+    for (int i = threadIdx.x; i < nxa * M; i += blockDim.x) {
+      const int xa = fdiv(i, fm);
+      const int l = i - xa * M;
+      if (l < mz) base[xa * plane + l] = r2[x_list[xa] * ld + l];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- pass D
+template <typename C2, typename R>
+__global__ void __launch_bounds__(kThreads)
+    k_planes_to_cols(const C2* __restrict__ T2, C2* __restrict__ T1,
+                     const int* __restrict__ pcol_start, const int* __restrict__ col_iy,
+                     const C2* __restrict__ tw_g, Radices ry, FastDiv fm, int ncol, int nxa,
+                     int nz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ny = ry.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + ny;
+  C2* b = a + ny * ld;
+  const int z0 = blockIdx.x * M, xa = blockIdx.y;
+  const int64_t bk = blockIdx.z;
+  const int mz = min(M, nz - z0);
+  load_twiddles(tw, tw_g, ny);
+  const C2* src = T2 + (bk * nxa + xa) * (int64_t)ny * nz + z0;
+  for (int i = threadIdx.x; i < ny * M; i += blockDim.x) {
+    const int y = fdiv(i, fm);
+    const int l = i - y * M;
+    a[y * ld + l] = l < mz ? src[(int64_t)y * nz + l] : C2{0, 0};
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, -1>(a, b, tw, ry, fm, ld);
+  const int c0 = pcol_start[xa], nc = pcol_start[xa + 1] - c0;
+  C2* dst = T1 + (bk * ncol + c0) * nz + z0;
+  for (int i = threadIdx.x; i < nc * M; i += blockDim.x) {
+    const int c = fdiv(i, fm);
+    const int l = i - c * M;
+    if (l < mz) dst[(int64_t)c * nz + l] = res[col_iy[c0 + c] * ld + l];
+  }
+}
+
+// ---------------------------------------------------------------- pass E
+template <typename C2, typename R>
+__global__ void __launch_bounds__(kThreads)
+    k_cols_to_sphere(const C2* __restrict__ T1, const C2* C, int64_t ldb, int64_t ldk,
+                     C2* out, int64_t oldb, int64_t oldk, const R* __restrict__ ke,
+                     int64_t ke_ldk, const int* __restrict__ t1idx,
+                     const int* __restrict__ cstart, const int* __restrict__ nvec,
+                     const C2* __restrict__ tw_g, Radices rz, FastDiv fm, int ncol, int nb,
+                     int n_tot, int b0, R scale, int accumulate) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nz = rz.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + nz;
+  C2* b = a + nz * ld;
+  const int col0 = blockIdx.x * M, ib = blockIdx.y, k = blockIdx.z;
+  const int mcols = min(M, ncol - col0);
+  load_twiddles(tw, tw_g, nz);
+  const C2* src = T1 + ((int64_t)k * nb + ib) * ncol * nz + (int64_t)col0 * nz;
+  for (int i = threadIdx.x; i < M * nz; i += blockDim.x) {
+    const int l = i / nz;
+    const int p = i - l * nz;
+    a[p * ld + l] = l < mcols ? src[i] : C2{0, 0};
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, -1>(a, b, tw, rz, fm, ld);
+  const int* cs = cstart + (int64_t)k * (ncol + 1);
+  const int gs = cs[col0], ge = cs[col0 + mcols];
+  const int64_t band = b0 + ib;
+  const C2* Cb = C + k * ldk + band * ldb;
+  C2* ob = out + k * oldk + band * oldb;
+  const R* keb = ke ? ke + k * ke_ldk : nullptr;
+  const int* ti = t1idx + (int64_t)k * n_tot;
+  const int base = col0 * nz;
+  for (int g = gs + threadIdx.x; g < ge; g += blockDim.x) {
+    const int t = ti[g] - base;
+    const int l = t / nz;
+    C2 v = cscale(res[(t - l * nz) * ld + l], scale);
+    if (keb) {
+      const C2 c = Cb[g];
+      const R e = keb[g];
+      v.x += e * c.x;
+      v.y += e * c.y;
+    }
+    if (accumulate) v = cadd(v, ob[g]);
+    ob[g] = v;
+  }
+  if (blockIdx.x == 0) {
+    for (int g = nvec[k] + threadIdx.x; g < n_tot; g += blockDim.x) ob[g] = C2{0, 0};
+  }
+}
+
+// ------------------------------------------------------------ dense FFT
+// Lines along the contiguous last axis: M lines of length n per workgroup.
+template <typename C2, typename R, int S>
+__global__ void __launch_bounds__(kThreads)
+    k_fft_contig(C2* __restrict__ data, int64_t n_lines, const C2* __restrict__ tw_g,
+                 Radices rad, FastDiv fm, R scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = rad.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + n;
+  C2* b = a + n * ld;
+  const int64_t line0 = (int64_t)blockIdx.x * M;
+  const int m = (int)min<int64_t>(M, n_lines - line0);
+  load_twiddles(tw, tw_g, n);
+  C2* base = data + line0 * n;
+  for (int i = threadIdx.x; i < M * n; i += blockDim.x) {
+    const int l = i / n;
+    const int p = i - l * n;
+    a[p * ld + l] = l < m ? base[i] : C2{0, 0};
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, S>(a, b, tw, rad, fm, ld);
+  for (int i = threadIdx.x; i < m * n; i += blockDim.x) {
+    const int l = i / n;
+    const int p = i - l * n;
+    base[i] = cscale(res[p * ld + l], scale);
+  }
+}
+
+// Lines along a strided axis, tiled over the contiguous last axis (length
+// n_inner). Line o starts at (o / cnt) * s_hi + (o % cnt) * s_lo.
+template <typename C2, typename R, int S>
+__global__ void __launch_bounds__(kThreads)
+    k_fft_strided(C2* __restrict__ data, int64_t line_stride, int n_inner, int ntile, int cnt,
+                  int64_t s_hi, int64_t s_lo, const C2* __restrict__ tw_g, Radices rad,
+                  FastDiv fm, R scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = rad.n, M = fm.d, ld = odd_ld(M);
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* a = tw + n;
+  C2* b = a + n * ld;
+  const int64_t o = blockIdx.x / ntile;
+  const int z0 = (int)(blockIdx.x - o * ntile) * M;
+  const int mz = min(M, n_inner - z0);
+  C2* base = data + (o / cnt) * s_hi + (o % cnt) * s_lo + z0;
+  load_twiddles(tw, tw_g, n);
+  for (int i = threadIdx.x; i < n * M; i += blockDim.x) {
+    const int p = fdiv(i, fm);
+    const int l = i - p * M;
+    a[p * ld + l] = l < mz ? base[p * line_stride + l] : C2{0, 0};
+  }
+  __syncthreads();
+  C2* res = fft_lds<C2, S>(a, b, tw, rad, fm, ld);
+  for (int i = threadIdx.x; i < n * M; i += blockDim.x) {
+    const int p = fdiv(i, fm);
+    const int l = i - p * M;
+    if (l < mz) base[p * line_stride + l] = cscale(res[p * ld + l], scale);
+  }
+}
+
+// Twiddle tables e^(-2 pi i t/n) in double, rounded once to float for c64.
+template <typename C2>
+static std::vector<C2> make_twiddles(int n) {
+  std::vector<C2> out(n);
+  for (int t = 0; t < n; ++t) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * t / n;
+    out[t].x = (decltype(out[t].x))std::cos(ang);
+    out[t].y = (decltype(out[t].x))std::sin(ang);
+  }
+  return out;
+}
+
+struct TwCache {
+  std::mutex mu;
+  // (device, n) -> tables; small and kept for the process lifetime.
+  std::vector<std::pair<std::pair<int, int>, std::pair<float2*, double2*>>> entries;
+};
+TwCache g_tw;
+
+static int get_twiddles(int n, float2** t32, double2** t64) {
+  int dev = 0;
+  QMX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tw.mu);
+  for (auto& e : g_tw.entries) {
+    if (e.first.first == dev && e.first.second == n) {
+      *t32 = e.second.first;
+      *t64 = e.second.second;
+      return QMX_OK;
+    }
+  }
+  auto h32 = make_twiddles<float2>(n);
+  auto h64 = make_twiddles<double2>(n);
+  float2* d32 = nullptr;
+  double2* d64 = nullptr;
+  QMX_HIP(hipMalloc(&d32, sizeof(float2) * n));
+  QMX_HIP(hipMalloc(&d64, sizeof(double2) * n));
+  QMX_HIP(hipMemcpy(d32, h32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  QMX_HIP(hipMemcpy(d64, h64.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  g_tw.entries.push_back({{dev, n}, {d32, d64}});
+  *t32 = d32;
+  *t64 = d64;
+  return QMX_OK;
+}
+
+template <typename C2>
+const C2* tw_of(const qmx_plan* p, int axis);
+template <>
+const float2* tw_of<float2>(const qmx_plan* p, int axis) {
+  return p->d_tw32[axis];
+}
+template <>
+const double2* tw_of<double2>(const qmx_plan* p, int axis) {
+  return p->d_tw64[axis];
+}
+
+template <typename R>
+int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk, void* outv,
+                     int64_t oldb, int64_t oldk, const void* Vv, const void* kev,
+                     int64_t ke_ldk, int64_t nb, int accumulate, void* ws, int64_t ws_bytes,
+                     hipStream_t st) {
+  using C2 = typename Cplx<R>::T;
+  const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
+  const int nk = p->nk, ncol = p->ncol, nxa = p->nxa;
+  const int64_t per_band = ((int64_t)ncol * nz + (int64_t)nxa * ny * nz) * sizeof(C2);
+  int64_t blk = ws_bytes / (per_band * nk);
+  QMX_CHECK_ARG(blk >= 1, "qmx_apply_local: workspace smaller than one band of every k");
+  blk = std::min<int64_t>(blk, nb);
+  blk = std::min<int64_t>(blk, 65535 / nk);  // grid.z of passes B and D is nk * blk
+  QMX_CHECK_ARG(blk >= 1, "qmx_apply_local: too many k-points for one launch");
+  const C2* C = static_cast<const C2*>(Cv);
+  C2* out = static_cast<C2*>(outv);
+  const R* V = static_cast<const R*>(Vv);
+  const R* ke = static_cast<const R*>(kev);
+  const int cs = sizeof(C2);
+  // Tile sizes under the LDS budget:
+  const int Mz = max_lines(nz, cs, 0, 64);
+  const int My = even_tile(nz, max_lines(ny, cs, 0, 64));
+  const int Mx = even_tile(nz, max_lines(nx, cs, nx * (int)sizeof(R), 64));
+  const FastDiv fz = make_fastdiv(Mz), fy = make_fastdiv(My), fx = make_fastdiv(Mx);
+  const size_t shA = (size_t)(2 * odd_ld(Mz) * nz + nz) * cs;
+  const size_t shB = (size_t)(2 * odd_ld(My) * ny + ny) * cs;
+  const size_t shC = (size_t)(2 * odd_ld(Mx) * nx + nx) * cs + (size_t)nx * Mx * sizeof(R);
+  const R scale = R(1) / R(p->N);
+  // Sphere points per band summed over k (for the byte counts):
+  const double n_sphere = (double)p->n_sum;
+  const double bc = (double)cs, br = (double)sizeof(R);
+  const double t1_elems = (double)nk * ncol * nz, t2_elems = (double)nk * nxa * ny * nz;
+  hipEvent_t ev = nullptr;
+  for (int64_t b0 = 0; b0 < nb; b0 += blk) {
+    const int cur = (int)std::min<int64_t>(blk, nb - b0);
+    C2* T1 = static_cast<C2*>(ws);
+    C2* T2 = T1 + (int64_t)nk * cur * ncol * nz;
+    dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_sphere_to_cols<C2, R>), gA, dim3(kThreads), shA, st, C, ldb, ldk, T1,
+                       p->d_t1idx, p->d_cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                       p->n_tot, (int)b0);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_PASS_A, cur * (n_sphere * bc + t1_elems * bc));
+    dim3 gB((nz + My - 1) / My, nxa, nk * cur);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_cols_to_planes<C2, R>), gB, dim3(kThreads), shB, st, T1, T2,
+                       p->d_pcol_start, p->d_col_iy, tw_of<C2>(p, 1), p->rad[1], fy, ncol,
+                       nxa, nz);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_PASS_B, cur * (t1_elems + t2_elems) * bc);
+    // Bands per chunk: enough workgroups to fill the chip, V reused within.
+    const int64_t wg_base = (int64_t)((nz + Mx - 1) / Mx) * ny * nk;
+    int n_chunks = (int)std::min<int64_t>(cur, std::max<int64_t>(1, (2048 + wg_base - 1) / wg_base));
+    const int bpc = (cur + n_chunks - 1) / n_chunks;
+    n_chunks = (cur + bpc - 1) / bpc;
+    dim3 gC((nz + Mx - 1) / Mx, ny, nk * n_chunks);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_xpass_potential<C2, R>), gC, dim3(kThreads), shC, st, T2, V,
+                       p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fx, ny, nz, nxa, cur, bpc,
+                       n_chunks);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_PASS_C,
+             cur * 2.0 * t2_elems * bc + (double)nk * n_chunks * (double)p->N * br);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_planes_to_cols<C2, R>), gB, dim3(kThreads), shB, st, T2, T1,
+                       p->d_pcol_start, p->d_col_iy, tw_of<C2>(p, 1), p->rad[1], fy, ncol,
+                       nxa, nz);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_cols_to_sphere<C2, R>), gA, dim3(kThreads), shA, st, T1, C, ldb, ldk,
+                       out, oldb, oldk, ke, ke_ldk, p->d_t1idx, p->d_cstart, p->d_n,
+                       tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
+                       accumulate);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_PASS_E,
+             cur * (t1_elems * bc + n_sphere * bc * (1 + (ke ? 1 : 0) + (accumulate ? 1 : 0)) +
+                    (ke ? n_sphere * br : 0.0)));
+  }
+  return QMX_OK;
+}
+
+template <typename R>
+int fft3d_impl(const int32_t* shape, void* datav, int64_t batch, int direction, double scale,
+               hipStream_t st) {
+  using C2 = typename Cplx<R>::T;
+  C2* data = static_cast<C2*>(datav);
+  const int n0 = shape[0], n1 = shape[1], n2 = shape[2];
+  const int cs = sizeof(C2);
+  for (int axis = 2; axis >= 0; --axis) {
+    const int n = shape[axis];
+    Radices rad;
+    QMX_CHECK_ARG(factorize(n, &rad), "qmx_fft3d: box size must be a product of 2, 3 and 5");
+    float2* t32;
+    double2* t64;
+    int rc = get_twiddles(n, &t32, &t64);
+    if (rc) return rc;
+    const C2* tw = reinterpret_cast<const C2*>(sizeof(R) == 4 ? (const void*)t32 : (const void*)t64);
+    const R sc = axis == 0 ? R(scale) : R(1);
+    if (axis == 2) {
+      const int M = max_lines(n, cs, 0, 64);
+      const FastDiv fm = make_fastdiv(M);
+      const int64_t n_lines = batch * n0 * n1;
+      const size_t sh = (size_t)(2 * odd_ld(M) * n + n) * cs;
+      const int64_t nblk = (n_lines + M - 1) / M;
+      QMX_CHECK_ARG(nblk < (1ll << 31), "qmx_fft3d: batch too large");
+      if (direction < 0)
+        hipLaunchKernelGGL((k_fft_contig<C2, R, -1>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
+                           data, n_lines, tw, rad, fm, sc);
+      else
+        hipLaunchKernelGGL((k_fft_contig<C2, R, +1>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
+                           data, n_lines, tw, rad, fm, sc);
+    } else {
+      const int M = even_tile(n2, max_lines(n, cs, 0, 64));
+      const FastDiv fm = make_fastdiv(M);
+      const int ntile = (n2 + M - 1) / M;
+      int64_t n_outer, s_hi, s_lo, line_stride;
+      int cnt;
+      if (axis == 1) {
+        n_outer = batch * n0;
+        cnt = 1;
+        s_hi = (int64_t)n1 * n2;
+        s_lo = 0;
+        line_stride = n2;
+      } else {
+        n_outer = batch * n1;
+        cnt = n1;
+        s_hi = (int64_t)n0 * n1 * n2;
+        s_lo = n2;
+        line_stride = (int64_t)n1 * n2;
+      }
+      const size_t sh = (size_t)(2 * odd_ld(M) * n + n) * cs;
+      const int64_t nblk = n_outer * ntile;
+      QMX_CHECK_ARG(nblk < (1ll << 31), "qmx_fft3d: batch too large");
+      if (direction < 0)
+        hipLaunchKernelGGL((k_fft_strided<C2, R, -1>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
+                           data, line_stride, n2, ntile, cnt, s_hi, s_lo, tw, rad, fm, sc);
+      else
+        hipLaunchKernelGGL((k_fft_strided<C2, R, +1>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
+                           data, line_stride, n2, ntile, cnt, s_hi, s_lo, tw, rad, fm, sc);
+    }
+    QMX_LAUNCHED();
+  }
+  return QMX_OK;
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src) {
+  *dst = nullptr;
+  if (src.empty()) return QMX_OK;
+  QMX_HIP(hipMalloc(dst, sizeof(T) * src.size()));
+  QMX_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return QMX_OK;
+}
+
+static void free_plan(qmx_plan* p) {
+  if (!p) return;
+  int* arrs[] = {p->d_x_list, p->d_pcol_start, p->d_col_iy, p->d_cstart, p->d_t1idx, p->d_n};
+  for (int* a : arrs)
+    if (a) (void)hipFree(a);
+  delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qmx_last_error(void) { return g_last_error.c_str(); }
+const char* qmx_version(void) { return "qmx 0.1 gfx950"; }
+
+}  // extern "C"
+
+namespace {
+
+// Host-side index tables of a basis (no device calls).
+struct GeomTables {
+  std::vector<int> x_list, pcol_start, col_iy, cstart, t1idx;
+};
+
+int build_tables(const qmx_geom* geom, GeomTables* t) {
+  QMX_CHECK_ARG(geom, "qmx_geom_tables: null geometry");
+  const int n0 = geom->shape[0], n1 = geom->shape[1], n2 = geom->shape[2];
+  const int nk = geom->nk, n_tot = geom->n_tot;
+  QMX_CHECK_ARG(n0 > 0 && n1 > 0 && n2 > 0 && n0 <= 512 && n1 <= 512 && n2 <= 512,
+                "qmx_geom_tables: box sizes must be in 1..512");
+  QMX_CHECK_ARG(nk >= 1 && nk < 65536 && n_tot >= 1, "qmx_geom_tables: bad nk / n_tot");
+  QMX_CHECK_ARG(geom->n && geom->fft_index, "qmx_geom_tables: null n / fft_index");
+  const int64_t N = (int64_t)n0 * n1 * n2;
+  QMX_CHECK_ARG(N < (1ll << 31), "qmx_geom_tables: box too large");
+  for (int a = 0; a < 3; ++a) {
+    Radices rad;
+    QMX_CHECK_ARG(factorize(geom->shape[a], &rad),
+                  "qmx_geom_tables: box sizes must be products of 2, 3 and 5");
+  }
+  // Columns and planes touched by any k:
+  std::vector<char> col_seen((size_t)n0 * n1, 0), x_seen(n0, 0);
+  for (int k = 0; k < nk; ++k) {
+    const int nkk = geom->n[k];
+    QMX_CHECK_ARG(nkk >= 0 && nkk <= n_tot, "qmx_geom_tables: n[k] outside 0..n_tot");
+    const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
+    for (int g = 0; g < nkk; ++g) {
+      const int64_t idx = fi[g];
+      QMX_CHECK_ARG(idx >= 0 && idx < N && (g == 0 || idx > fi[g - 1]),
+                    "qmx_geom_tables: fft_index must be increasing box indices within each k");
+      const int64_t col = idx / n2;
+      col_seen[col] = 1;
+      x_seen[col / n1] = 1;
+    }
+  }
+  std::vector<int> col_of((size_t)n0 * n1, -1);
+  t->x_list.clear();
+  t->pcol_start.clear();
+  t->col_iy.clear();
+  for (int x = 0; x < n0; ++x) {
+    if (!x_seen[x]) continue;
+    t->x_list.push_back(x);
+    t->pcol_start.push_back((int)t->col_iy.size());
+    for (int y = 0; y < n1; ++y) {
+      const int64_t c = (int64_t)x * n1 + y;
+      if (col_seen[c]) {
+        col_of[c] = (int)t->col_iy.size();
+        t->col_iy.push_back(y);
+      }
+    }
+  }
+  t->pcol_start.push_back((int)t->col_iy.size());
+  const int ncol = (int)t->col_iy.size();
+  t->cstart.assign((size_t)nk * (ncol + 1), 0);
+  t->t1idx.assign((size_t)nk * n_tot, 0);
+  for (int k = 0; k < nk; ++k) {
+    const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
+    int* cs = t->cstart.data() + (size_t)k * (ncol + 1);
+    std::vector<int> counts(ncol, 0);
+    for (int g = 0; g < geom->n[k]; ++g) {
+      const int64_t idx = fi[g];
+      const int c = col_of[idx / n2];
+      counts[c]++;
+      t->t1idx[(size_t)k * n_tot + g] = c * n2 + (int)(idx % n2);
+    }
+    cs[0] = 0;
+    for (int c = 0; c < ncol; ++c) cs[c + 1] = cs[c] + counts[c];
+  }
+  return QMX_OK;
+}
+
+template <typename T>
+void copy_out(T* dst, const std::vector<T>& src) {
+  if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(T) * src.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmx_geom_tables(const qmx_geom* geom, int64_t* sizes, int32_t* x_list,
+                    int32_t* pcol_start, int32_t* col_iy, int32_t* cstart, int32_t* t1idx) {
+  QMX_CHECK_ARG(sizes, "qmx_geom_tables: null sizes");
+  GeomTables t;
+  int rc = build_tables(geom, &t);
+  if (rc) return rc;
+  sizes[0] = (int64_t)t.x_list.size();
+  sizes[1] = (int64_t)t.col_iy.size();
+  copy_out(x_list, t.x_list);
+  copy_out(pcol_start, t.pcol_start);
+  copy_out(col_iy, t.col_iy);
+  copy_out(cstart, t.cstart);
+  copy_out(t1idx, t.t1idx);
+  return QMX_OK;
+}
+
+int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
+  QMX_CHECK_ARG(geom && plan_out, "qmx_plan_create: null argument");
+  *plan_out = nullptr;
+  GeomTables t;
+  int rc = build_tables(geom, &t);
+  if (rc) return rc;
+  auto* p = new qmx_plan();
+  std::memset(p, 0, sizeof(qmx_plan));
+  for (int a = 0; a < 3; ++a) {
+    p->shape[a] = geom->shape[a];
+    factorize(p->shape[a], &p->rad[a]);
+  }
+  p->nk = geom->nk;
+  p->n_tot = geom->n_tot;
+  p->N = (int64_t)p->shape[0] * p->shape[1] * p->shape[2];
+  p->nxa = (int)t.x_list.size();
+  p->n_sum = 0;
+  for (int k = 0; k < geom->nk; ++k) p->n_sum += geom->n[k];
+  p->ncol = (int)t.col_iy.size();
+  std::vector<int> nvec(geom->n, geom->n + geom->nk);
+  if ((rc = upload(&p->d_x_list, t.x_list)) || (rc = upload(&p->d_pcol_start, t.pcol_start)) ||
+      (rc = upload(&p->d_col_iy, t.col_iy)) || (rc = upload(&p->d_cstart, t.cstart)) ||
+      (rc = upload(&p->d_t1idx, t.t1idx)) || (rc = upload(&p->d_n, nvec))) {
+    free_plan(p);
+    return rc;
+  }
+  hipError_t e = hipGetDevice(&p->device);
+  if (e != hipSuccess) {
+    free_plan(p);
+    set_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return QMX_ERR_HIP;
+  }
+  for (int a = 0; a < 3; ++a) {
+    if ((rc = get_twiddles(p->shape[a], &p->d_tw32[a], &p->d_tw64[a]))) {
+      free_plan(p);
+      return rc;
+    }
+  }
+  *plan_out = p;
+  return QMX_OK;
+}
+
+int qmx_plan_destroy(qmx_plan* plan) {
+  free_plan(plan);
+  return QMX_OK;
+}
+
+int64_t qmx_plan_workspace_bytes(const qmx_plan* p, int dtype, int64_t n_bands) {
+  if (!p || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
+  const int64_t cs = dtype == QMX_C64 ? 8 : 16;
+  const int64_t nz = p->shape[2], ny = p->shape[1];
+  return ((int64_t)p->ncol * nz + (int64_t)p->nxa * ny * nz) * cs * p->nk * n_bands;
+}
+
+int qmx_plan_info(const qmx_plan* p, int64_t* info) {
+  QMX_CHECK_ARG(p && info, "qmx_plan_info: null argument");
+  info[0] = p->nxa;
+  info[1] = p->ncol;
+  return QMX_OK;
+}
+
+int qmx_apply_local(const qmx_plan* plan, int dtype, const void* C, int64_t ldb, int64_t ldk,
+                    void* out, int64_t out_ldb, int64_t out_ldk, const void* V, const void* ke,
+                    int64_t ke_ldk, int64_t n_bands, int accumulate, void* workspace,
+                    int64_t workspace_bytes, void* stream) {
+  QMX_CHECK_ARG(plan && C && out && V, "qmx_apply_local: null argument");
+  QMX_CHECK_ARG(n_bands >= 0, "qmx_apply_local: negative n_bands");
+  if (n_bands == 0) return QMX_OK;
+  QMX_CHECK_ARG(workspace && workspace_bytes > 0, "qmx_apply_local: workspace required");
+  QMX_CHECK_ARG(ldb >= plan->n_tot && out_ldb >= plan->n_tot, "qmx_apply_local: ldb < n_tot");
+  QMX_CHECK_ARG(plan->nk == 1 || (ldk > 0 && out_ldk > 0), "qmx_apply_local: bad k stride");
+  QMX_CHECK_ARG(!ke || plan->nk == 1 || ke_ldk >= plan->n_tot, "qmx_apply_local: bad ke stride");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == QMX_C64)
+    return apply_local_impl<float>(plan, C, ldb, ldk, out, out_ldb, out_ldk, V, ke, ke_ldk,
+                                   n_bands, accumulate, workspace, workspace_bytes, st);
+  if (dtype == QMX_C128)
+    return apply_local_impl<double>(plan, C, ldb, ldk, out, out_ldb, out_ldk, V, ke, ke_ldk,
+                                    n_bands, accumulate, workspace, workspace_bytes, st);
+  set_error("qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
+  return QMX_ERR_ARG;
+}
+
+int qmx_fft3d(int dtype, const int32_t* shape, void* data, int64_t batch, int direction,
+              double scale, void* stream) {
+  QMX_CHECK_ARG(shape && data, "qmx_fft3d: null argument");
+  QMX_CHECK_ARG(direction == 1 || direction == -1, "qmx_fft3d: direction must be +-1");
+  QMX_CHECK_ARG(batch >= 0, "qmx_fft3d: negative batch");
+  for (int a = 0; a < 3; ++a)
+    QMX_CHECK_ARG(shape[a] >= 1 && shape[a] <= 512, "qmx_fft3d: box sizes must be in 1..512");
+  if (batch == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == QMX_C64) return fft3d_impl<float>(shape, data, batch, direction, scale, st);
+  if (dtype == QMX_C128) return fft3d_impl<double>(shape, data, batch, direction, scale, st);
+  set_error("qmx_fft3d: dtype must be QMX_C64 or QMX_C128");
+  return QMX_ERR_ARG;
+}
+
+
+int qmx_profile_enable(int on) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof_on = on != 0;
+  return QMX_OK;
+}
+
+int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids) {
+  QMX_CHECK_ARG(ms && bytes && count && n_ids > 0, "qmx_profile_read: null argument");
+  for (int i = 0; i < n_ids; ++i) {
+    ms[i] = 0;
+    bytes[i] = 0;
+    count[i] = 0;
+  }
+  std::vector<ProfRec> recs;
+  {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    recs.swap(g_prof);
+  }
+  int rc = QMX_OK;
+  for (auto& r : recs) {
+    float t = 0;
+    if (rc == QMX_OK) {
+      hipError_t e = hipEventSynchronize(r.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+      if (e != hipSuccess) {
+        set_error(std::string("qmx_profile_read: ") + hipGetErrorString(e));
+        rc = QMX_ERR_HIP;
+      }
+    }
+    if (r.id >= 0 && r.id < n_ids) {
+      ms[r.id] += t;
+      bytes[r.id] += r.bytes;
+      count[r.id] += 1;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// Band-wise elementwise and reduction kernels of the Davidson loop (gfx950).
+//
+// All are HBM-bound streams over (k, band, g) rows: one workgroup per row,
+// accumulating in double with a wave64 shuffle reduction.
+#include "qmx_common.h"
+
+using namespace qmx;
+
+namespace {
+
+constexpr int kRowThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of two values; result valid in thread 0.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+  __shared__ double sa[kRowThreads / 64], sb[kRowThreads / 64];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sa[w] = a;
+    sb[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0;
+    b = 0;
+    for (int i = 0; i < kRowThreads / 64; ++i) {
+      a += sa[i];
+      b += sb[i];
+    }
+  }
+}
+
+template <typename C2>
+__global__ void __launch_bounds__(kRowThreads)
+    k_band_reduce(const C2* __restrict__ C, int64_t ldb, int64_t ldk, int nb, int n_tot,
+                  const double* __restrict__ ke, int64_t ke_ldk, double* __restrict__ norm2,
+                  double* __restrict__ kin) {
+  const int b = blockIdx.x, k = blockIdx.y;
+  const C2* row = C + k * ldk + (int64_t)b * ldb;
+  const double* kk = ke ? ke + k * ke_ldk : nullptr;
+  double s = 0, e = 0;
+  for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+    const C2 c = row[g];
+    const double m = (double)c.x * c.x + (double)c.y * c.y;
+    s += m;
+    if (kk) e += m * kk[g];
+  }
+  block_sum2(s, e);
+  if (threadIdx.x == 0) {
+    norm2[(int64_t)k * nb + b] = s;
+    if (kin) kin[(int64_t)k * nb + b] = e;
+  }
+}
+
+template <typename C2>
+__global__ void __launch_bounds__(kRowThreads)
+    k_precond_residual(const C2* __restrict__ C, const C2* __restrict__ HC, int64_t ldb,
+                       int64_t ldk, const double* __restrict__ eig,
+                       const double* __restrict__ KEref, const double* __restrict__ ke,
+                       int64_t ke_ldk, C2* __restrict__ out, int64_t oldb, int64_t oldk, int nb,
+                       int n_tot, double* __restrict__ norm2) {
+  using R = decltype(C2{}.x);
+  const int b = blockIdx.x, k = blockIdx.y;
+  const int64_t ir = (int64_t)k * nb + b;
+  const C2* c = C + k * ldk + (int64_t)b * ldb;
+  const C2* h = HC + k * ldk + (int64_t)b * ldb;
+  C2* o = out + k * oldk + (int64_t)b * oldb;
+  const double* kk = ke + k * ke_ldk;
+  const double e = eig[ir];
+  const double inv_ref = 1.0 / KEref[ir];
+  double s = 0, dummy = 0;
+  for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+    double x = kk[g] * inv_ref;
+    x += exp(-x);
+    const double inv_x = 1.0 / x;
+    const C2 cc = c[g], hh = h[g];
+    const double rx = ((double)hh.x - e * (double)cc.x) * inv_x;
+    const double ry = ((double)hh.y - e * (double)cc.y) * inv_x;
+    const C2 r = C2{(R)rx, (R)ry};
+    o[g] = r;
+    s += (double)r.x * r.x + (double)r.y * r.y;
+  }
+  block_sum2(s, dummy);
+  if (threadIdx.x == 0) norm2[ir] = s;
+}
+
+template <typename C2>
+__global__ void __launch_bounds__(kRowThreads)
+    k_scale_constrain(C2* __restrict__ C, int64_t ldb, int64_t ldk, int nb, int n_tot,
+                      const double* __restrict__ scale, const int* __restrict__ nvec) {
+  using R = decltype(C2{}.x);
+  const int b = blockIdx.x, k = blockIdx.y;
+  C2* row = C + k * ldk + (int64_t)b * ldb;
+  const R s = scale ? (R)scale[(int64_t)k * nb + b] : R(1);
+  const int nk_valid = nvec[k];
+  for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+    if (g >= nk_valid) {
+      row[g] = C2{0, 0};
+    } else if (scale) {
+      const C2 v = row[g];
+      row[g] = C2{v.x * s, v.y * s};
+    }
+  }
+}
+
+// Signed int64 xorshift step with arithmetic right shift, as on torch
+// int64 tensors; returns 2^-64 x + 1/2 in double.
+__device__ __forceinline__ double xorshift_u(int64_t& x) {
+  x ^= (int64_t)((uint64_t)x << 13);
+  x ^= x >> 7;
+  x ^= (int64_t)((uint64_t)x << 17);
+  return 5.42101086242752217e-20 * (double)x + 0.5;
+}
+
+template <typename C2>
+__global__ void k_randomize(C2* __restrict__ C, int64_t ldb, int64_t ldk, int n_tot,
+                            int64_t n_max, int64_t k_offset, const int* __restrict__ nvec,
+                            const double* __restrict__ ke, int64_t ke_ldk, int64_t seed,
+                            int64_t b_start, int64_t b_stop) {
+  using R = decltype(C2{}.x);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (g >= n_tot) return;
+  int64_t x = (1 + seed) + g + n_max * (k_offset + k);
+  for (int64_t i = 0; i <= b_start; ++i) {
+    xorshift_u(x);
+    xorshift_u(x);
+  }
+  const bool pad = g >= nvec[k];
+  const double e = ke[k * ke_ldk + g];
+  const double t = (4.0 / 3) * e;
+  const double t2 = t * t;
+  const double limit = 1.0 / (1.0 + t2 * t2 * t2);
+  for (int64_t b = b_start; b < b_stop; ++b) {
+    const double u0 = xorshift_u(x);
+    const double u1 = xorshift_u(x);
+    const double mag = sqrt(-2.0 * log(u0));
+    double sn, cs;
+    sincos(2.0 * 3.141592653589793 * u1, &sn, &cs);
+    C2 v = pad ? C2{0, 0} : C2{(R)(mag * cs * limit), (R)(mag * sn * limit)};
+    C[k * ldk + b * ldb + g] = v;
+  }
+}
+
+}  // namespace
+
+#define QMX_DISPATCH(dtype, KERNEL_CALL_F32, KERNEL_CALL_F64)          \
+  do {                                                                \
+    if ((dtype) == QMX_C64) {                                         \
+      KERNEL_CALL_F32;                                                \
+    } else if ((dtype) == QMX_C128) {                                 \
+      KERNEL_CALL_F64;                                                \
+    } else {                                                          \
+      qmx::set_error("dtype must be QMX_C64 or QMX_C128");            \
+      return QMX_ERR_ARG;                                             \
+    }                                                                 \
+    QMX_LAUNCHED();                                                   \
+  } while (0)
+
+extern "C" {
+
+int qmx_band_reduce(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t nk, int64_t nb,
+                    int64_t n_tot, const double* ke, int64_t ke_ldk, double* norm2, double* kin,
+                    void* stream) {
+  QMX_CHECK_ARG(C && norm2, "qmx_band_reduce: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
+                "qmx_band_reduce: bad sizes");
+  QMX_CHECK_ARG(!kin || ke, "qmx_band_reduce: kin requires ke");
+  if (nk == 0 || nb == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_band_reduce<float2>, grid, dim3(kRowThreads), 0, st,
+                                  (const float2*)C, ldb, ldk, (int)nb, (int)n_tot, ke, ke_ldk,
+                                  norm2, kin),
+               hipLaunchKernelGGL(k_band_reduce<double2>, grid, dim3(kRowThreads), 0, st,
+                                  (const double2*)C, ldb, ldk, (int)nb, (int)n_tot, ke, ke_ldk,
+                                  norm2, kin));
+  return QMX_OK;
+}
+
+int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, int64_t ldk,
+                         const double* eig, const double* KEref, const double* ke,
+                         int64_t ke_ldk, void* out, int64_t out_ldb, int64_t out_ldk,
+                         int64_t nk, int64_t nb, int64_t n_tot, double* norm2, void* stream) {
+  QMX_CHECK_ARG(C && HC && eig && KEref && ke && out && norm2,
+                "qmx_precond_residual: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
+                "qmx_precond_residual: bad sizes");
+  if (nk == 0 || nb == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_precond_residual<float2>, grid, dim3(kRowThreads), 0, st,
+                                  (const float2*)C, (const float2*)HC, ldb, ldk, eig, KEref, ke,
+                                  ke_ldk, (float2*)out, out_ldb, out_ldk, (int)nb, (int)n_tot,
+                                  norm2),
+               hipLaunchKernelGGL(k_precond_residual<double2>, grid, dim3(kRowThreads), 0, st,
+                                  (const double2*)C, (const double2*)HC, ldb, ldk, eig, KEref,
+                                  ke, ke_ldk, (double2*)out, out_ldb, out_ldk, (int)nb,
+                                  (int)n_tot, norm2));
+  return QMX_OK;
+}
+
+int qmx_scale_constrain(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk, int64_t nb,
+                        int64_t n_tot, const double* scale, const int32_t* n, void* stream) {
+  QMX_CHECK_ARG(C && n, "qmx_scale_constrain: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
+                "qmx_scale_constrain: bad sizes");
+  if (nk == 0 || nb == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_scale_constrain<float2>, grid, dim3(kRowThreads), 0, st,
+                                  (float2*)C, ldb, ldk, (int)nb, (int)n_tot, scale, n),
+               hipLaunchKernelGGL(k_scale_constrain<double2>, grid, dim3(kRowThreads), 0, st,
+                                  (double2*)C, ldb, ldk, (int)nb, (int)n_tot, scale, n));
+  return QMX_OK;
+}
+
+int qmx_randomize(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk, int64_t n_tot,
+                  int64_t n_max, int64_t k_offset, const int32_t* n, const double* ke,
+                  int64_t ke_ldk, int64_t seed, int64_t b_start, int64_t b_stop, void* stream) {
+  QMX_CHECK_ARG(C && n && ke, "qmx_randomize: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && n_tot >= 0 && n_tot < (1 << 30) && b_start >= 0,
+                "qmx_randomize: bad sizes");
+  if (nk == 0 || n_tot == 0 || b_stop <= b_start) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)((n_tot + 255) / 256), (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_randomize<float2>, grid, dim3(256), 0, st, (float2*)C, ldb,
+                                  ldk, (int)n_tot, n_max, k_offset, n, ke, ke_ldk, seed, b_start,
+                                  b_stop),
+               hipLaunchKernelGGL(k_randomize<double2>, grid, dim3(256), 0, st, (double2*)C, ldb,
+                                  ldk, (int)n_tot, n_max, k_offset, n, ke, ke_ldk, seed, b_start,
+                                  b_stop));
+  return QMX_OK;
+}
+
+}  // extern "C"

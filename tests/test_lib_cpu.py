@@ -1,0 +1,98 @@
+"""Host-side checks of libqmx.so: it loads without a GPU, exports every entry
+point include/qmx.h declares, and its sparse-FFT index tables (built by the
+same C++ code a plan uploads) reproduce the FP64 oracle's local-potential
+apply when the five passes are emulated in numpy on them."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.basis import Basis
+from oracle.hamiltonian import apply_potential
+from qimpy_amd import _lib, systems
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "qmx.h").read_text()
+    return sorted(set(re.findall(r"\b(qmx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    lib = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert lib.qmx_version().decode().startswith("qmx")
+
+
+def emulate_local(basis, V, C, T):
+    """Numpy model of passes A-E on the plan tables."""
+    K, B, G = C.shape
+    nx, ny, nz = basis.shape
+    N = nx * ny * nz
+    out = np.zeros_like(C)
+    for k in range(K):
+        n = basis.n[k]
+        t = T["t1idx"][k, :n]
+        cs = T["cstart"][k]
+        assert cs[-1] == n and np.all(np.diff(cs) >= 0)
+        cols = np.repeat(np.arange(T["ncol"]), np.diff(cs))
+        assert np.array_equal(t // nz, cols)  # each column's points are contiguous
+        for b in range(B):
+            T1 = np.zeros((T["ncol"], nz), complex)
+            T1[t // nz, t % nz] = C[k, b, :n]
+            T1 = np.fft.ifft(T1, axis=1) * nz
+            T2 = np.zeros((T["nxa"], ny, nz), complex)
+            for xa in range(T["nxa"]):
+                c0, c1 = T["pcol_start"][xa], T["pcol_start"][xa + 1]
+                T2[xa, T["col_iy"][c0:c1]] = T1[c0:c1]
+            T2 = np.fft.ifft(T2, axis=1) * ny
+            box = np.zeros((nx, ny, nz), complex)
+            box[T["x_list"]] = T2
+            box = np.fft.fft(np.fft.ifft(box, axis=0) * nx * V, axis=0)
+            T2 = np.fft.fft(box[T["x_list"]], axis=1)
+            for xa in range(T["nxa"]):
+                c0, c1 = T["pcol_start"][xa], T["pcol_start"][xa + 1]
+                T1[c0:c1] = T2[xa, T["col_iy"][c0:c1]]
+            T1 = np.fft.fft(T1, axis=1) / N
+            out[k, b, :n] = T1[t // nz, t % nz]
+    return out
+
+
+@pytest.mark.parametrize("which", ["small", "cfg0"])
+def test_pass_model_matches_oracle(which):
+    spec = systems.small_cell(n_bands=3) if which == "small" else systems.make_cell(0, n_bands=2, kmesh=(1, 1, 2))
+    b = Basis(spec)
+    T = _lib.geom_tables(spec.shape, b.n, b.fft_index)
+    assert T["nxa"] <= spec.shape[0] and T["ncol"] <= spec.shape[0] * spec.shape[1]
+    assert np.all(np.diff(T["x_list"]) > 0)
+    rng = np.random.default_rng(0)
+    C = rng.standard_normal((len(b.k), 2, b.n_tot)) + 1j * rng.standard_normal((len(b.k), 2, b.n_tot))
+    b.constrain(C)
+    ref = apply_potential(b, spec.V, C)
+    assert np.abs(emulate_local(b, spec.V, C, T) - ref).max() < 1e-13
+
+
+def test_workload_sparsity():
+    """Active planes and columns of the measured boxes (cfg0 36^3, cfg1 96^3)."""
+    b0 = Basis(systems.make_cell(0))
+    T0 = _lib.geom_tables(b0.shape, b0.n, b0.fft_index)
+    b1 = Basis(systems.make_cell(1))
+    T1 = _lib.geom_tables(b1.shape, b1.n, b1.fft_index)
+    assert T0["nxa"] < 36 // 2 + 2 and T1["nxa"] < 96 // 2
+    assert T1["ncol"] < 0.2 * 96 * 96
+
+
+def test_geom_errors():
+    fi = np.array([[5, 3, 7]], dtype=np.int64)  # not increasing
+    with pytest.raises(RuntimeError, match="increasing"):
+        _lib.geom_tables((4, 4, 4), np.array([3]), fi)
+    with pytest.raises(RuntimeError, match="2, 3 and 5"):
+        _lib.geom_tables((7, 4, 4), np.array([1]), np.array([[0]], dtype=np.int64))
+    with pytest.raises(RuntimeError, match="n\\[k\\]"):
+        _lib.geom_tables((4, 4, 4), np.array([5]), np.array([[0, 1, 2]], dtype=np.int64))
