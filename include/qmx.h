@@ -109,13 +109,15 @@ int qmx_band_reduce(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t 
 
 /* Davidson residual + inverse-kinetic preconditioner (_davidson.py:93-99,
  * :182-184):  out = (HC - eig C) / x,  x = KE/KEref,  x <- x + exp(-x);
- * also writes norm2[k, b] = sum_g |out|^2 (double). eig, KEref: double
- * [nk * nb]; ke double [nk * ke_ldk]. */
+ * also writes norm2[k, b] = sum_g |out|^2 and, if rnorm2 != NULL, the
+ * unpreconditioned residual norm rnorm2[k, b] = sum_g |HC - eig C|^2, both
+ * accumulated in double. eig, KEref: double [nk * nb]; ke double
+ * [nk * ke_ldk]. */
 int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb,
                          int64_t ldk, const double* eig, const double* KEref,
                          const double* ke, int64_t ke_ldk, void* out, int64_t out_ldb,
                          int64_t out_ldk, int64_t nk, int64_t nb, int64_t n_tot,
-                         double* norm2, void* stream);
+                         double* norm2, double* rnorm2, void* stream);
 
 /* Scale each band by scale[k, b] (double, or 1 if NULL) and zero the
  * padding g >= n[k] (n: device int32 [nk]) — Wavefunction.constrain

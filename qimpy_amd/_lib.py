@@ -48,7 +48,7 @@ SIGNATURES = {
     "qmx_band_reduce": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, i64, vp, vp, vp]),
     "qmx_precond_residual": (
         i32,
-        [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp],
+        [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp, vp],
     ),
     "qmx_scale_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
     "qmx_randomize": (

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from .davidson import Davidson
-from .linalg import eighg
+from .linalg import eighg, eighg_robust
 
 
 class CheFSI(Davidson):
@@ -88,7 +88,10 @@ class CheFSI(Davidson):
                 HC = HC0.cat(HC, clear=True)
                 del C0, HC0
             eig_prev = el.eig
-            el.eig, V = eighg(el.C ^ HC, el.C.dot_O(el.C).wait())
+            if el.C.coeff.dtype == torch.complex64:
+                el.eig, V = eighg_robust(el.C ^ HC, el.C.dot_O(el.C).wait())
+            else:
+                el.eig, V = eighg(el.C ^ HC, el.C.dot_O(el.C).wait())
             el.C = el.C @ V
             HC = HC @ V
             deig = torch.abs(el.eig - eig_prev)
@@ -97,6 +100,7 @@ class CheFSI(Davidson):
             self._report(n_eigs_done, converged)
             if converged:
                 break
+        self.last_HC = HC
         if type(self) != CheFSI:
             self._HC = HC
 

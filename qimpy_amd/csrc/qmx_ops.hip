@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kRowThreads)
                        int64_t ldk, const double* __restrict__ eig,
                        const double* __restrict__ KEref, const double* __restrict__ ke,
                        int64_t ke_ldk, C2* __restrict__ out, int64_t oldb, int64_t oldk, int nb,
-                       int n_tot, double* __restrict__ norm2) {
+                       int n_tot, double* __restrict__ norm2, double* __restrict__ rnorm2) {
   using R = decltype(C2{}.x);
   const int b = blockIdx.x, k = blockIdx.y;
   const int64_t ir = (int64_t)k * nb + b;
@@ -75,20 +75,24 @@ __global__ void __launch_bounds__(kRowThreads)
   const double* kk = ke + k * ke_ldk;
   const double e = eig[ir];
   const double inv_ref = 1.0 / KEref[ir];
-  double s = 0, dummy = 0;
+  double s = 0, sr = 0;
   for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
     double x = kk[g] * inv_ref;
     x += exp(-x);
     const double inv_x = 1.0 / x;
     const C2 cc = c[g], hh = h[g];
-    const double rx = ((double)hh.x - e * (double)cc.x) * inv_x;
-    const double ry = ((double)hh.y - e * (double)cc.y) * inv_x;
-    const C2 r = C2{(R)rx, (R)ry};
+    const double ex = (double)hh.x - e * (double)cc.x;
+    const double ey = (double)hh.y - e * (double)cc.y;
+    sr += ex * ex + ey * ey;
+    const C2 r = C2{(R)(ex * inv_x), (R)(ey * inv_x)};
     o[g] = r;
     s += (double)r.x * r.x + (double)r.y * r.y;
   }
-  block_sum2(s, dummy);
-  if (threadIdx.x == 0) norm2[ir] = s;
+  block_sum2(s, sr);
+  if (threadIdx.x == 0) {
+    norm2[ir] = s;
+    if (rnorm2) rnorm2[ir] = sr;
+  }
 }
 
 template <typename C2>
@@ -189,7 +193,8 @@ int qmx_band_reduce(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t 
 int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, int64_t ldk,
                          const double* eig, const double* KEref, const double* ke,
                          int64_t ke_ldk, void* out, int64_t out_ldb, int64_t out_ldk,
-                         int64_t nk, int64_t nb, int64_t n_tot, double* norm2, void* stream) {
+                         int64_t nk, int64_t nb, int64_t n_tot, double* norm2, double* rnorm2,
+                         void* stream) {
   QMX_CHECK_ARG(C && HC && eig && KEref && ke && out && norm2,
                 "qmx_precond_residual: null argument");
   QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
@@ -201,11 +206,11 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, 
                hipLaunchKernelGGL(k_precond_residual<float2>, grid, dim3(kRowThreads), 0, st,
                                   (const float2*)C, (const float2*)HC, ldb, ldk, eig, KEref, ke,
                                   ke_ldk, (float2*)out, out_ldb, out_ldk, (int)nb, (int)n_tot,
-                                  norm2),
+                                  norm2, rnorm2),
                hipLaunchKernelGGL(k_precond_residual<double2>, grid, dim3(kRowThreads), 0, st,
                                   (const double2*)C, (const double2*)HC, ldb, ldk, eig, KEref,
                                   ke, ke_ldk, (double2*)out, out_ldb, out_ldk, (int)nb,
-                                  (int)n_tot, norm2));
+                                  (int)n_tot, norm2, rnorm2));
   return QMX_OK;
 }
 

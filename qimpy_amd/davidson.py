@@ -18,15 +18,26 @@ import torch
 
 from . import _lib, rc
 from .basis import DTYPE_CODE
-from .linalg import dagger, eighg
+from .linalg import dagger, eighg, eighg_robust
 from .wavefunction import Wavefunction
 
 
 class Davidson:
     """Davidson diagonalization of the Hamiltonian in `electrons`."""
 
-    def __init__(self, *, electrons, n_iterations: int = 100, eig_threshold: float = 1e-8):
+    def __init__(self, *, electrons, n_iterations: int = 100, eig_threshold: float = 1e-8,
+                 canonical_overlap: Optional[bool] = None,
+                 residual_threshold: Optional[float] = None):
         self.electrons = electrons
+        #: When set, bands count as converged once ||H C - eig C|| (accumulated
+        #: in double) is below this, instead of the eigenvalue change of
+        #: _davidson.py:126-139: in float32 the eigenvalue change has a noise
+        #: floor near 1e-6 Ha that an FP64-sized threshold never passes.
+        self.residual_threshold = residual_threshold
+        #: Solve the expanded subspace by canonical orthogonalisation instead
+        #: of Cholesky; defaults to on for complex64 wavefunctions, whose
+        #: float32-accumulated overlaps can lose positive definiteness.
+        self.canonical_overlap = canonical_overlap
         self.n_iterations = n_iterations
         self.eig_threshold = eig_threshold
         self._line_prefix = "Davidson"
@@ -54,6 +65,7 @@ class Davidson:
         S, K, B, X, G = c.shape
         out = torch.empty((S, K, B, X, G), dtype=c.dtype, device=c.device)
         norm2 = torch.empty((S, K, B), dtype=torch.float64, device=c.device)
+        rnorm2 = torch.empty((S, K, B), dtype=torch.float64, device=c.device)
         ke = self.electrons.basis.get_ke()
         eig = eig_sel.contiguous()
         kref = KEref.contiguous()
@@ -66,9 +78,37 @@ class Davidson:
                 "qmx_precond_residual", DTYPE_CODE[c.dtype], cs.data_ptr(), hs.data_ptr(),
                 cs.stride(1), cs.stride(0), eig[s].data_ptr(), kref[s].data_ptr(),
                 ke.data_ptr(), ke.stride(0), out[s].data_ptr(), out.stride(2), out.stride(1),
-                K, B, G, norm2[s].data_ptr(), rc.stream_ptr(),
+                K, B, G, norm2[s].data_ptr(), rnorm2[s].data_ptr(), rc.stream_ptr(),
             )
+        self.residual_norm = rnorm2.sqrt()
         return Wavefunction(C_sel.basis, coeff=out), norm2.sqrt()
+
+    def _orthogonalize(self, C: Wavefunction) -> bool:
+        ortho = self.canonical_overlap
+        return (C.coeff.dtype == torch.complex64) if ortho is None else ortho
+
+    def _project_out(self, C: Wavefunction, Cexp: Wavefunction, drop_below: float = 1e-3) -> None:
+        """Cexp <- (1 - C C^H) Cexp twice (block classical Gram-Schmidt with
+        re-orthogonalisation), then renormalise. Directions that keep less
+        than `drop_below` of their norm lie in span(C) up to float32 rounding;
+        they are re-drawn at random (as _regularize does for null bands) and
+        projected again. Used for float32 subspaces: the reference's expansion
+        (_davidson.py:183-197) leaves the near-dependence between C and Cexp
+        to its FP64 generalized eigensolve, which float32 overlaps cannot
+        resolve. In exact arithmetic the span, and so the Ritz values, are
+        unchanged."""
+        for attempt in range(2):
+            for _ in range(2):
+                M = (C ^ Cexp).wait()
+                Cexp -= C @ M
+            norm = Cexp.band_norm()
+            low = norm < drop_below
+            i_spin, i_k, i_band = torch.where(low)
+            if attempt or not len(i_spin):
+                break
+            Cexp.randomize_selected(i_spin, i_k, i_band, 1000 + self._i_iter)
+            Cexp.scale_constrain_(1.0 / Cexp.band_norm())
+        Cexp.scale_constrain_(1.0 / norm.clamp_min(1e-300))
 
     def _regularize(self, C: Wavefunction, norm: torch.Tensor, i_iter: int) -> None:
         """Randomize null bands, seeded by the iteration (:101-115)."""
@@ -127,8 +167,20 @@ class Davidson:
             HC_sel = HC[:, :, n_eigs_done:] if n_eigs_done else HC
             KEref = C_sel.band_ke()
             Cexp, norm_exp = self._residual_expansion(C_sel, HC_sel, eig_sel, KEref)
+            res_thr = self.residual_threshold
+            if res_thr is not None:
+                # residuals of the bands not yet locked (index >= n_eigs_done):
+                r = self.residual_norm[..., : max(n_bands - n_eigs_done, 0)]
+                pending = torch.where((r > res_thr).flatten(0, 1).any(dim=0))[0]
+                n_res_done = n_eigs_done + (
+                    int(pending[0].item()) if len(pending) else n_bands - n_eigs_done)
+                if n_res_done >= n_eigs_target:
+                    self._report(n_res_done, True)
+                    break
             self._regularize(Cexp, norm_exp, self._i_iter)
             Cexp.scale_constrain_(1.0 / norm_exp)
+            if self._orthogonalize(el.C):
+                self._project_out(el.C, Cexp)
             n_bands_exp = Cexp.n_bands()
             n_bands_new = n_bands_cur + n_bands_exp
 
@@ -143,7 +195,13 @@ class Davidson:
             C_HC = torch.diag_embed(el.eig)
             C_HC_new = TileExpansion(C_HC, Cnew ^ HCexp)
 
-            eig_new, V_new = eighg(C_HC_new, C_OC_new.wait())
+            canonical = self.canonical_overlap
+            if canonical is None:
+                canonical = Cnew.coeff.dtype == torch.complex64
+            if canonical:
+                eig_new, V_new = eighg_robust(C_HC_new, C_OC_new.wait())
+            else:
+                eig_new, V_new = eighg(C_HC_new, C_OC_new.wait())
             n_bands_next = min(n_bands_new, n_bands_max)
             V_new = V_new[..., :n_bands_next]
             Vcur, Vexp = V_new.split((n_bands_cur, n_bands_exp), dim=2)
@@ -157,10 +215,13 @@ class Davidson:
             el.eig = eig_new[..., :n_bands_next]
 
             el.deig_max, n_eigs_done = self._check_deig(deig, eig_threshold)
-            converged = n_eigs_done >= n_eigs_target
+            if res_thr is not None:
+                n_eigs_done = n_res_done  # lock by residual; test at the next start
+            converged = res_thr is None and n_eigs_done >= n_eigs_target
             self._report(n_eigs_done, converged)
             if converged:
                 break
+        self.last_HC = HC  # H C of the final subspace, for residual reports
         if helper:
             self._HC = HC
 

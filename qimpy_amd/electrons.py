@@ -55,6 +55,7 @@ class Electrons:
         eig_threshold: float = 1e-8,
         filter_order: int = 10,
         init_threshold: float = 1e-1,
+        residual_threshold: Optional[float] = None,
     ):
         self.basis = basis
         self.ions = basis.ions
@@ -68,7 +69,8 @@ class Electrons:
         self.deig_max = np.inf
         if diagonalizer == "davidson":
             self.diagonalize = Davidson(electrons=self, n_iterations=n_iterations,
-                                        eig_threshold=eig_threshold)
+                                        eig_threshold=eig_threshold,
+                                        residual_threshold=residual_threshold)
         elif diagonalizer == "chefsi":
             self.diagonalize = CheFSI(electrons=self, n_iterations=n_iterations,
                                       eig_threshold=eig_threshold, filter_order=filter_order,
@@ -123,7 +125,9 @@ class Electrons:
         if fp64_finish:
             rep = self.rayleigh_ritz_fp64()
         else:
-            HC = self.hamiltonian(self.C)
+            HC = getattr(diag, "last_HC", None)
+            if HC is None or HC.n_bands() != self.C.n_bands():
+                HC = self.hamiltonian(self.C)
             resid = HC.coeff - self.C.coeff * self.eig[..., None, None].to(self.C.coeff.real.dtype)
             rnorm = torch.view_as_real(resid).square().sum(dim=(-3, -2, -1)).sqrt().to(torch.float64)
             n = self.fillings.n_bands
