@@ -137,6 +137,34 @@ int qmx_randomize(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk,
                   const double* ke, int64_t ke_ldk, int64_t seed, int64_t b_start,
                   int64_t b_stop, void* stream);
 
+/* Householder reduction of one Hermitian complex128 matrix to real
+ * tridiagonal form, T = Q^H A Q, in one cooperative (grid-synchronised)
+ * launch: the latency-bound core of the subspace eigensolve eigh(U^H H U)
+ * in eighg (math/_linalg.py:181) and the initial eigh of Davidson
+ * (_davidson.py:165). A is row-major with leading dimension lda and only its
+ * lower triangle is read (UPLO = L). Outputs: d[n] and e[n-1] (real), and Q
+ * (n x n) in column-major order, as a tridiagonal divide-and-conquer with
+ * "original" eigenvectors expects. n_groups / n_threads (256, 512, 1024)
+ * set the launch shape; 0 picks the defaults. */
+int64_t qmx_hetrd_workspace_bytes(int64_t n);
+int qmx_hetrd(const void* A, int64_t n, int64_t lda, double* d, double* e, void* Q,
+              void* workspace, int64_t workspace_bytes, int n_groups, int n_threads,
+              void* stream);
+
+/* Batched Hermitian eigensolver for small complex128 matrices (n <= 96) by
+ * cyclic Jacobi, one workgroup per matrix: the subspace eigensolves
+ * eigh(U^H H U) of eighg (math/_linalg.py:181) and Davidson's initial eigh
+ * (_davidson.py:165) for all k-points / cells of a step in one launch.
+ * Input: batch matrices, row-major with leading dimension lda, batch stride
+ * stride_a, lower triangle read (UPLO = L). Output: E[batch * n] ascending,
+ * V row-major (ldv, stride_v) with eigenvector j in column j. Workspace of
+ * qmx_heevj_workspace_bytes(n, batch) bytes is needed for n > 70.
+ * max_sweeps <= 0 selects 30. */
+int64_t qmx_heevj_workspace_bytes(int64_t n, int64_t batch);
+int qmx_heevj_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a, int64_t batch,
+                      double* E, void* V, int64_t ldv, int64_t stride_v, void* workspace,
+                      int64_t workspace_bytes, int max_sweeps, void* stream);
+
 /* Kernel ids reported by the profiler. */
 enum {
   QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from .davidson import Davidson
-from .linalg import eighg, eighg_robust
+from .linalg import eigh, eighg, eighg_robust
 
 
 class CheFSI(Davidson):
@@ -44,7 +44,7 @@ class CheFSI(Davidson):
             del self._HC
         else:
             HC = el.hamiltonian(el.C)
-            el.eig, V = torch.linalg.eigh((el.C ^ HC).wait().to(torch.complex128))
+            el.eig, V = eigh((el.C ^ HC).wait().to(torch.complex128))
             el.deig_max = np.inf
             el.C = el.C @ V
             HC = HC @ V

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib, rc
 from .basis import DTYPE_CODE
-from .linalg import dagger, eighg, eighg_robust
+from .linalg import dagger, eigh, eighg, eighg_robust
 from .wavefunction import Wavefunction
 
 
@@ -152,7 +152,7 @@ class Davidson:
             )
         self.history = []
         HC = el.hamiltonian(el.C)
-        el.eig, V = torch.linalg.eigh((el.C ^ HC).wait().to(torch.complex128))
+        el.eig, V = eigh((el.C ^ HC).wait().to(torch.complex128))
         el.deig_max = np.inf
         el.C = el.C @ V
         HC = HC @ V
