@@ -22,6 +22,37 @@ from .linalg import dagger, eigh, eighg, eighg_robust
 from .wavefunction import Wavefunction
 
 
+class _PhaseTimer:
+    """Accumulates GPU time per solver phase with events on the current stream
+    (enabled only when a caller asks for a breakdown)."""
+
+    def __init__(self):
+        self.enabled = False
+        self.times: dict = {}
+        self._pending: list = []
+
+    def mark(self, name: str):
+        if not self.enabled:
+            return None
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return (name, ev)
+
+    def done(self, token) -> None:
+        if token is None:
+            return
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self._pending.append((token[0], token[1], ev))
+
+    def collect(self) -> dict:
+        torch.cuda.synchronize()
+        for name, a, b in self._pending:
+            self.times[name] = self.times.get(name, 0.0) + a.elapsed_time(b)
+        self._pending = []
+        return self.times
+
+
 class Davidson:
     """Davidson diagonalization of the Hamiltonian in `electrons`."""
 
@@ -44,6 +75,7 @@ class Davidson:
         self._norm_cut = np.sqrt(electrons.basis.n_tot * 1e-15)  # :56-58
         self._i_iter = 0
         self.history: list = []
+        self.timer = _PhaseTimer()
 
     def __repr__(self) -> str:
         return f"Davidson(n_iterations: {self.n_iterations}, eig_threshold: {self.eig_threshold:g})"
@@ -165,8 +197,10 @@ class Davidson:
             eig_sel = el.eig[:, :, n_eigs_done:]
             C_sel = el.C[:, :, n_eigs_done:] if n_eigs_done else el.C
             HC_sel = HC[:, :, n_eigs_done:] if n_eigs_done else HC
+            tk = self.timer.mark("residual")
             KEref = C_sel.band_ke()
             Cexp, norm_exp = self._residual_expansion(C_sel, HC_sel, eig_sel, KEref)
+            self.timer.done(tk)
             res_thr = self.residual_threshold
             if res_thr is not None:
                 # residuals of the bands not yet locked (index >= n_eigs_done):
@@ -180,20 +214,29 @@ class Davidson:
             self._regularize(Cexp, norm_exp, self._i_iter)
             Cexp.scale_constrain_(1.0 / norm_exp)
             if self._orthogonalize(el.C):
+                tk = self.timer.mark("project")
                 self._project_out(el.C, Cexp)
+                self.timer.done(tk)
             n_bands_exp = Cexp.n_bands()
             n_bands_new = n_bands_cur + n_bands_exp
 
             Cnew = el.C.cat(Cexp, clear=True)
             Cexp = Cnew[:, :, n_bands_cur:]
 
+            tk = self.timer.mark("overlap")
             C_OC = torch.eye(n_bands_cur, device=rc.device, dtype=torch.float64)[None, None]
             C_OC_new = TileExpansion(C_OC, Cnew.dot_O(Cexp))
+            self.timer.done(tk)
 
+            tk = self.timer.mark("hamiltonian")
             HCexp = el.hamiltonian(Cexp)
+            self.timer.done(tk)
             del Cexp
+            tk = self.timer.mark("overlap")
             C_HC = torch.diag_embed(el.eig)
             C_HC_new = TileExpansion(C_HC, Cnew ^ HCexp)
+            self.timer.done(tk)
+            tk = self.timer.mark("eighg")
 
             canonical = self.canonical_overlap
             if canonical is None:
@@ -202,19 +245,24 @@ class Davidson:
                 eig_new, V_new = eighg_robust(C_HC_new, C_OC_new.wait())
             else:
                 eig_new, V_new = eighg(C_HC_new, C_OC_new.wait())
+            self.timer.done(tk)
             n_bands_next = min(n_bands_new, n_bands_max)
             V_new = V_new[..., :n_bands_next]
             Vcur, Vexp = V_new.split((n_bands_cur, n_bands_exp), dim=2)
 
+            tk = self.timer.mark("rotate")
             el.C = Cnew @ V_new
             del Cnew
             HC = HC @ Vcur
             HC += HCexp @ Vexp
             del HCexp
+            self.timer.done(tk)
             deig = torch.abs(el.eig - eig_new[..., :n_bands_cur])
             el.eig = eig_new[..., :n_bands_next]
 
+            tk = self.timer.mark("check")
             el.deig_max, n_eigs_done = self._check_deig(deig, eig_threshold)
+            self.timer.done(tk)
             if res_thr is not None:
                 n_eigs_done = n_res_done  # lock by residual; test at the next start
             converged = res_thr is None and n_eigs_done >= n_eigs_target
