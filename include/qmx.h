@@ -165,6 +165,17 @@ int qmx_heevj_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a, i
                       double* E, void* V, int64_t ldv, int64_t stride_v, void* workspace,
                       int64_t workspace_bytes, int max_sweeps, void* stream);
 
+/* Batched Hermitian eigensolver for n <= 128, one workgroup per matrix:
+ * Householder tridiagonalisation, implicit-shift QL with eigenvectors, and the
+ * back-transformation V = Q Z, all inside one launch (the same role as
+ * qmx_heevj_batched, for larger n). Same input and output conventions;
+ * info[b] != 0 marks a matrix whose QL iteration did not converge.
+ * Workspace: qmx_heev_small_workspace_bytes(n, batch) bytes. */
+int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch);
+int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a,
+                           int64_t batch, double* E, void* V, int64_t ldv, int64_t stride_v,
+                           int* info, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Kernel ids reported by the profiler. */
 enum {
   QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */

@@ -29,7 +29,6 @@ using namespace qmx;
 
 namespace {
 
-constexpr int kMaxEigThreads = 1024;
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
@@ -451,6 +450,308 @@ extern "C" int qmx_heevj_batched(const void* A, int64_t n, int64_t lda, int64_t 
                      static_cast<const double2*>(A), (int)n, lda, stride_a, E,
                      static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace),
                      max_sweeps > 0 ? max_sweeps : 30);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Hermitian eigensolver for n <= 128, one workgroup per matrix:
+//   1. Householder reduction to real tridiagonal form T = Q^H A Q (the matrix
+//      lives in a workspace and stays in the cache; reflector v_k is kept in
+//      column k below the subdiagonal, tau_k in LDS);
+//   2. implicit-shift QL on T (tql2-style): thread 0 runs the bulge chase of
+//      one QL iteration and logs its plane rotations in LDS, then every thread
+//      applies the logged rotations to its own row of Z (LDS, row stride
+//      n + 1), one barrier pair per QL iteration;
+//   3. ascending order by rank, then V = Q Z with one thread per column,
+//      applying H_{n-2} ... H_0 with the reflectors staged in LDS.
+// Output as qmx_heevj_batched; info[b] != 0 if QL did not converge.
+namespace {
+
+constexpr int kSmallThreads = 256;
+constexpr int kSmallMaxN = 128;
+
+template <int NW>
+__device__ __forceinline__ double block_sum_small(double x, double* red) {
+  x = wave_sum_d(x);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = x;
+  __syncthreads();
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) s += red[i];
+  return s;
+}
+
+__global__ void __launch_bounds__(kSmallThreads)
+    k_heev_small(const double2* __restrict__ A, int n, int64_t lda, int64_t stride_a,
+                 double* __restrict__ E, double2* __restrict__ Vout, int64_t ldv,
+                 int64_t stride_v, double2* __restrict__ Wws, int max_iter, int* __restrict__ info) {
+  constexpr int NW = kSmallThreads / 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ldz = n + 1;
+  double2* v = reinterpret_cast<double2*>(smem_raw);
+  double2* w = v + n;
+  double2* tau = w + n;
+  double* Z = reinterpret_cast<double*>(tau + n);
+  double* d = Z + n * ldz;
+  double* e = d + n;
+  double* lc = e + n;
+  double* ls = lc + n;
+  double* red = ls + n;
+  int* li = reinterpret_cast<int*>(red + NW);
+  int* perm = li + n;
+  int* ctl = perm + n;  // [0] log length, [1] done flag, [2] failure
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t b = blockIdx.x;
+  const double2* Ab = A + b * stride_a;
+  double2* W = Wws + b * (int64_t)n * n;
+
+  for (int idx = tid; idx < n * n; idx += kSmallThreads) {
+    const int i = idx / n, j = idx - i * n;
+    double2 x;
+    if (j < i) {
+      x = Ab[(int64_t)i * lda + j];
+    } else if (j > i) {
+      const double2 t = Ab[(int64_t)j * lda + i];
+      x = double2{t.x, -t.y};
+    } else {
+      x = double2{Ab[(int64_t)i * lda + i].x, 0.0};
+    }
+    W[(int64_t)i * n + j] = x;
+  }
+  if (tid == 0) ctl[2] = 0;
+  __syncthreads();
+
+  // 1. Householder tridiagonalisation.
+  for (int k = 0; k + 1 < n; ++k) {
+    double s = 0;
+    for (int i = k + 2 + tid; i < n; i += kSmallThreads) {
+      const double2 x = W[(int64_t)i * n + k];
+      s += x.x * x.x + x.y * x.y;
+    }
+    const double sigma = block_sum_small<NW>(s, red);
+    const double2 alpha = W[(int64_t)(k + 1) * n + k];
+    double beta, sr = 0, si = 0;
+    double2 tk;
+    if (sigma == 0.0 && alpha.y == 0.0) {
+      tk = double2{0.0, 0.0};
+      beta = alpha.x;
+    } else {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + sigma);
+      beta = alpha.x >= 0 ? -nrm : nrm;
+      tk = double2{(beta - alpha.x) / beta, -alpha.y / beta};
+      const double ar = alpha.x - beta, ai = alpha.y, den = ar * ar + ai * ai;
+      sr = ar / den;
+      si = -ai / den;
+    }
+    for (int i = k + 2 + tid; i < n; i += kSmallThreads) {
+      const double2 x = W[(int64_t)i * n + k];
+      const double2 vv = double2{x.x * sr - x.y * si, x.x * si + x.y * sr};
+      v[i] = vv;
+      W[(int64_t)i * n + k] = vv;
+    }
+    if (tid == 0) {
+      v[k + 1] = double2{1.0, 0.0};
+      tau[k] = tk;
+      e[k] = beta;
+      d[k] = W[(int64_t)k * n + k].x;
+    }
+    __syncthreads();
+    if (tk.x == 0.0 && tk.y == 0.0) continue;
+    // y = W[k+1:, k+1:] v, one wave per row
+    for (int i = k + 1 + wave; i < n; i += NW) {
+      const double2* Wi = W + (int64_t)i * n;
+      double yr = 0, yi = 0;
+      for (int j = k + 1 + lane; j < n; j += 64) {
+        const double2 a = Wi[j], bb = v[j];
+        yr += a.x * bb.x - a.y * bb.y;
+        yi += a.x * bb.y + a.y * bb.x;
+      }
+      yr = wave_sum_d(yr);
+      yi = wave_sum_d(yi);
+      if (lane == 0) w[i] = double2{yr, yi};
+    }
+    __syncthreads();
+    double part = 0;
+    for (int i = k + 1 + tid; i < n; i += kSmallThreads) {
+      const double2 vi = v[i], yi = w[i];
+      part += vi.x * yi.x + vi.y * yi.y;
+    }
+    const double vy = block_sum_small<NW>(part, red);
+    const double t2 = tk.x * tk.x + tk.y * tk.y;
+    for (int i = k + 1 + tid; i < n; i += kSmallThreads) {
+      const double2 y = w[i], vi = v[i];
+      w[i] = double2{tk.x * y.x - tk.y * y.y - 0.5 * t2 * vy * vi.x,
+                     tk.x * y.y + tk.y * y.x - 0.5 * t2 * vy * vi.y};
+    }
+    __syncthreads();
+    const int m = n - k - 1;
+    for (int idx = tid; idx < m * m; idx += kSmallThreads) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      const double2 vi = v[i], wi = w[i], vj = v[j], wj = w[j];
+      double2 a = W[(int64_t)i * n + j];
+      a.x -= vi.x * wj.x + vi.y * wj.y + wi.x * vj.x + wi.y * vj.y;
+      a.y -= vi.y * wj.x - vi.x * wj.y + wi.y * vj.x - wi.x * vj.y;
+      W[(int64_t)i * n + j] = a;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    d[n - 1] = W[(int64_t)(n - 1) * n + (n - 1)].x;
+    e[n - 1] = 0.0;
+    if (n == 1) tau[0] = double2{0.0, 0.0};
+  }
+  for (int idx = tid; idx < n * ldz; idx += kSmallThreads) {
+    const int i = idx / ldz, j = idx - i * ldz;
+    Z[idx] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  // 2. Implicit-shift QL; thread 0 chases, everyone rotates its row of Z.
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    for (;;) {
+      if (tid == 0) {
+        int m = l;
+        while (m < n - 1) {
+          const double dd = fabs(d[m]) + fabs(d[m + 1]);
+          if (fabs(e[m]) <= eps * dd) break;
+          ++m;
+        }
+        int nlog = 0;
+        if (m == l || ++iter > max_iter) {
+          if (m != l) ctl[2] = 1;
+          ctl[1] = 1;
+        } else {
+          ctl[1] = 0;
+          double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+          double r = hypot(g, 1.0);
+          g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+          double s = 1.0, c = 1.0, p = 0.0;
+          bool underflow = false;
+          for (int i = m - 1; i >= l; --i) {
+            const double f = s * e[i], bb = c * e[i];
+            r = hypot(f, g);
+            e[i + 1] = r;
+            if (r == 0.0) {
+              d[i + 1] -= p;
+              e[m] = 0.0;
+              underflow = true;
+              break;
+            }
+            s = f / r;
+            c = g / r;
+            g = d[i + 1] - p;
+            r = (d[i] - g) * s + 2.0 * c * bb;
+            p = s * r;
+            d[i + 1] = g + p;
+            g = c * r - bb;
+            li[nlog] = i;
+            lc[nlog] = c;
+            ls[nlog] = s;
+            ++nlog;
+          }
+          if (!underflow) {
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+          }
+        }
+        ctl[0] = nlog;
+      }
+      __syncthreads();
+      const int nlog = ctl[0];
+      const int done = ctl[1];
+      if (tid < n) {
+        double* zr = Z + tid * ldz;
+        for (int t = 0; t < nlog; ++t) {
+          const int i = li[t];
+          const double c = lc[t], s = ls[t];
+          const double f = zr[i + 1];
+          zr[i + 1] = s * zr[i] + c * f;
+          zr[i] = c * zr[i] - s * f;
+        }
+      }
+      __syncthreads();
+      if (done) break;
+    }
+  }
+
+  // 3. Ascending order and V = Q Z.
+  for (int i = tid; i < n; i += kSmallThreads) {
+    const double li_ = d[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double lj = d[j];
+      rank += (lj < li_) || (lj == li_ && j < i);
+    }
+    perm[rank] = i;
+    E[b * n + rank] = li_;
+  }
+  if (tid == 0 && info) info[b] = ctl[2];
+  __syncthreads();
+  double2* Vb = Vout + b * stride_v;
+  for (int j = tid; j < n; j += kSmallThreads) {
+    const int pj = perm[j];
+    for (int i = 0; i < n; ++i) Vb[(int64_t)i * ldv + j] = double2{Z[i * ldz + pj], 0.0};
+  }
+  for (int k = n - 2; k >= 0; --k) {
+    __syncthreads();
+    for (int i = k + 2 + tid; i < n; i += kSmallThreads) v[i] = W[(int64_t)i * n + k];
+    if (tid == 0) v[k + 1] = double2{1.0, 0.0};
+    __syncthreads();
+    const double2 tk = tau[k];
+    if (tk.x == 0.0 && tk.y == 0.0) continue;
+    for (int j = tid; j < n; j += kSmallThreads) {
+      double sr = 0, si = 0;  // s = v^H V[:, j]
+      for (int i = k + 1; i < n; ++i) {
+        const double2 vi = v[i], x = Vb[(int64_t)i * ldv + j];
+        sr += vi.x * x.x + vi.y * x.y;
+        si += vi.x * x.y - vi.y * x.x;
+      }
+      const double ts_r = tk.x * sr - tk.y * si, ts_i = tk.x * si + tk.y * sr;  // tau * s
+      for (int i = k + 1; i < n; ++i) {
+        const double2 vi = v[i];
+        double2 x = Vb[(int64_t)i * ldv + j];
+        x.x -= vi.x * ts_r - vi.y * ts_i;
+        x.y -= vi.x * ts_i + vi.y * ts_r;
+        Vb[(int64_t)i * ldv + j] = x;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch) {
+  if (n < 1 || batch < 0) return -1;
+  return batch * n * n * (int64_t)sizeof(double2);
+}
+
+extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a,
+                                      int64_t batch, double* E, void* V, int64_t ldv,
+                                      int64_t stride_v, int* info, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+  QMX_CHECK_ARG(A && E && V && workspace, "qmx_heev_small_batched: null argument");
+  QMX_CHECK_ARG(n >= 1 && n <= kSmallMaxN, "qmx_heev_small_batched: n must be in 1..128");
+  QMX_CHECK_ARG(lda >= n && ldv >= n && batch >= 0 && batch < (1ll << 31),
+                "qmx_heev_small_batched: bad strides or batch");
+  QMX_CHECK_ARG(workspace_bytes >= qmx_heev_small_workspace_bytes(n, batch),
+                "qmx_heev_small_batched: workspace too small");
+  if (batch == 0) return QMX_OK;
+  const size_t sh = (size_t)3 * n * sizeof(double2) + (size_t)n * (n + 1) * sizeof(double) +
+                    (size_t)4 * n * sizeof(double) + 4 * sizeof(double) +
+                    (size_t)2 * n * sizeof(int) + 4 * sizeof(int);
+  QMX_HIP(hipFuncSetAttribute((const void*)k_heev_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sh));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_heev_small, dim3((unsigned)batch), dim3(kSmallThreads), sh, st,
+                     static_cast<const double2*>(A), (int)n, lda, stride_a, E,
+                     static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace), 60,
+                     info);
   QMX_LAUNCHED();
   return QMX_OK;
 }

@@ -124,6 +124,65 @@ __device__ C2* fft_lds(C2* a, C2* b, const C2* tw, const Radices& rad, const Fas
   return a;
 }
 
+// LDS exchange between the lanes of one wave: all LDS stores of this wave
+// are complete before any later LDS load (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One Stockham pass over the mw lines [l0, l0 + mw) owned by the calling wave.
+template <typename C2, int R, int S>
+__device__ __forceinline__ void stockham_pass_wave(const C2* __restrict__ a, C2* __restrict__ b,
+                                                   const C2* __restrict__ tw, int n,
+                                                   const FastDiv& fns, const FastDiv& fmw,
+                                                   int ld, int l0, int lane) {
+  const int Ns = fns.d;
+  const int stride = n / R;
+  const int twstep = n / (Ns * R);
+  const int total = stride * fmw.d;
+  for (int t = lane; t < total; t += 64) {
+    const int j = fdiv(t, fmw);
+    const int l = l0 + t - j * fmw.d;
+    const int k = j - fdiv(j, fns) * Ns;
+    C2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[(j + r * stride) * ld + l];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      C2 w = tw[r * k * twstep];
+      if constexpr (S > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    dft<C2, R, S>(v);
+    const int idxD = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[(idxD + r * Ns) * ld + l] = v[r];
+  }
+}
+
+// Transform the lines [l0, l0 + mw) of `a` (scratch `b`) with the calling wave
+// only; returns the buffer holding the result (the same for every wave).
+template <typename C2, int S>
+__device__ C2* fft_lds_wave(C2* a, C2* b, const C2* tw, const Radices& rad, const FastDiv& fmw,
+                            int ld, int l0) {
+  const int lane = threadIdx.x & 63;
+  for (int ip = 0; ip < rad.nr; ++ip) {
+    switch (rad.r[ip]) {
+      case 4: stockham_pass_wave<C2, 4, S>(a, b, tw, rad.n, rad.ns[ip], fmw, ld, l0, lane); break;
+      case 2: stockham_pass_wave<C2, 2, S>(a, b, tw, rad.n, rad.ns[ip], fmw, ld, l0, lane); break;
+      case 3: stockham_pass_wave<C2, 3, S>(a, b, tw, rad.n, rad.ns[ip], fmw, ld, l0, lane); break;
+      case 5: stockham_pass_wave<C2, 5, S>(a, b, tw, rad.n, rad.ns[ip], fmw, ld, l0, lane); break;
+    }
+    wave_lds_sync();
+    C2* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
 // Stage the twiddle table of length n into LDS.
 template <typename C2>
 __device__ __forceinline__ void load_twiddles(C2* dst, const C2* __restrict__ src, int n) {

@@ -194,56 +194,68 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---------------------------------------------------------------- pass C
-// One workgroup: one y, one z-tile, one k, a chunk of bands (V reused).
+// One workgroup: one y, one z-tile of Mz, one k, a chunk of bands. Lines are
+// (band, z) pairs, nbg bands at a time (M = nbg * Mz lines, M / 4 per wave);
+// each wave transforms its own lines with wave-level LDS fences only, so a
+// group costs three workgroup barriers. V(x, y, z-tile) is staged once.
 template <typename C2, typename R>
 __global__ void __launch_bounds__(kThreads)
     k_xpass_potential(C2* __restrict__ T2, const R* __restrict__ V,
                       const int* __restrict__ x_list, const C2* __restrict__ tw_g, Radices rx,
-                      FastDiv fm, int ny, int nz, int nxa, int nb, int bands_per_chunk,
-                      int n_chunks) {
+                      FastDiv fmw, FastDiv fM, int zshift, int nbg, int ny, int nz, int nxa,
+                      int nb, int bands_per_chunk, int n_chunks) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nx = rx.n, M = fm.d, ld = odd_ld(M);
+  const int nx = rx.n, M = fM.d, ld = odd_ld(M), Mz = 1 << zshift, zmask = Mz - 1;
+  const int mw = fmw.d;
   C2* tw = reinterpret_cast<C2*>(smem_raw);
   C2* a = tw + nx;
   C2* b = a + nx * ld;
   R* vt = reinterpret_cast<R*>(b + nx * ld);
-  const int z0 = blockIdx.x * M, y = blockIdx.y;
+  const int z0 = blockIdx.x * Mz, y = blockIdx.y;
   const int k = blockIdx.z / n_chunks, chunk = blockIdx.z - k * n_chunks;
-  const int mz = min(M, nz - z0);
+  const int mz = min(Mz, nz - z0);
   const int bstart = chunk * bands_per_chunk;
   const int bstop = min(nb, bstart + bands_per_chunk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   load_twiddles(tw, tw_g, nx);
-  for (int i = threadIdx.x; i < nx * M; i += blockDim.x) {
-    const int x = fdiv(i, fm);
-    const int l = i - x * M;
-    vt[i] = l < mz ? V[((int64_t)x * ny + y) * nz + z0 + l] : R(0);
+  for (int i = threadIdx.x; i < nx * Mz; i += blockDim.x) {
+    const int x = i >> zshift;
+    const int zl = i & zmask;
+    vt[i] = zl < mz ? V[((int64_t)x * ny + y) * nz + z0 + zl] : R(0);
   }
   const int64_t plane = (int64_t)ny * nz;
-  for (int ib = bstart; ib < bstop; ++ib) {
+  for (int g0 = bstart; g0 < bstop; g0 += nbg) {
+    __syncthreads();  // previous group's stores have read LDS
     zero_lds(a, nx * ld);
     __syncthreads();
-    C2* base = T2 + ((int64_t)k * nb + ib) * nxa * plane + (int64_t)y * nz + z0;
     for (int i = threadIdx.x; i < nxa * M; i += blockDim.x) {
-      const int xa = fdiv(i, fm);
+      const int xa = fdiv(i, fM);
       const int l = i - xa * M;
-      if (l < mz) a[x_list[xa] * ld + l] = base[xa * plane + l];
+      const int ib = g0 + (l >> zshift), zl = l & zmask;
+      if (ib < bstop && zl < mz)
+        a[x_list[xa] * ld + l] =
+            T2[(((int64_t)k * nb + ib) * nxa + xa) * plane + (int64_t)y * nz + z0 + zl];
     }
     __syncthreads();
-    C2* r1 = fft_lds<C2, +1>(a, b, tw, rx, fm, ld);
+    const int l0 = wave * mw;
+    C2* r1 = fft_lds_wave<C2, +1>(a, b, tw, rx, fmw, ld, l0);
     C2* other = (r1 == a) ? b : a;
-    for (int i = threadIdx.x; i < nx * M; i += blockDim.x) {
-      const int x = fdiv(i, fm);
-      const int l = i - x * M;
-      r1[x * ld + l] = cscale(r1[x * ld + l], vt[i]);
+    for (int t = lane; t < nx * mw; t += 64) {
+      const int x = fdiv(t, fmw);
+      const int l = l0 + t - x * mw;
+      r1[x * ld + l] = cscale(r1[x * ld + l], vt[x * Mz + (l & zmask)]);
     }
+    wave_lds_sync();
+    C2* r2 = fft_lds_wave<C2, -1>(r1, other, tw, rx, fmw, ld, l0);
     __syncthreads();
-    C2* r2 = fft_lds<C2, -1>(r1, other, tw, rx, fm, ld);
     for (int i = threadIdx.x; i < nxa * M; i += blockDim.x) {
-      const int xa = fdiv(i, fm);
+      const int xa = fdiv(i, fM);
       const int l = i - xa * M;
-      if (l < mz) base[xa * plane + l] = r2[x_list[xa] * ld + l];
+      const int ib = g0 + (l >> zshift), zl = l & zmask;
+      if (ib < bstop && zl < mz)
+        T2[(((int64_t)k * nb + ib) * nxa + xa) * plane + (int64_t)y * nz + z0 + zl] =
+            r2[x_list[xa] * ld + l];
     }
-    __syncthreads();
   }
 }
 
@@ -468,11 +480,29 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
   // Tile sizes under the LDS budget:
   const int Mz = max_lines(nz, cs, 0, 64);
   const int My = even_tile(nz, max_lines(ny, cs, 0, 64));
-  const int Mx = even_tile(nz, max_lines(nx, cs, nx * (int)sizeof(R), 64));
-  const FastDiv fz = make_fastdiv(Mz), fy = make_fastdiv(My), fx = make_fastdiv(Mx);
+  // Pass C: z-tile of Mzc (power of two, <= 16) and nbg bands per group,
+  // M = nbg * Mzc lines, a multiple of the 4 waves, under the LDS budget.
+  int Mzc = 16, nbg = 1;
+  while (Mzc > 1 && Mzc / 2 >= nz) Mzc /= 2;
+  for (;;) {
+    int best = 0;
+    for (int g = 8; g >= 1; g /= 2) {
+      const int M = g * Mzc;
+      const size_t need = (size_t)(2 * odd_ld(M) * nx + nx) * cs + (size_t)nx * Mzc * sizeof(R);
+      if (M % 4 == 0 && need <= (size_t)kLdsBudget) { best = g; break; }
+    }
+    if (best) { nbg = best; break; }
+    if (Mzc <= 4) { nbg = 4 / Mzc > 0 ? 4 / Mzc : 1; break; }
+    Mzc /= 2;
+  }
+  const int Mx = nbg * Mzc;
+  int zshift = 0;
+  while ((1 << zshift) < Mzc) ++zshift;
+  const FastDiv fz = make_fastdiv(Mz), fy = make_fastdiv(My), fxw = make_fastdiv(Mx / 4),
+                fxM = make_fastdiv(Mx);
   const size_t shA = (size_t)(2 * odd_ld(Mz) * nz + nz) * cs;
   const size_t shB = (size_t)(2 * odd_ld(My) * ny + ny) * cs;
-  const size_t shC = (size_t)(2 * odd_ld(Mx) * nx + nx) * cs + (size_t)nx * Mx * sizeof(R);
+  const size_t shC = (size_t)(2 * odd_ld(Mx) * nx + nx) * cs + (size_t)nx * Mzc * sizeof(R);
   const R scale = R(1) / R(p->N);
   // Sphere points per band summed over k (for the byte counts):
   const double n_sphere = (double)p->n_sum;
@@ -498,15 +528,18 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_B, cur * (t1_elems + t2_elems) * bc);
     // Bands per chunk: enough workgroups to fill the chip, V reused within.
-    const int64_t wg_base = (int64_t)((nz + Mx - 1) / Mx) * ny * nk;
-    int n_chunks = (int)std::min<int64_t>(cur, std::max<int64_t>(1, (2048 + wg_base - 1) / wg_base));
-    const int bpc = (cur + n_chunks - 1) / n_chunks;
+    const int64_t wg_base = (int64_t)((nz + Mzc - 1) / Mzc) * ny * nk;
+    int n_chunks = (int)std::min<int64_t>((cur + nbg - 1) / nbg,
+                                          std::max<int64_t>(1, (2048 + wg_base - 1) / wg_base));
+    n_chunks = std::min<int>(n_chunks, 65535 / nk);
+    int bpc = (cur + n_chunks - 1) / n_chunks;
+    bpc = (bpc + nbg - 1) / nbg * nbg;
     n_chunks = (cur + bpc - 1) / bpc;
-    dim3 gC((nz + Mx - 1) / Mx, ny, nk * n_chunks);
+    dim3 gC((nz + Mzc - 1) / Mzc, ny, nk * n_chunks);
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_xpass_potential<C2, R>), gC, dim3(kThreads), shC, st, T2, V,
-                       p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fx, ny, nz, nxa, cur, bpc,
-                       n_chunks);
+                       p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fxw, fxM, zshift, nbg, ny, nz,
+                       nxa, cur, bpc, n_chunks);
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_C,
              cur * 2.0 * t2_elems * bc + (double)nk * n_chunks * (double)p->N * br);

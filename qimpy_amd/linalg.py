@@ -31,8 +31,10 @@ class Waitless(Generic[T]):
         return self.result
 
 
-#: Matrices up to this size go to the batched Jacobi kernel (qmx_heevj_batched).
+#: Size limits of the batched Jacobi (qmx_heevj_batched) and Householder + QL
+#: (qmx_heev_small_batched) kernels.
 JACOBI_MAX_N = 96
+SMALL_QL_MAX_N = 128
 #: Size window of the cooperative tridiagonalisation (qmx_hetrd): below it the
 #: library eigensolver is as fast, above it grid barriers make it slower.
 HETRD_MIN_N, HETRD_MAX_N = 200, 800
@@ -70,16 +72,57 @@ _rocsolver = None
 _hetrd_ws: dict = {}
 
 
-def _jacobi_pays(n: int, n_mat: int) -> bool:
-    """Batched Jacobi (all matrices concurrently, ~0.8 ms * (n/36)^3 on MI355X)
-    against the library eigensolver (~0.3 + 0.02 n ms per matrix, serial)."""
-    return 0.8 * (n / 36.0) ** 3 < n_mat * (0.3 + 0.02 * n)
+#: Batched on-chip eigensolvers are latency-bound per launch but run all
+#: matrices of a batch concurrently; the library one runs them in turn. Cost
+#: models (ms, measured on MI355X) pick the cheapest for (n, batch).
+def _cost_jacobi(n: int) -> float:
+    return 0.8 * (n / 36.0) ** 2.4 if n <= JACOBI_MAX_N else float("inf")
+
+
+def _cost_ql(n: int) -> float:
+    return 1.1 * (n / 36.0) ** 1.95 if n <= SMALL_QL_MAX_N else float("inf")
+
+
+def _cost_library(n: int, n_mat: int) -> float:
+    return n_mat * (0.35 + 0.02 * n)
+
+
+def small_method(n: int, n_mat: int) -> str:
+    """'jacobi', 'ql' or 'library' for a batch of n_mat matrices of size n."""
+    costs = {"jacobi": _cost_jacobi(n), "ql": _cost_ql(n), "library": _cost_library(n, n_mat)}
+    return min(costs, key=costs.get)
+
+
+def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch.Tensor]:
+    """Batched on-chip eigensolve of (..., n, n) complex128 on the GPU:
+    method "ql" (Householder + QL, n <= 128) or "jacobi" (n <= 96)."""
+    n = H.shape[-1]
+    batch = H.shape[:-2]
+    Hf = _fp64(H).reshape((-1, n, n)).contiguous()
+    B = Hf.shape[0]
+    E = torch.empty((B, n), dtype=torch.float64, device=H.device)
+    V = torch.empty((B, n, n), dtype=torch.complex128, device=H.device)
+    if method == "jacobi":
+        need = int(_lib.lib().qmx_heevj_workspace_bytes(n, B))
+        ws = torch.empty(max(need, 1), dtype=torch.uint8, device=H.device)
+        _lib.call("qmx_heevj_batched", Hf.data_ptr(), n, n, n * n, B, E.data_ptr(), V.data_ptr(),
+                  n, n * n, ws.data_ptr(), need, 0, rc.stream_ptr())
+    else:
+        need = int(_lib.lib().qmx_heev_small_workspace_bytes(n, B))
+        ws = torch.empty(max(need, 1), dtype=torch.uint8, device=H.device)
+        info = torch.zeros(B, dtype=torch.int32, device=H.device)
+        _lib.call("qmx_heev_small_batched", Hf.data_ptr(), n, n, n * n, B, E.data_ptr(),
+                  V.data_ptr(), n, n * n, info.data_ptr(), ws.data_ptr(), need, rc.stream_ptr())
+        # QL non-convergence is checked lazily by callers that synchronise anyway.
+        eigh_small.last_info = info
+    return E.reshape(batch + (n,)), V.reshape(batch + (n, n))
 
 
 def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Hermitian eigendecomposition in FP64 (lower triangle, ascending), like
-    torch.linalg.eigh. On the GPU: n <= JACOBI_MAX_N runs the whole batch in
-    one batched Jacobi launch; HETRD_MIN_N <= n <= HETRD_MAX_N reduces each
+    torch.linalg.eigh. On the GPU: n <= SMALL_QL_MAX_N runs the whole batch
+    in one batched Jacobi or Householder + QL launch where the cost model
+    says it beats the library; HETRD_MIN_N <= n <= HETRD_MAX_N reduces each
     matrix to tridiagonal form in one cooperative launch (qmx_hetrd) and
     finishes with rocSOLVER's tridiagonal divide-and-conquer; other sizes use
     the library eigensolver."""
@@ -92,16 +135,13 @@ def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     n_mat = 1
     for b in batch:
         n_mat *= int(b)
-    if n <= JACOBI_MAX_N and _jacobi_pays(n, n_mat):
-        Hf = H.reshape((-1, n, n)).contiguous()
-        B = Hf.shape[0]
-        E = torch.empty((B, n), dtype=torch.float64, device=H.device)
-        V = torch.empty((B, n, n), dtype=torch.complex128, device=H.device)
-        need = int(_lib.lib().qmx_heevj_workspace_bytes(n, B))
-        ws = torch.empty(max(need, 1), dtype=torch.uint8, device=H.device) if need else None
-        _lib.call("qmx_heevj_batched", Hf.data_ptr(), n, n, n * n, B, E.data_ptr(), V.data_ptr(),
-                  n, n * n, ws.data_ptr() if ws is not None else None, need, 0, rc.stream_ptr())
-        return E.reshape(batch + (n,)), V.reshape(batch + (n, n))
+    if n <= SMALL_QL_MAX_N:
+        method = small_method(n, n_mat)
+        if method != "library":
+            E, V = eigh_small(H, method)
+            if method == "ql" and bool((eigh_small.last_info != 0).any()):
+                return torch.linalg.eigh(H)  # QL did not converge on some matrix
+            return E, V
     if not (HETRD_MIN_N <= n <= HETRD_MAX_N):
         return torch.linalg.eigh(H)
     if _rocsolver is None:
