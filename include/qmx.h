@@ -44,6 +44,7 @@ typedef struct qmx_geom {
   int32_t n_tot;             /* padded basis length */
   const int32_t* n;          /* [nk] in-sphere counts */
   const int64_t* fft_index;  /* [nk * n_tot] linear box indices */
+  const int32_t* v_slot;     /* [nk] potential box used by each k, or NULL (all 0) */
 } qmx_geom;
 
 typedef struct qmx_plan qmx_plan;
@@ -84,13 +85,26 @@ int qmx_plan_info(const qmx_plan* plan, int64_t* info);
  * i.e. _apply_potential (dft/electrons/_basis_ops.py:39-113, scatter,
  * ifftn(norm="forward"), *= V, fftn(norm="forward"), gather, constrain)
  * fused with _apply_ke (_basis_ops.py:29-37) when ke != NULL.
- * V: real box (float for QMX_C64, double for QMX_C128), shared by all k.
- * ke: [nk * ke_ldk] real of the same precision, or NULL. out may equal C. */
+ * V: real boxes (float for QMX_C64, double for QMX_C128), N = box size
+ * each; k uses box v_slot[k] (geom.v_slot; all k use box 0 when it was
+ * NULL). ke: [nk * ke_ldk] real of the same precision, or NULL. out may
+ * equal C. */
 int qmx_apply_local(const qmx_plan* plan, int dtype, const void* C, int64_t ldb,
                     int64_t ldk, void* out, int64_t out_ldb, int64_t out_ldk,
                     const void* V, const void* ke, int64_t ke_ldk, int64_t n_bands,
                     int accumulate, void* workspace, int64_t workspace_bytes,
                     void* stream);
+
+/* qmx_apply_local on the k-points [k0, k0 + nk) of the plan only: C, out
+ * and ke start at k-point k0 (their k-strides as above), V is the same slot
+ * array. A plan over the k-points of many cells (one potential slot per
+ * cell) then serves both a batched apply over every cell and the apply for
+ * one cell. Same semantics and reference as qmx_apply_local. */
+int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,
+                      int64_t ldb, int64_t ldk, void* out, int64_t out_ldb, int64_t out_ldk,
+                      const void* V, const void* ke, int64_t ke_ldk, int64_t n_bands,
+                      int accumulate, void* workspace, int64_t workspace_bytes,
+                      void* stream);
 
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
@@ -165,12 +179,13 @@ int qmx_heevj_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a, i
                       double* E, void* V, int64_t ldv, int64_t stride_v, void* workspace,
                       int64_t workspace_bytes, int max_sweeps, void* stream);
 
-/* Batched Hermitian eigensolver for n <= 128, one workgroup per matrix:
+/* Batched Hermitian eigensolver for n <= 256, one workgroup per matrix:
  * Householder tridiagonalisation, implicit-shift QL with eigenvectors, and the
  * back-transformation V = Q Z, all inside one launch (the same role as
  * qmx_heevj_batched, for larger n). Same input and output conventions;
  * info[b] != 0 marks a matrix whose QL iteration did not converge.
- * Workspace: qmx_heev_small_workspace_bytes(n, batch) bytes. */
+ * Workspace: qmx_heev_small_workspace_bytes(n, batch) bytes (the QL
+ * eigenvector accumulator moves from LDS to the workspace above n = 128). */
 int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch);
 int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a,
                            int64_t batch, double* E, void* V, int64_t ldv, int64_t stride_v,

@@ -30,6 +30,7 @@ class QmxGeom(ct.Structure):
         ("n_tot", ct.c_int32),
         ("n", c_i32p),
         ("fft_index", c_i64p),
+        ("v_slot", c_i32p),
     ]
 
 
@@ -43,6 +44,10 @@ SIGNATURES = {
     "qmx_apply_local": (
         i32,
         [vp, i32, vp, i64, i64, vp, i64, i64, vp, vp, i64, i64, i32, vp, i64, vp],
+    ),
+    "qmx_apply_local_k": (
+        i32,
+        [vp, i64, i64, i32, vp, i64, i64, vp, i64, i64, vp, vp, i64, i64, i32, vp, i64, vp],
     ),
     "qmx_fft3d": (i32, [i32, c_i32p, vp, i64, i32, ct.c_double, vp]),
     "qmx_band_reduce": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, i64, vp, vp, vp]),
@@ -99,7 +104,8 @@ def call(name: str, *args) -> int:
     return status
 
 
-def make_geom(shape, n: np.ndarray, fft_index: np.ndarray) -> tuple[QmxGeom, tuple]:
+def make_geom(shape, n: np.ndarray, fft_index: np.ndarray,
+              v_slot: np.ndarray | None = None) -> tuple[QmxGeom, tuple]:
     """Geometry struct over host arrays (returned tuple keeps them alive)."""
     n32 = np.ascontiguousarray(n, dtype=np.int32)
     fi = np.ascontiguousarray(fft_index, dtype=np.int64)
@@ -111,7 +117,13 @@ def make_geom(shape, n: np.ndarray, fft_index: np.ndarray) -> tuple[QmxGeom, tup
     g.n_tot = n_tot
     g.n = n32.ctypes.data_as(c_i32p)
     g.fft_index = fi.ctypes.data_as(c_i64p)
-    return g, (n32, fi)
+    vs = None
+    if v_slot is not None:
+        vs = np.ascontiguousarray(v_slot, dtype=np.int32)
+        if vs.shape != (nk,):
+            raise ValueError("v_slot needs one entry per k-point")
+        g.v_slot = vs.ctypes.data_as(c_i32p)
+    return g, (n32, fi, vs)
 
 
 def geom_tables(shape, n: np.ndarray, fft_index: np.ndarray) -> dict:

@@ -1,0 +1,650 @@
+"""Batched band solves of many independent cells on one GPU.
+
+QimPy solves one cell at a time: each request below is one fixed-Hamiltonian
+band solve (all k-points and bands of a cell) with the Davidson iteration of
+dft/electrons/_davidson.py:141-235 and H of dft/electrons/_hamiltonian.py:7-29.
+This module runs many such solves together.
+
+Layout. Jobs (one cell each) that share an FFT box and a band class form a
+bucket, which stacks the k-points of all its jobs along one axis:
+    X  = [C; R]     (Kb, 2 nb, Gb)  bands, then their preconditioned residuals
+    HX = [HC; HR]   (Kb, 2 nb, Gb)  H applied to both
+    eig             (Kb, nb)        float64 Ritz values
+    beta, D         (Kb, Pb, Gb), (Kb, Pb)  projectors / couplings, zero-padded
+    V               (J, N)          one potential box per job (plan slot)
+with Gb = the largest basis of the bucket's k-points, Pb = the largest
+projector count and nb the band class (>= n_bands + n_bands_extra of every
+job). Every pass of an iteration is then one launch over all jobs of the
+bucket: the sparse FFT through a plan whose k-points carry the potential
+slot of their job, the overlaps, rotations and nonlocal products as batched
+GEMMs over Kb, and the subspace eigensolves as one batched on-chip launch.
+
+Iteration. The reference Davidson step with every band expanded in every
+iteration (the reference's locking counter n_eigs_done stays 0), so shapes
+stay fixed while jobs join and leave between iterations. Each job converges
+on its own n_bands: the plain (complex128) form by the reference's test,
+eigenvalue change below eig_threshold (_davidson.py:126-139); the mixed
+(complex64) form by residual norms ||HC - eig C|| below residual_threshold,
+then an FP64 Rayleigh-Ritz (complex128 H C and eighg) gives its eigenvalues.
+A job's start is the reference's: xorshift bands with its own seed and basis
+(_wavefunction_init.py:27-125), Cholesky orthonormalisation and a subspace
+rotation, so its result does not depend on the other jobs of its bucket.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import time
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib, gemm, rc
+from .basis import DTYPE_CODE, REAL_OF
+from .ions import build_projectors
+from .linalg import dagger, eigh, eighg, eighg_robust, ortho_matrix
+from .systems import signed_freqs
+
+C64, C128 = torch.complex64, torch.complex128
+
+#: Largest sparse-FFT workspace of one bucket; bands beyond it are blocked.
+WORKSPACE_CAP = 4 << 30
+
+
+# ---------------------------------------------------------------- geometry
+def sphere_tables(Rbasis: np.ndarray, k: np.ndarray, ke_cutoff: float, shape):
+    """Plane waves with KE(G + k) < ke_cutoff of each k (_basis.py:167-194):
+    per k, the box linear indices in increasing order and the integer
+    G-vectors. Only the bounding box of the sphere is enumerated:
+    |iG_a + k_a| = |R_a . G| / 2 pi < |R_a| sqrt(2 ke_cutoff) / 2 pi."""
+    Rbasis = np.asarray(Rbasis, dtype=np.float64)
+    Gbasis = 2 * np.pi * np.linalg.inv(Rbasis.T)
+    gmax = np.sqrt(2.0 * ke_cutoff)
+    shape = tuple(int(s) for s in shape)
+    out = []
+    for kk in np.asarray(k, dtype=np.float64).reshape(-1, 3):
+        axes = []
+        for a in range(3):
+            m = gmax * np.linalg.norm(Rbasis[:, a]) / (2 * np.pi)
+            f = signed_freqs(shape[a])
+            lo = max(int(np.ceil(-m - kk[a])), int(f.min()))
+            hi = min(int(np.floor(m - kk[a])), int(f.max()))
+            axes.append(np.arange(lo, hi + 1))
+        mesh = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+        Gk = (mesh + kk) @ Gbasis.T
+        sel = mesh[0.5 * np.einsum("gi,gi->g", Gk, Gk) < ke_cutoff]
+        lin = ((sel[:, 0] % shape[0]) * shape[1] + (sel[:, 1] % shape[1])) * shape[2] + (
+            sel[:, 2] % shape[2])
+        order = np.argsort(lin, kind="stable")
+        out.append((lin[order], sel[order]))
+    return out
+
+
+def choose_band_classes(n_max_bands: Sequence[int], overhead_bands: float = 12.0,
+                        granule: int = 8) -> list[int]:
+    """Band classes (ascending) for jobs needing `n_max_bands` bands each.
+
+    A job runs padded to its class; every class is a separate bucket whose
+    iteration costs a fixed launch and synchronisation overhead, worth about
+    `overhead_bands` bands of work. Classes are multiples of `granule`;
+    dynamic programming over the sorted demands minimises
+    sum(class size over its jobs) + overhead_bands * (number of classes)."""
+    if not len(n_max_bands):
+        return []
+    cand = sorted({int(np.ceil(n / granule)) * granule for n in n_max_bands})
+    need = sorted(int(n) for n in n_max_bands)
+    # cost[i][j]: jobs with demand in (cand[i-1], cand[j]] padded to cand[j].
+    m = len(cand)
+    best = [0.0] + [np.inf] * m
+    choice = [-1] * (m + 1)
+    for j in range(1, m + 1):
+        for i in range(j):
+            lo = cand[i - 1] if i else 0
+            cnt = sum(1 for n in need if lo < n <= cand[j - 1])
+            c = best[i] + cnt * cand[j - 1] + overhead_bands
+            if c < best[j]:
+                best[j], choice[j] = c, i
+    classes = []
+    j = m
+    while j > 0:
+        classes.append(cand[j - 1])
+        j = choice[j]
+    return sorted(classes)
+
+
+# ---------------------------------------------------------------- requests
+@dataclass
+class CellResult:
+    job_id: int
+    eig: np.ndarray  #: float64 (K, n_bands)
+    residual_norm: np.ndarray  #: float64 (K, n_bands)
+    Eband: float
+    iterations: int
+    converged: bool
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class CellJob:
+    """One band-solve request: a cell, its k-points and its bands."""
+
+    def __init__(self, spec, job_id: int = 0, arrival: float = 0.0, seed: int = 0):
+        self.spec = spec
+        self.job_id = job_id
+        self.arrival = arrival
+        self.seed = seed
+        self.shape = tuple(int(s) for s in spec.shape)
+        self.N = int(np.prod(self.shape))
+        self.K = int(np.asarray(spec.k).reshape(-1, 3).shape[0])
+        self.n_bands = int(spec.n_bands)
+        self.n_bands_max = int(spec.n_bands + spec.n_bands_extra)
+        tables = sphere_tables(spec.Rbasis, spec.k, spec.ke_cutoff, self.shape)
+        self.n = np.array([len(t[0]) for t in tables], dtype=np.int64)
+        self.n_tot = int(self.n.max())
+        if 2 * self.n_bands_max >= int(self.n.min()):  # _davidson.py:159-163
+            raise ValueError(f"n_bands + n_bands_extra = {self.n_bands_max} exceeds"
+                             f" min(n_basis)/2 = {int(self.n.min()) // 2} in Davidson")
+        self.fft_index = np.zeros((self.K, self.n_tot), dtype=np.int64)
+        self.iG_np = np.zeros((self.K, self.n_tot, 3), dtype=np.int64)
+        for i, (lin, sel) in enumerate(tables):
+            self.fft_index[i, : len(lin)] = lin
+            self.iG_np[i, : len(lin)] = sel
+        self.P = int(spec.n_projectors)
+        self.wk = np.asarray(spec.wk, dtype=np.float64)
+        self.iterations = 0
+        self.admitted = None
+        self.result: Optional[CellResult] = None
+        self._dev = None
+
+    def device_tables(self) -> dict:
+        """iG, k, KE and the potential on the device (built once)."""
+        if self._dev is None:
+            dev = rc.device
+            k = torch.tensor(np.asarray(self.spec.k, dtype=np.float64).reshape(-1, 3), device=dev)
+            iG = torch.tensor(self.iG_np, device=dev)
+            Gk = (iG.to(torch.float64) + k[:, None, :]) @ torch.tensor(self.spec.Gbasis.T, device=dev)
+            ke = 0.5 * Gk.square().sum(-1)
+            ke[torch.arange(self.n_tot, device=dev)[None, :] >= torch.tensor(self.n, device=dev)[:, None]] = 0.0
+            V = torch.tensor(np.asarray(self.spec.V, dtype=np.float64).reshape(-1), device=dev)
+            self._dev = dict(k=k, iG=iG, ke=ke, V=V,
+                             pos=torch.tensor(np.asarray(self.spec.positions, dtype=np.float64),
+                                              device=dev),
+                             D=torch.tensor(np.asarray(self.spec.D, dtype=np.float64), device=dev))
+        return self._dev
+
+    def release(self) -> None:
+        self._dev = None
+
+    def projectors(self, G: int) -> torch.Tensor:
+        """beta (K, P, G) complex128, zero beyond each k's sphere."""
+        t = self.device_tables()
+        iG = t["iG"]
+        if G > self.n_tot:
+            iG = torch.nn.functional.pad(iG, (0, 0, 0, G - self.n_tot))
+        return build_projectors(t["pos"], self.spec.proj_rc, self.spec.proj_l, iG[:, :G], t["k"],
+                                self.spec.Gbasis, self.spec.volume,
+                                torch.tensor(self.n, device=rc.device))
+
+    def hbm_bytes(self, nb: int, dtype: torch.dtype, G: Optional[int] = None) -> int:
+        """Device bytes this job holds inside a bucket of band class nb."""
+        G = G or self.n_tot
+        cs = torch.empty((), dtype=dtype).element_size()
+        state = 2 * 2 * (2 * nb) * G * cs * self.K  # X and HX, double-buffered
+        temps = 3 * nb * G * cs * self.K + 16 * self.P * G * self.K  # GEMM outputs, beta c128
+        return int(state + temps + self.P * G * cs * self.K + 12 * self.N)
+
+
+# ---------------------------------------------------------------- raw kernels
+def _c(x: torch.Tensor) -> int:
+    return x.data_ptr()
+
+
+def _band_ke(X: torch.Tensor, ke: torch.Tensor) -> torch.Tensor:
+    K, B, G = X.shape
+    norm2 = torch.empty((K, B), dtype=torch.float64, device=X.device)
+    kin = torch.empty_like(norm2)
+    _lib.call("qmx_band_reduce", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K, B, G,
+              _c(ke), ke.stride(0), _c(norm2), _c(kin), rc.stream_ptr())
+    return kin
+
+
+def _band_norm(X: torch.Tensor) -> torch.Tensor:
+    K, B, G = X.shape
+    norm2 = torch.empty((K, B), dtype=torch.float64, device=X.device)
+    _lib.call("qmx_band_reduce", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K, B, G,
+              None, 0, _c(norm2), None, rc.stream_ptr())
+    return norm2.sqrt()
+
+
+def _scale_constrain(X: torch.Tensor, scale: Optional[torch.Tensor], n32: torch.Tensor) -> None:
+    K, B, G = X.shape
+    s = None if scale is None else scale.to(torch.float64).contiguous()
+    _lib.call("qmx_scale_constrain", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K, B,
+              G, None if s is None else _c(s), _c(n32), rc.stream_ptr())
+
+
+def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """conj(A) @ B^T over the basis axis, batched over k."""
+    return torch.matmul(A.conj(), B.transpose(-2, -1))
+
+
+# ---------------------------------------------------------------- buckets
+class Bucket:
+    """Jobs sharing an FFT box and a band class, iterated in lockstep."""
+
+    def __init__(self, shape, nb: int, dtype: torch.dtype, *, residual_threshold: float,
+                 eig_threshold: float, max_iterations: int, stream=None):
+        self.shape = tuple(int(s) for s in shape)
+        self.N = int(np.prod(self.shape))
+        self.nb = int(nb)
+        self.dtype = dtype
+        self.rdtype = REAL_OF[dtype]
+        self.mixed = dtype == C64
+        self.residual_threshold = residual_threshold
+        self.eig_threshold = eig_threshold
+        self.max_iterations = max_iterations
+        self.stream = stream
+        self.jobs: list[CellJob] = []
+        self.k0: list[int] = []
+        self._plan = None
+        self.Kb = 0
+        self.Gb = 0
+        self.Pb = 0
+        self.n_ticks = 0
+
+    # ------------------------------------------------------------ structure
+    def __del__(self):
+        self._destroy_plan()
+
+    def _destroy_plan(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            try:
+                _lib.lib().qmx_plan_destroy(plan)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+        self._plan = None
+
+    def _restack(self, keep: list[CellJob], new: list[CellJob]) -> None:
+        """Rebuild the stacked tensors for jobs `keep` (state carried over)
+        followed by `new` (state zero, to be initialised)."""
+        dev = rc.device
+        nb, dt = self.nb, self.dtype
+        jobs = keep + new
+        if not jobs:
+            self.jobs, self.k0, self.Kb = [], [], 0
+            for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "norm", "beta", "D",
+                         "ke", "ke_r", "n32", "V", "V64"):
+                setattr(self, name, None)
+            self._destroy_plan()
+            return
+        Gb = max(j.n_tot for j in jobs)
+        Pb = max(j.P for j in jobs)
+        Kb = sum(j.K for j in jobs)
+        Kkeep = sum(j.K for j in keep)
+        X = torch.zeros((Kb, 2 * nb, Gb), dtype=dt, device=dev)
+        HX = torch.zeros_like(X)
+        eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
+        deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
+        if keep:
+            idx = torch.cat([torch.arange(self.k0[self.jobs.index(j)],
+                                          self.k0[self.jobs.index(j)] + j.K) for j in keep]).to(dev)
+            g = min(Gb, self.Gb)
+            X[:Kkeep, :, :g] = self.X.index_select(0, idx)[:, :, :g]
+            HX[:Kkeep, :, :g] = self.HX.index_select(0, idx)[:, :, :g]
+            eig[:Kkeep] = self.eig.index_select(0, idx)
+            deig[:Kkeep] = self.deig.index_select(0, idx)
+        k0, acc = [], 0
+        for j in jobs:
+            k0.append(acc)
+            acc += j.K
+        ke = torch.zeros((Kb, Gb), dtype=torch.float64, device=dev)
+        beta = torch.zeros((Kb, Pb, Gb), dtype=dt, device=dev)
+        D = torch.zeros((Kb, Pb), dtype=self.rdtype, device=dev)
+        V64 = torch.empty((len(jobs), self.N), dtype=torch.float64, device=dev)
+        fi = np.zeros((Kb, Gb), dtype=np.int64)
+        v_slot = np.zeros(Kb, dtype=np.int32)
+        for i, (j, a) in enumerate(zip(jobs, k0)):
+            t = j.device_tables()
+            ke[a : a + j.K, : j.n_tot] = t["ke"]
+            if j.P:
+                beta[a : a + j.K, : j.P] = j.projectors(Gb).to(dt)
+                D[a : a + j.K, : j.P] = t["D"].to(self.rdtype)
+            V64[i] = t["V"]
+            fi[a : a + j.K, : j.n_tot] = j.fft_index
+            v_slot[a : a + j.K] = i
+        n = np.concatenate([j.n for j in jobs])
+        self._destroy_plan()
+        geom, keepalive = _lib.make_geom(self.shape, n, fi, v_slot)
+        handle = ct.c_void_p()
+        _lib.call("qmx_plan_create", ct.byref(geom), ct.byref(handle))
+        del keepalive
+        self._plan = handle
+        self.jobs, self.k0, self.Kb, self.Gb, self.Pb = jobs, k0, Kb, Gb, Pb
+        self.X, self.HX, self.eig, self.deig = X, HX, eig, deig
+        self.X2, self.HX2 = torch.empty_like(X), torch.empty_like(HX)
+        self.ke = ke
+        self.ke_r = ke.to(self.rdtype)
+        self.n32 = torch.tensor(n, dtype=torch.int32, device=dev)
+        self.beta, self.D = beta, D
+        self.V64 = V64
+        self.V = V64 if dt == C128 else V64.to(self.rdtype)
+        self.rnorm = None
+        self.norm = None
+        need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], nb))
+        one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], 1))
+        size = max(one, min(need, WORKSPACE_CAP))
+        self.workspace = torch.empty(size, dtype=torch.uint8, device=dev)
+
+    # ------------------------------------------------------------ operators
+    def hamiltonian(self, Cin: torch.Tensor, out: torch.Tensor, k0: int, nk: int,
+                    beta: Optional[torch.Tensor] = None, D: Optional[torch.Tensor] = None) -> None:
+        """out = H Cin for the bucket k-points [k0, k0 + nk) (Cin, out
+        (nk, B, Gb) row-contiguous views, any precision)."""
+        B = Cin.shape[1]
+        dt = Cin.dtype
+        V = self.V64 if dt == C128 else self.V
+        ke = self.ke if dt == C128 else self.ke_r
+        _lib.call("qmx_apply_local_k", self._plan, k0, nk, DTYPE_CODE[dt], _c(Cin), Cin.stride(1),
+                  Cin.stride(0), _c(out), out.stride(1), out.stride(0), _c(V), _c(ke[k0:]),
+                  ke.stride(0), B, 0, _c(self.workspace), self.workspace.numel(), rc.stream_ptr())
+        if self.Pb:
+            if beta is None:
+                beta, D = self.beta[k0 : k0 + nk], self.D[k0 : k0 + nk]
+            proj = _dot(beta, Cin)  # (nk, P, B)
+            proj *= D[:, :, None].to(proj.real.dtype)
+            out.baddbmm_(proj.transpose(-2, -1), beta)
+
+    # ------------------------------------------------------------ job start
+    def _initialise(self, new: list[CellJob]) -> None:
+        """Random start, orthonormalisation and subspace rotation of the new
+        jobs' rows (_electrons.py:271-287, _davidson.py:164-169), in FP64:
+        bandwidth-limited random bands span their space with a condition
+        number far beyond float32, so the mixed form casts only the
+        resulting orthonormal Ritz vectors to complex64."""
+        nb = self.nb
+        for j, a in zip(self.jobs, self.k0):
+            if j not in new:
+                continue
+            C = torch.zeros((j.K, nb, self.Gb), dtype=C128, device=rc.device)
+            # Seeds follow the job's own basis (n_max = its n_tot):
+            _lib.call("qmx_randomize", DTYPE_CODE[C128], _c(C), C.stride(1), C.stride(0), j.K,
+                      self.Gb, j.n_tot, 0, _c(self.n32[a:]), _c(self.ke[a:]), self.ke.stride(0),
+                      j.seed, 0, nb, rc.stream_ptr())
+            C = gemm.rotate(ortho_matrix(_dot(C, C)), C)
+            HC = torch.empty_like(C)
+            if self.dtype == C128:
+                self.hamiltonian(C, HC, a, j.K)
+            else:
+                self.hamiltonian(C, HC, a, j.K, j.projectors(self.Gb) if j.P else None,
+                                 j.device_tables()["D"][None, :].expand(j.K, -1) if j.P else None)
+            E, Vr = eigh(_dot(C, HC))
+            self.X[a : a + j.K, :nb] = gemm.rotate(Vr, C)
+            self.HX[a : a + j.K, :nb] = gemm.rotate(Vr, HC)
+            self.eig[a : a + j.K] = E
+            j.admitted = time.perf_counter()
+
+    # ------------------------------------------------------------ iteration
+    def residuals(self) -> None:
+        """R = precondition(HC - C eig) into X[:, nb:] with norms (_davidson.py:182-184)."""
+        nb = self.nb
+        C, HC, R = self.X[:, :nb], self.HX[:, :nb], self.X[:, nb:]
+        KEref = _band_ke(C, self.ke)
+        norm2 = torch.empty((self.Kb, nb), dtype=torch.float64, device=rc.device)
+        rnorm2 = torch.empty_like(norm2)
+        eig = self.eig.contiguous()
+        _lib.call("qmx_precond_residual", DTYPE_CODE[self.dtype], _c(C), _c(HC), C.stride(1),
+                  C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0), _c(R),
+                  R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
+                  rc.stream_ptr())
+        self.norm = norm2.sqrt()
+        self.rnorm = rnorm2.sqrt()
+
+    def converged_jobs(self, rnorm_host: np.ndarray, deig_host: np.ndarray) -> list[CellJob]:
+        done = []
+        for j, a in zip(self.jobs, self.k0):
+            rows = slice(a, a + j.K)
+            if self.mixed:
+                ok = rnorm_host[rows, : j.n_bands].max() < self.residual_threshold
+            else:
+                ok = deig_host[rows, : j.n_bands].max() < self.eig_threshold
+            if ok or j.iterations >= self.max_iterations:
+                j.converged = bool(ok)
+                done.append(j)
+        return done
+
+    def _project_out(self, C: torch.Tensor, R: torch.Tensor, seed: int) -> torch.Tensor:
+        """R <- (1 - C C^H) R twice, re-drawing directions that vanish in
+        float32 (as Davidson._project_out); returns the new norms."""
+        for attempt in range(2):
+            for _ in range(2):
+                R.baddbmm_(_dot(C, R).transpose(-2, -1), C, alpha=-1)
+            norm = _band_norm(R)
+            low = norm < 1e-3
+            if attempt or not bool(low.any()):
+                break
+            self._redraw(R, low, seed)
+            _scale_constrain(R, 1.0 / _band_norm(R), self.n32)
+        return norm
+
+    def _redraw(self, R: torch.Tensor, mask: torch.Tensor, seed: int) -> None:
+        """Replace the selected (k, band) rows of R by bandwidth-limited random
+        vectors (the role of randomize_selected, _wavefunction_init.py:128-188)."""
+        i_k, i_b = torch.where(mask)
+        gen = torch.Generator(device=rc.device).manual_seed(int(seed))
+        G = self.Gb
+        re = torch.randn((len(i_k), G), generator=gen, device=rc.device, dtype=torch.float64)
+        im = torch.randn((len(i_k), G), generator=gen, device=rc.device, dtype=torch.float64)
+        ke = self.ke[i_k]
+        vals = torch.complex(re, im) / (1.0 + ((4.0 / 3) * ke) ** 6)
+        vals[torch.arange(G, device=rc.device)[None, :] >= self.n32[i_k].long()[:, None]] = 0.0
+        R[i_k, i_b] = vals.to(R.dtype)
+
+    def expand(self) -> None:
+        """One Davidson step on every row (_davidson.py:180-233, no locking)."""
+        nb = self.nb
+        C, R = self.X[:, :nb], self.X[:, nb:]
+        HC, HR = self.HX[:, :nb], self.HX[:, nb:]
+        norm = self.norm
+        low = norm < np.sqrt(self.Gb * 1e-15)  # _regularize, _davidson.py:56-58,101-115
+        if bool(low.any()):
+            self._redraw(R, low, 7919 * self.n_ticks + 1)
+            norm = torch.where(low, _band_norm(R), norm)
+        _scale_constrain(R, 1.0 / norm, self.n32)
+        if self.mixed:
+            norm = self._project_out(C, R, 104729 * self.n_ticks + 3)
+            _scale_constrain(R, 1.0 / norm.clamp_min(1e-300), self.n32)
+        self.hamiltonian(R, HR, 0, self.Kb)
+        XO = _dot(self.X, R).to(C128)  # (Kb, 2nb, nb): [C^H R; R^H R]
+        XH = _dot(self.X, HR).to(C128)
+        n2 = 2 * nb
+        O = torch.zeros((self.Kb, n2, n2), dtype=C128, device=rc.device)
+        H = torch.zeros_like(O)
+        O.diagonal(dim1=-2, dim2=-1)[:, :nb] = 1.0
+        H.diagonal(dim1=-2, dim2=-1)[:, :nb] = self.eig.to(C128)
+        O[:, :, nb:] = XO
+        H[:, :, nb:] = XH
+        O[:, nb:, :nb] = dagger(XO[:, :nb])
+        H[:, nb:, :nb] = dagger(XH[:, :nb])
+        E, Vs = (eighg_robust if self.mixed else eighg)(H, O)
+        Vs = Vs[:, :, :nb].to(self.dtype)
+        torch.matmul(Vs.transpose(-2, -1), self.X, out=self.X2[:, :nb])
+        torch.matmul(Vs.transpose(-2, -1), self.HX, out=self.HX2[:, :nb])
+        self.X, self.X2 = self.X2, self.X
+        self.HX, self.HX2 = self.HX2, self.HX
+        E = E[:, :nb]
+        self.deig = (E - self.eig).abs()
+        self.eig = E.contiguous()
+        for j in self.jobs:
+            j.iterations += 1
+        self.n_ticks += 1
+
+    # ------------------------------------------------------------ results
+    def finish(self, job: CellJob) -> CellResult:
+        """Result of a converged (or exhausted) job; the mixed form first
+        recomputes its subspace in FP64 (complex128 H C and eighg)."""
+        a = self.k0[self.jobs.index(job)]
+        rows = slice(a, a + job.K)
+        nb = self.nb
+        if self.mixed:
+            C = self.X[rows, :nb].to(C128)
+            HC = torch.empty_like(C)
+            beta = job.projectors(self.Gb) if job.P else None
+            D = job.device_tables()["D"][None, :].expand(job.K, -1) if job.P else None
+            self.hamiltonian(C, HC, a, job.K, beta, D)
+            E, Vs = eighg(_dot(C, HC), _dot(C, C))
+            C = gemm.rotate(Vs, C)
+            HC = gemm.rotate(Vs, HC)
+            resid = HC - C * E[:, :, None].to(C128)
+            rnorm = torch.linalg.vector_norm(resid, dim=-1)
+        else:
+            E = self.eig[rows]
+            rnorm = self.rnorm[rows] if self.rnorm is not None else torch.full_like(E, np.nan)
+        B = job.n_bands
+        eig = E[:, :B].cpu().numpy()
+        res = CellResult(
+            job_id=job.job_id, eig=eig, residual_norm=rnorm[:, :B].cpu().numpy(),
+            Eband=float(2.0 * (job.wk[:, None] * eig).sum()), iterations=job.iterations,
+            converged=getattr(job, "converged", False), arrival=job.arrival,
+            admitted=job.admitted, finished=time.perf_counter())
+        job.result = res
+        return res
+
+
+# ---------------------------------------------------------------- solver
+class BatchSolver:
+    """Runs band-solve jobs in buckets, admitting new jobs as others finish.
+
+    ``submit`` queues jobs; ``tick`` performs one iteration of every bucket:
+    residuals (one device-to-host read per bucket for the convergence
+    test), retirement of converged jobs, admission of queued jobs into the
+    freed capacity (HBM budget and per-bucket k-point limit) and one
+    Davidson step. ``run`` ticks until every submitted job is done.
+    """
+
+    def __init__(self, dtype: torch.dtype = C64, *, residual_threshold: float = 3e-4,
+                 eig_threshold: float = 1e-8, max_iterations: int = 200,
+                 band_classes: Optional[Sequence[int]] = None, hbm_budget: float = 200e9,
+                 max_k_per_bucket: int = 2048, n_streams: int = 1):
+        rc.require_gpu()
+        self.dtype = dtype
+        self.residual_threshold = residual_threshold
+        self.eig_threshold = eig_threshold
+        self.max_iterations = max_iterations
+        self.band_classes = sorted(band_classes) if band_classes else None
+        self.hbm_budget = hbm_budget
+        self.max_k_per_bucket = max_k_per_bucket
+        self.streams = [torch.cuda.Stream(device=rc.device) for _ in range(max(0, n_streams - 1))]
+        self.buckets: dict = {}
+        self.queue: list[CellJob] = []
+        self.results: dict[int, CellResult] = {}
+        self.hbm_in_use = 0
+        self._held: dict[int, int] = {}
+
+    def band_class(self, job: CellJob) -> int:
+        if self.band_classes:
+            for c in self.band_classes:
+                if c >= job.n_bands_max:
+                    return c
+        return int(np.ceil(job.n_bands_max / 8) * 8)
+
+    def submit(self, jobs: Sequence[CellJob]) -> None:
+        self.queue.extend(jobs)
+
+    def _bucket(self, key) -> Bucket:
+        b = self.buckets.get(key)
+        if b is None:
+            stream = self.streams[len(self.buckets) % len(self.streams)] if self.streams else None
+            b = Bucket(key[0], key[1], self.dtype, residual_threshold=self.residual_threshold,
+                       eig_threshold=self.eig_threshold, max_iterations=self.max_iterations,
+                       stream=stream)
+            self.buckets[key] = b
+        return b
+
+    def _admit(self, now: Optional[float]) -> dict:
+        """Take queued jobs (arrived by `now`) that fit; returns key -> jobs."""
+        taken: dict = {}
+        rest = []
+        for job in self.queue:
+            if now is not None and job.arrival > now:
+                rest.append(job)
+                continue
+            key = (job.shape, self.band_class(job))
+            b = self.buckets.get(key)
+            k_in = (b.Kb if b else 0) + sum(j.K for j in taken.get(key, []))
+            need = job.hbm_bytes(key[1], self.dtype)
+            if (self.hbm_in_use + need > self.hbm_budget and self.hbm_in_use > 0) or (
+                    k_in + job.K > self.max_k_per_bucket and k_in > 0):
+                rest.append(job)
+                continue
+            self.hbm_in_use += need
+            self._held[job.job_id] = need
+            taken.setdefault(key, []).append(job)
+        self.queue = rest
+        return taken
+
+    def tick(self, now: Optional[float] = None) -> list[CellResult]:
+        finished: list[CellResult] = []
+        active = [b for b in self.buckets.values() if b.jobs]
+        # 1. residuals of every active bucket, then one host read each.
+        for b in active:
+            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
+                b.residuals()
+        retire: dict = {}
+        for b in active:
+            rn = b.rnorm.cpu().numpy()
+            de = b.deig.cpu().numpy()
+            retire[id(b)] = b.converged_jobs(rn, de)
+        # 2. retire, admit, restack, start the new jobs, one Davidson step.
+        for b in active:
+            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
+                for job in retire[id(b)]:
+                    finished.append(b.finish(job))
+        newcomers = self._free_and_admit(retire, now)
+        for key in sorted(set(newcomers) | {(b.shape, b.nb) for b in active}):
+            b = self._bucket(key)
+            new = newcomers.get(key, [])
+            gone = retire.get(id(b), [])
+            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
+                if new or gone:
+                    b._restack([j for j in b.jobs if j not in gone], new)
+                    if not b.jobs:
+                        continue
+                    b._initialise(new)
+                    b.residuals()
+                if b.jobs:
+                    b.expand()
+        for res in finished:
+            self.results[res.job_id] = res
+        return finished
+
+    def _free_and_admit(self, retire: dict, now) -> dict:
+        for jobs in retire.values():
+            for job in jobs:
+                self.hbm_in_use -= self._held.pop(job.job_id, 0)
+                job.release()
+        return self._admit(now)
+
+    @property
+    def idle(self) -> bool:
+        return not self.queue and not any(b.jobs for b in self.buckets.values())
+
+    def run(self) -> dict[int, CellResult]:
+        while not self.idle:
+            self.tick()
+        torch.cuda.synchronize()
+        return self.results
+
+
+class _null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False

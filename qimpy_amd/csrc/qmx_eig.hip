@@ -455,21 +455,25 @@ extern "C" int qmx_heevj_batched(const void* A, int64_t n, int64_t lda, int64_t 
 }
 
 // ---------------------------------------------------------------------------
-// Batched Hermitian eigensolver for n <= 128, one workgroup per matrix:
+// Batched Hermitian eigensolver for n <= 256, one workgroup per matrix:
 //   1. Householder reduction to real tridiagonal form T = Q^H A Q (the matrix
 //      lives in a workspace and stays in the cache; reflector v_k is kept in
 //      column k below the subdiagonal, tau_k in LDS);
 //   2. implicit-shift QL on T (tql2-style): thread 0 runs the bulge chase of
 //      one QL iteration and logs its plane rotations in LDS, then every thread
-//      applies the logged rotations to its own row of Z (LDS, row stride
-//      n + 1), one barrier pair per QL iteration;
+//      applies the logged rotations to its own row of Z, one barrier pair per
+//      QL iteration. A chase rotates the consecutive planes (m-1, m), ...,
+//      (l, l+1), so the shared element of two successive rotations stays in
+//      a register. Z is in LDS (row stride n + 1) for n <= kSmallLdsN and in
+//      the workspace (row stride n, one row per thread, cached) above;
 //   3. ascending order by rank, then V = Q Z with one thread per column,
 //      applying H_{n-2} ... H_0 with the reflectors staged in LDS.
 // Output as qmx_heevj_batched; info[b] != 0 if QL did not converge.
 namespace {
 
 constexpr int kSmallThreads = 256;
-constexpr int kSmallMaxN = 128;
+constexpr int kSmallMaxN = 256;
+constexpr int kSmallLdsN = 128;
 
 template <int NW>
 __device__ __forceinline__ double block_sum_small(double x, double* red) {
@@ -484,18 +488,19 @@ __device__ __forceinline__ double block_sum_small(double x, double* red) {
   return s;
 }
 
+template <bool ZLDS>
 __global__ void __launch_bounds__(kSmallThreads)
     k_heev_small(const double2* __restrict__ A, int n, int64_t lda, int64_t stride_a,
                  double* __restrict__ E, double2* __restrict__ Vout, int64_t ldv,
                  int64_t stride_v, double2* __restrict__ Wws, int max_iter, int* __restrict__ info) {
   constexpr int NW = kSmallThreads / 64;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int ldz = n + 1;
+  const int ldz = ZLDS ? n + 1 : n;
   double2* v = reinterpret_cast<double2*>(smem_raw);
   double2* w = v + n;
   double2* tau = w + n;
-  double* Z = reinterpret_cast<double*>(tau + n);
-  double* d = Z + n * ldz;
+  double* Zl = reinterpret_cast<double*>(tau + n);
+  double* d = Zl + (ZLDS ? n * ldz : 0);
   double* e = d + n;
   double* lc = e + n;
   double* ls = lc + n;
@@ -506,7 +511,9 @@ __global__ void __launch_bounds__(kSmallThreads)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t b = blockIdx.x;
   const double2* Ab = A + b * stride_a;
-  double2* W = Wws + b * (int64_t)n * n;
+  double2* W = Wws + b * (int64_t)n * n * (ZLDS ? 1 : 2);
+  // Global Z: n * n doubles right after this matrix's n * n complex W.
+  double* Z = ZLDS ? Zl : reinterpret_cast<double*>(W + (int64_t)n * n);
 
   for (int idx = tid; idx < n * n; idx += kSmallThreads) {
     const int i = idx / n, j = idx - i * n;
@@ -607,6 +614,7 @@ __global__ void __launch_bounds__(kSmallThreads)
     const int i = idx / ldz, j = idx - i * ldz;
     Z[idx] = (i == j) ? 1.0 : 0.0;
   }
+  if (!ZLDS) __threadfence_block();  // global Z written by other threads of the group
   __syncthreads();
 
   // 2. Implicit-shift QL; thread 0 chases, everyone rotates its row of Z.
@@ -665,15 +673,19 @@ __global__ void __launch_bounds__(kSmallThreads)
       __syncthreads();
       const int nlog = ctl[0];
       const int done = ctl[1];
-      if (tid < n) {
-        double* zr = Z + tid * ldz;
+      if (tid < n && nlog > 0) {
+        // Rotation t acts on the planes (i, i + 1), i = li[0] - t.
+        double* zr = Z + (int64_t)tid * ldz;
+        const int ihi = li[0];
+        double carry = zr[ihi + 1];
         for (int t = 0; t < nlog; ++t) {
-          const int i = li[t];
+          const int i = ihi - t;
           const double c = lc[t], s = ls[t];
-          const double f = zr[i + 1];
-          zr[i + 1] = s * zr[i] + c * f;
-          zr[i] = c * zr[i] - s * f;
+          const double zi = zr[i];
+          zr[i + 1] = s * zi + c * carry;
+          carry = c * zi - s * carry;
         }
+        zr[ihi - nlog + 1] = carry;
       }
       __syncthreads();
       if (done) break;
@@ -696,7 +708,7 @@ __global__ void __launch_bounds__(kSmallThreads)
   double2* Vb = Vout + b * stride_v;
   for (int j = tid; j < n; j += kSmallThreads) {
     const int pj = perm[j];
-    for (int i = 0; i < n; ++i) Vb[(int64_t)i * ldv + j] = double2{Z[i * ldz + pj], 0.0};
+    for (int i = 0; i < n; ++i) Vb[(int64_t)i * ldv + j] = double2{Z[(int64_t)i * ldz + pj], 0.0};
   }
   for (int k = n - 2; k >= 0; --k) {
     __syncthreads();
@@ -728,7 +740,8 @@ __global__ void __launch_bounds__(kSmallThreads)
 
 extern "C" int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch) {
   if (n < 1 || batch < 0) return -1;
-  return batch * n * n * (int64_t)sizeof(double2);
+  const int64_t per = n * n * (int64_t)sizeof(double2) + (n > kSmallLdsN ? n * n * 8 : 0);
+  return batch * per;
 }
 
 extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a,
@@ -736,22 +749,30 @@ extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int
                                       int64_t stride_v, int* info, void* workspace,
                                       int64_t workspace_bytes, void* stream) {
   QMX_CHECK_ARG(A && E && V && workspace, "qmx_heev_small_batched: null argument");
-  QMX_CHECK_ARG(n >= 1 && n <= kSmallMaxN, "qmx_heev_small_batched: n must be in 1..128");
+  QMX_CHECK_ARG(n >= 1 && n <= kSmallMaxN, "qmx_heev_small_batched: n must be in 1..256");
   QMX_CHECK_ARG(lda >= n && ldv >= n && batch >= 0 && batch < (1ll << 31),
                 "qmx_heev_small_batched: bad strides or batch");
   QMX_CHECK_ARG(workspace_bytes >= qmx_heev_small_workspace_bytes(n, batch),
                 "qmx_heev_small_batched: workspace too small");
   if (batch == 0) return QMX_OK;
-  const size_t sh = (size_t)3 * n * sizeof(double2) + (size_t)n * (n + 1) * sizeof(double) +
+  const bool zlds = n <= kSmallLdsN;
+  const size_t sh = (size_t)3 * n * sizeof(double2) +
+                    (zlds ? (size_t)n * (n + 1) * sizeof(double) : 0) +
                     (size_t)4 * n * sizeof(double) + 4 * sizeof(double) +
                     (size_t)2 * n * sizeof(int) + 4 * sizeof(int);
-  QMX_HIP(hipFuncSetAttribute((const void*)k_heev_small, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)sh));
+  const void* fn = zlds ? (const void*)k_heev_small<true> : (const void*)k_heev_small<false>;
+  QMX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_heev_small, dim3((unsigned)batch), dim3(kSmallThreads), sh, st,
-                     static_cast<const double2*>(A), (int)n, lda, stride_a, E,
-                     static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace), 60,
-                     info);
+  if (zlds)
+    hipLaunchKernelGGL(k_heev_small<true>, dim3((unsigned)batch), dim3(kSmallThreads), sh, st,
+                       static_cast<const double2*>(A), (int)n, lda, stride_a, E,
+                       static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace),
+                       60, info);
+  else
+    hipLaunchKernelGGL(k_heev_small<false>, dim3((unsigned)batch), dim3(kSmallThreads), sh, st,
+                       static_cast<const double2*>(A), (int)n, lda, stride_a, E,
+                       static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace),
+                       60, info);
   QMX_LAUNCHED();
   return QMX_OK;
 }

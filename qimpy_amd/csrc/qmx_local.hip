@@ -113,6 +113,9 @@ struct qmx_plan {
   int* d_cstart;      // [nk][ncol + 1] first sphere point of each column
   int* d_t1idx;       // [nk][n_tot] col * n2 + z of each sphere point
   int* d_n;           // [nk]
+  int* d_vslot;       // [nk] potential box of each k
+  int64_t* h_nsum;    // [nk + 1] prefix sums of n[k] (host)
+  int n_vslots;       // 1 + largest slot
   float2* d_tw32[3];
   double2* d_tw64[3];
 };
@@ -201,9 +204,10 @@ __global__ void __launch_bounds__(kThreads)
 template <typename C2, typename R>
 __global__ void __launch_bounds__(kThreads)
     k_xpass_potential(C2* __restrict__ T2, const R* __restrict__ V,
-                      const int* __restrict__ x_list, const C2* __restrict__ tw_g, Radices rx,
-                      FastDiv fmw, FastDiv fM, int zshift, int nbg, int ny, int nz, int nxa,
-                      int nb, int bands_per_chunk, int n_chunks) {
+                      const int* __restrict__ vslot, const int* __restrict__ x_list,
+                      const C2* __restrict__ tw_g, Radices rx, FastDiv fmw, FastDiv fM,
+                      int zshift, int nbg, int ny, int nz, int nxa, int nb,
+                      int bands_per_chunk, int n_chunks) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = rx.n, M = fM.d, ld = odd_ld(M), Mz = 1 << zshift, zmask = Mz - 1;
   const int mw = fmw.d;
@@ -218,10 +222,11 @@ __global__ void __launch_bounds__(kThreads)
   const int bstop = min(nb, bstart + bands_per_chunk);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   load_twiddles(tw, tw_g, nx);
+  const R* Vk = V + (int64_t)vslot[k] * nx * ny * nz;
   for (int i = threadIdx.x; i < nx * Mz; i += blockDim.x) {
     const int x = i >> zshift;
     const int zl = i & zmask;
-    vt[i] = zl < mz ? V[((int64_t)x * ny + y) * nz + z0 + zl] : R(0);
+    vt[i] = zl < mz ? Vk[((int64_t)x * ny + y) * nz + z0 + zl] : R(0);
   }
   const int64_t plane = (int64_t)ny * nz;
   for (int g0 = bstart; g0 < bstop; g0 += nbg) {
@@ -459,13 +464,17 @@ const double2* tw_of<double2>(const qmx_plan* p, int axis) {
 }
 
 template <typename R>
-int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk, void* outv,
-                     int64_t oldb, int64_t oldk, const void* Vv, const void* kev,
+int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t ldb, int64_t ldk,
+                     void* outv, int64_t oldb, int64_t oldk, const void* Vv, const void* kev,
                      int64_t ke_ldk, int64_t nb, int accumulate, void* ws, int64_t ws_bytes,
                      hipStream_t st) {
   using C2 = typename Cplx<R>::T;
   const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
-  const int nk = p->nk, ncol = p->ncol, nxa = p->nxa;
+  const int ncol = p->ncol, nxa = p->nxa;
+  const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
+  const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
+  const int* nvec = p->d_n + k0;
+  const int* vslot = p->d_vslot + k0;
   const int64_t per_band = ((int64_t)ncol * nz + (int64_t)nxa * ny * nz) * sizeof(C2);
   int64_t blk = ws_bytes / (per_band * nk);
   QMX_CHECK_ARG(blk >= 1, "qmx_apply_local: workspace smaller than one band of every k");
@@ -505,7 +514,7 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
   const size_t shC = (size_t)(2 * odd_ld(Mx) * nx + nx) * cs + (size_t)nx * Mzc * sizeof(R);
   const R scale = R(1) / R(p->N);
   // Sphere points per band summed over k (for the byte counts):
-  const double n_sphere = (double)p->n_sum;
+  const double n_sphere = (double)(p->h_nsum[k0 + nk] - p->h_nsum[k0]);
   const double bc = (double)cs, br = (double)sizeof(R);
   const double t1_elems = (double)nk * ncol * nz, t2_elems = (double)nk * nxa * ny * nz;
   hipEvent_t ev = nullptr;
@@ -516,7 +525,7 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
     dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_sphere_to_cols<C2, R>), gA, dim3(kThreads), shA, st, C, ldb, ldk, T1,
-                       p->d_t1idx, p->d_cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                       t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
                        p->n_tot, (int)b0);
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_A, cur * (n_sphere * bc + t1_elems * bc));
@@ -537,7 +546,7 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
     n_chunks = (cur + bpc - 1) / bpc;
     dim3 gC((nz + Mzc - 1) / Mzc, ny, nk * n_chunks);
     prof_begin(st, &ev);
-    hipLaunchKernelGGL((k_xpass_potential<C2, R>), gC, dim3(kThreads), shC, st, T2, V,
+    hipLaunchKernelGGL((k_xpass_potential<C2, R>), gC, dim3(kThreads), shC, st, T2, V, vslot,
                        p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fxw, fxM, zshift, nbg, ny, nz,
                        nxa, cur, bpc, n_chunks);
     QMX_LAUNCHED();
@@ -551,7 +560,7 @@ int apply_local_impl(const qmx_plan* p, const void* Cv, int64_t ldb, int64_t ldk
     prof_end(st, ev, QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_cols_to_sphere<C2, R>), gA, dim3(kThreads), shA, st, T1, C, ldb, ldk,
-                       out, oldb, oldk, ke, ke_ldk, p->d_t1idx, p->d_cstart, p->d_n,
+                       out, oldb, oldk, ke, ke_ldk, t1idx, cstart, nvec,
                        tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
                        accumulate);
     QMX_LAUNCHED();
@@ -637,9 +646,11 @@ static int upload(T** dst, const std::vector<T>& src) {
 
 static void free_plan(qmx_plan* p) {
   if (!p) return;
-  int* arrs[] = {p->d_x_list, p->d_pcol_start, p->d_col_iy, p->d_cstart, p->d_t1idx, p->d_n};
+  int* arrs[] = {p->d_x_list, p->d_pcol_start, p->d_col_iy, p->d_cstart,
+                 p->d_t1idx,  p->d_n,          p->d_vslot};
   for (int* a : arrs)
     if (a) (void)hipFree(a);
+  delete[] p->h_nsum;
   delete p;
 }
 
@@ -766,13 +777,29 @@ int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
   p->n_tot = geom->n_tot;
   p->N = (int64_t)p->shape[0] * p->shape[1] * p->shape[2];
   p->nxa = (int)t.x_list.size();
-  p->n_sum = 0;
-  for (int k = 0; k < geom->nk; ++k) p->n_sum += geom->n[k];
+  p->h_nsum = new int64_t[geom->nk + 1];
+  p->h_nsum[0] = 0;
+  for (int k = 0; k < geom->nk; ++k) p->h_nsum[k + 1] = p->h_nsum[k] + geom->n[k];
+  p->n_sum = p->h_nsum[geom->nk];
   p->ncol = (int)t.col_iy.size();
   std::vector<int> nvec(geom->n, geom->n + geom->nk);
+  std::vector<int> vslot(geom->nk, 0);
+  p->n_vslots = 1;
+  if (geom->v_slot) {
+    for (int k = 0; k < geom->nk; ++k) {
+      if (geom->v_slot[k] < 0) {
+        free_plan(p);
+        set_error("qmx_plan_create: negative v_slot");
+        return QMX_ERR_ARG;
+      }
+      vslot[k] = geom->v_slot[k];
+      p->n_vslots = std::max(p->n_vslots, vslot[k] + 1);
+    }
+  }
   if ((rc = upload(&p->d_x_list, t.x_list)) || (rc = upload(&p->d_pcol_start, t.pcol_start)) ||
       (rc = upload(&p->d_col_iy, t.col_iy)) || (rc = upload(&p->d_cstart, t.cstart)) ||
-      (rc = upload(&p->d_t1idx, t.t1idx)) || (rc = upload(&p->d_n, nvec))) {
+      (rc = upload(&p->d_t1idx, t.t1idx)) || (rc = upload(&p->d_n, nvec)) ||
+      (rc = upload(&p->d_vslot, vslot))) {
     free_plan(p);
     return rc;
   }
@@ -811,26 +838,38 @@ int qmx_plan_info(const qmx_plan* p, int64_t* info) {
   return QMX_OK;
 }
 
+int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,
+                      int64_t ldb, int64_t ldk, void* out, int64_t out_ldb, int64_t out_ldk,
+                      const void* V, const void* ke, int64_t ke_ldk, int64_t n_bands,
+                      int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
+  QMX_CHECK_ARG(plan && C && out && V, "qmx_apply_local: null argument");
+  QMX_CHECK_ARG(n_bands >= 0, "qmx_apply_local: negative n_bands");
+  QMX_CHECK_ARG(k0 >= 0 && nk >= 0 && k0 + nk <= plan->nk, "qmx_apply_local: k range outside plan");
+  if (n_bands == 0 || nk == 0) return QMX_OK;
+  QMX_CHECK_ARG(workspace && workspace_bytes > 0, "qmx_apply_local: workspace required");
+  QMX_CHECK_ARG(ldb >= plan->n_tot && out_ldb >= plan->n_tot, "qmx_apply_local: ldb < n_tot");
+  QMX_CHECK_ARG(nk == 1 || (ldk > 0 && out_ldk > 0), "qmx_apply_local: bad k stride");
+  QMX_CHECK_ARG(!ke || nk == 1 || ke_ldk >= plan->n_tot, "qmx_apply_local: bad ke stride");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == QMX_C64)
+    return apply_local_impl<float>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk, V,
+                                   ke, ke_ldk, n_bands, accumulate, workspace, workspace_bytes,
+                                   st);
+  if (dtype == QMX_C128)
+    return apply_local_impl<double>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk,
+                                    V, ke, ke_ldk, n_bands, accumulate, workspace,
+                                    workspace_bytes, st);
+  set_error("qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
+  return QMX_ERR_ARG;
+}
+
 int qmx_apply_local(const qmx_plan* plan, int dtype, const void* C, int64_t ldb, int64_t ldk,
                     void* out, int64_t out_ldb, int64_t out_ldk, const void* V, const void* ke,
                     int64_t ke_ldk, int64_t n_bands, int accumulate, void* workspace,
                     int64_t workspace_bytes, void* stream) {
-  QMX_CHECK_ARG(plan && C && out && V, "qmx_apply_local: null argument");
-  QMX_CHECK_ARG(n_bands >= 0, "qmx_apply_local: negative n_bands");
-  if (n_bands == 0) return QMX_OK;
-  QMX_CHECK_ARG(workspace && workspace_bytes > 0, "qmx_apply_local: workspace required");
-  QMX_CHECK_ARG(ldb >= plan->n_tot && out_ldb >= plan->n_tot, "qmx_apply_local: ldb < n_tot");
-  QMX_CHECK_ARG(plan->nk == 1 || (ldk > 0 && out_ldk > 0), "qmx_apply_local: bad k stride");
-  QMX_CHECK_ARG(!ke || plan->nk == 1 || ke_ldk >= plan->n_tot, "qmx_apply_local: bad ke stride");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == QMX_C64)
-    return apply_local_impl<float>(plan, C, ldb, ldk, out, out_ldb, out_ldk, V, ke, ke_ldk,
-                                   n_bands, accumulate, workspace, workspace_bytes, st);
-  if (dtype == QMX_C128)
-    return apply_local_impl<double>(plan, C, ldb, ldk, out, out_ldb, out_ldk, V, ke, ke_ldk,
-                                    n_bands, accumulate, workspace, workspace_bytes, st);
-  set_error("qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
-  return QMX_ERR_ARG;
+  QMX_CHECK_ARG(plan, "qmx_apply_local: null argument");
+  return qmx_apply_local_k(plan, 0, plan->nk, dtype, C, ldb, ldk, out, out_ldb, out_ldk, V, ke,
+                           ke_ldk, n_bands, accumulate, workspace, workspace_bytes, stream);
 }
 
 int qmx_fft3d(int dtype, const int32_t* shape, void* data, int64_t batch, int direction,

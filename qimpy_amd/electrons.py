@@ -92,9 +92,14 @@ class Electrons:
 
     # --------------------------------------------------------------- set-up
     def initialize_wavefunctions(self, seed: int = 0) -> None:
-        """Random bands then Cholesky orthonormalisation (_electrons.py:271-287)."""
-        self.C.randomize(seed=seed)
-        self.C = self.C.orthonormalize()
+        """Random bands then Cholesky orthonormalisation (_electrons.py:271-287).
+        Both run in FP64 whatever the working precision: bandwidth-limited
+        random bands span their space with a condition number beyond float32,
+        so only the orthonormal result is cast."""
+        C = self.C if self.dtype == torch.complex128 else self.C.to(torch.complex128)
+        C.randomize(seed=seed)
+        C = C.orthonormalize()
+        self.C = C if self.dtype == torch.complex128 else C.to(self.dtype)
         self.deig_max = np.inf
 
     def get_Eband(self) -> float:

@@ -21,6 +21,38 @@ Y00 = 0.5 / np.sqrt(np.pi)
 Y1 = np.sqrt(3.0 / (4.0 * np.pi))
 
 
+def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
+                     k: torch.Tensor, Gbasis: np.ndarray, volume: float,
+                     n: torch.Tensor) -> torch.Tensor:
+    """beta (K, P, G) complex128 for atoms at `positions` (fractional) on the
+    plane waves iG (K, G, 3) + k (K, 3); entries g >= n[k] are zero
+    (_ions_projectors.py:28-72). Projector order atom -> radial -> (l, m)."""
+    dev = iG.device
+    K, G = iG.shape[0], iG.shape[1]
+    n_atoms = positions.shape[0]
+    if n_atoms == 0 or not proj_rc or not proj_l:
+        return torch.zeros((K, 0, G), dtype=torch.complex128, device=dev)
+    iGk = iG.to(torch.float64) + k[:, None, :]
+    Gk = iGk @ torch.tensor(Gbasis.T, device=dev)
+    Gmag2 = Gk.square().sum(-1)
+    Y = []
+    for l in proj_l:
+        if l == 0:
+            Y.append(torch.full_like(Gmag2, Y00).to(torch.complex128))
+        else:  # i^l r Y_1m, m order (y, z, x)
+            Y.extend(1j * Y1 * Gk[..., a] for a in (1, 2, 0))
+    Y = torch.stack(Y, dim=-1)  # (K, G, n_lm)
+    f = torch.exp(-0.5 * torch.tensor(proj_rc, device=dev, dtype=torch.float64) ** 2
+                  * Gmag2[..., None])  # (K, G, n_rc)
+    # exp(-2 pi i (iG + k) . tau) / sqrt(V), atoms last (_ions.py:304-311):
+    phase = torch.polar(torch.full((), 1.0 / np.sqrt(volume), device=dev, dtype=torch.float64),
+                        (-2 * np.pi) * (iGk @ positions.T))  # (K, G, A)
+    proj = phase[..., :, None, None] * f[..., None, :, None] * Y[..., None, None, :]
+    proj = proj.reshape(K, G, -1).transpose(1, 2).contiguous()  # (K, P, G)
+    pad = torch.arange(G, device=dev)[None, None, :] >= n.to(dev)[:, None, None]
+    return proj.masked_fill_(pad, 0.0)
+
+
 class Ions:
     beta_version: int
 
@@ -54,25 +86,9 @@ class Ions:
         """(Re)build beta on `basis` and bump beta_version."""
         from .wavefunction import Wavefunction
 
-        iGk = basis.iG.to(torch.float64) + basis.k[:, None, :]
-        Gk = iGk @ torch.tensor(basis.Gbasis.T, device=rc.device)
-        Gmag2 = Gk.square().sum(-1)
-        Ylm = {
-            0: [torch.full_like(Gmag2, Y00).to(torch.complex128)],
-            1: [1j * Y1 * Gk[..., 1], 1j * Y1 * Gk[..., 2], 1j * Y1 * Gk[..., 0]],
-        }
-        phase = self.translation_phase(iGk) / np.sqrt(basis.volume)  # (K, G, atoms)
-        rows = []
-        for i_atom in range(self.n_atoms):
-            ph = phase[..., i_atom]
-            for rc_ in self.proj_rc:
-                f = torch.exp(-0.5 * rc_**2 * Gmag2)
-                for l in self.proj_l:
-                    for Y in Ylm[l]:
-                        rows.append(f * Y * ph)
-        proj = torch.stack(rows, dim=1)[None, :, :, None, :]  # (1, K, P, 1, G)
-        proj[basis.pad_index] = 0.0
-        self.beta = Wavefunction(basis, coeff=proj.contiguous())
+        proj = build_projectors(self.positions, self.proj_rc, self.proj_l, basis.iG, basis.k,
+                                basis.Gbasis, basis.volume, basis.n)
+        self.beta = Wavefunction(basis, coeff=proj[None, :, :, None, :].contiguous())
         self.beta_version += 1
         self._beta_cast = {}
 

@@ -34,7 +34,7 @@ class Waitless(Generic[T]):
 #: Size limits of the batched Jacobi (qmx_heevj_batched) and Householder + QL
 #: (qmx_heev_small_batched) kernels.
 JACOBI_MAX_N = 96
-SMALL_QL_MAX_N = 128
+SMALL_QL_MAX_N = 256
 #: Size window of the cooperative tridiagonalisation (qmx_hetrd): below it the
 #: library eigensolver is as fast, above it grid barriers make it slower.
 HETRD_MIN_N, HETRD_MAX_N = 200, 800
@@ -95,7 +95,7 @@ def small_method(n: int, n_mat: int) -> str:
 
 def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch.Tensor]:
     """Batched on-chip eigensolve of (..., n, n) complex128 on the GPU:
-    method "ql" (Householder + QL, n <= 128) or "jacobi" (n <= 96)."""
+    method "ql" (Householder + QL, n <= 256) or "jacobi" (n <= 96)."""
     n = H.shape[-1]
     batch = H.shape[:-2]
     Hf = _fp64(H).reshape((-1, n, n)).contiguous()

@@ -267,7 +267,8 @@ def _hermitian_batch(n, seed):
 
 
 @pytest.mark.parametrize("method,n", [("jacobi", 1), ("jacobi", 36), ("jacobi", 96),
-                                      ("ql", 1), ("ql", 2), ("ql", 37), ("ql", 96), ("ql", 128)])
+                                      ("ql", 1), ("ql", 2), ("ql", 37), ("ql", 96), ("ql", 128),
+                                      ("ql", 142), ("ql", 200)])
 def test_small_eigensolvers(gpu, method, n):
     """Batched on-chip eigensolvers: eigenvalues equal LAPACK's (numpy, FP64),
     eigenvectors are orthonormal and satisfy H V = V E."""
