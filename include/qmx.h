@@ -79,6 +79,16 @@ int64_t qmx_plan_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_band
 /* Query plan geometry: [0] active x-planes, [1] active (x, y) columns. */
 int qmx_plan_info(const qmx_plan* plan, int64_t* info);
 
+/* Kernel sequence of qmx_apply_local. 0 (default): the fused plane pass
+ * (one kernel for the y and x transforms and the potential, each x-y plane
+ * in LDS) wherever a plane fits, else separate y, x and y passes; 1: always
+ * the separate passes; 2: same as 0. qmx_plan_local_path returns the path
+ * taken for a precision (2 plane, 1 separate, -1 bad argument). Set it
+ * before sizing the workspace. No reference counterpart (the reference
+ * calls a library 3D FFT, _basis_ops.py:152-205). */
+int qmx_plan_set_local_path(qmx_plan* plan, int path);
+int qmx_plan_local_path(const qmx_plan* plan, int dtype);
+
 /* Local-potential apply, optionally with the kinetic term:
  *   out(k, b, G) = [accumulate ? out : 0] + KE(k, G) C(k, b, G)
  *                  + (1/N) FFT[ V(r) IFFT[C](r) ](G)
@@ -200,6 +210,8 @@ enum {
   QMX_KERNEL_PASS_E = 4, /* columns -> sphere, forward z, KE, constrain */
   QMX_KERNEL_FFT3D = 5,
   QMX_KERNEL_GEMM = 6,
+  QMX_KERNEL_PLANE = 7,   /* fused plane pass: inverse y, x, * V, forward x, y */
+  QMX_KERNEL_VZMAJOR = 8, /* potential boxes to z-major order */
   QMX_N_KERNEL_IDS = 16
 };
 

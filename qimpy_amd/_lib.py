@@ -7,6 +7,7 @@ call fails, a ``RuntimeError`` carrying ``qmx_last_error()`` is raised.
 from __future__ import annotations
 
 import ctypes as ct
+import os
 from pathlib import Path
 
 import numpy as np
@@ -41,6 +42,8 @@ SIGNATURES = {
     "qmx_plan_destroy": (i32, [vp]),
     "qmx_plan_workspace_bytes": (i64, [vp, i32, i64]),
     "qmx_plan_info": (i32, [vp, c_i64p]),
+    "qmx_plan_set_local_path": (i32, [vp, i32]),
+    "qmx_plan_local_path": (i32, [vp, i32]),
     "qmx_apply_local": (
         i32,
         [vp, i32, vp, i64, i64, vp, i64, i64, vp, vp, i64, i64, i32, vp, i64, vp],
@@ -126,6 +129,18 @@ def make_geom(shape, n: np.ndarray, fft_index: np.ndarray,
     return g, (n32, fi, vs)
 
 
+def create_plan(geom: QmxGeom) -> ct.c_void_p:
+    """qmx_plan_create, with the local-apply kernel sequence taken from
+    QMX_LOCAL_PATH (0 automatic, 1 separate passes, 2 fused plane pass where
+    it fits; default 0)."""
+    handle = ct.c_void_p()
+    call("qmx_plan_create", ct.byref(geom), ct.byref(handle))
+    path = int(os.environ.get("QMX_LOCAL_PATH", "0"))
+    if path:
+        call("qmx_plan_set_local_path", handle, path)
+    return handle
+
+
 def geom_tables(shape, n: np.ndarray, fft_index: np.ndarray) -> dict:
     """Host-side sparse-FFT tables (qmx_geom_tables) as numpy arrays."""
     g, keep = make_geom(shape, n, fft_index)
@@ -153,6 +168,7 @@ def geom_tables(shape, n: np.ndarray, fft_index: np.ndarray) -> dict:
 KERNEL_NAMES = {
     0: "k_sphere_to_cols", 1: "k_cols_to_planes", 2: "k_xpass_potential",
     3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
+    7: "k_plane_potential", 8: "k_potential_zmajor",
 }
 N_KERNEL_IDS = 16
 

@@ -96,9 +96,7 @@ class Basis:
 
         # Sparse-FFT plan (index tables on the device):
         geom, keep = _lib.make_geom(self.grid.shape, self.n_np, self.fft_index_np)
-        handle = ct.c_void_p()
-        _lib.call("qmx_plan_create", ct.byref(geom), ct.byref(handle))
-        self._plan = handle
+        self._plan = _lib.create_plan(geom)
         info = np.zeros(2, dtype=np.int64)
         _lib.call("qmx_plan_info", self._plan, info.ctypes.data_as(_lib.c_i64p))
         self.n_active_planes, self.n_active_columns = int(info[0]), int(info[1])

@@ -21,14 +21,25 @@ GEMMs over Kb, and the subspace eigensolves as one batched on-chip launch.
 
 Iteration. The reference Davidson step with every band expanded in every
 iteration (the reference's locking counter n_eigs_done stays 0), so shapes
-stay fixed while jobs join and leave between iterations. Each job converges
-on its own n_bands: the plain (complex128) form by the reference's test,
-eigenvalue change below eig_threshold (_davidson.py:126-139); the mixed
-(complex64) form by residual norms ||HC - eig C|| below residual_threshold,
-then an FP64 Rayleigh-Ritz (complex128 H C and eighg) gives its eigenvalues.
-A job's start is the reference's: xorshift bands with its own seed and basis
-(_wavefunction_init.py:27-125), Cholesky orthonormalisation and a subspace
-rotation, so its result does not depend on the other jobs of its bucket.
+stay fixed while jobs join and leave between iterations. The expansion is
+made orthonormal before H is applied: projected out of span(C) twice
+(classical Gram-Schmidt with re-orthogonalisation) and orthonormalised by
+the FP64 eigenvectors of its own overlap, dropping directions whose overlap
+eigenvalue falls below a relative 1e-8 (they sit at +1e6 Ha in the projected
+H and sort out of the kept bands). The subspace problem is then an ordinary
+FP64 Hermitian eigenproblem of size 2 nb, with the same span and Ritz values
+as the reference's generalized one (_davidson.py:199-206). No step needs the
+host: one device-to-host read per iteration of the solver carries every
+bucket's convergence data.
+
+Each job converges on its own n_bands: the plain (complex128) form by the
+reference's test, eigenvalue change below eig_threshold
+(_davidson.py:126-139); the mixed (complex64) form by residual norms
+||HC - eig C|| below residual_threshold, after which an FP64 Rayleigh-Ritz
+(complex128 H C and eighg) gives its eigenvalues. A job starts as in the
+reference: xorshift bands with its own seed and basis
+(_wavefunction_init.py:27-125), orthonormalised in FP64, then rotated to
+Ritz vectors, so its result does not depend on the other jobs of its bucket.
 """
 from __future__ import annotations
 
@@ -43,7 +54,7 @@ import torch
 from . import _lib, gemm, rc
 from .basis import DTYPE_CODE, REAL_OF
 from .ions import build_projectors
-from .linalg import dagger, eigh, eighg, eighg_robust, ortho_matrix
+from .linalg import dagger, eigh_nosync
 from .systems import signed_freqs
 
 C64, C128 = torch.complex64, torch.complex128
@@ -238,8 +249,15 @@ def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
 class Bucket:
     """Jobs sharing an FFT box and a band class, iterated in lockstep."""
 
+    #: Overlap eigenvalue (of unit-norm residuals) below which an expansion
+    #: direction is dropped, in the plain and the mixed form.
+    DROP_RTOL = 1e-10
+    DROP_RTOL_MIXED = 1e-4
+    #: Diagonal shift (Ha) that moves dropped directions above every kept band.
+    DROP_SHIFT = 1e6
+
     def __init__(self, shape, nb: int, dtype: torch.dtype, *, residual_threshold: float,
-                 eig_threshold: float, max_iterations: int, stream=None):
+                 eig_threshold: float, max_iterations: int):
         self.shape = tuple(int(s) for s in shape)
         self.N = int(np.prod(self.shape))
         self.nb = int(nb)
@@ -249,14 +267,12 @@ class Bucket:
         self.residual_threshold = residual_threshold
         self.eig_threshold = eig_threshold
         self.max_iterations = max_iterations
-        self.stream = stream
         self.jobs: list[CellJob] = []
         self.k0: list[int] = []
         self._plan = None
-        self.Kb = 0
-        self.Gb = 0
-        self.Pb = 0
+        self.Kb = self.Gb = self.Pb = 0
         self.n_ticks = 0
+        self._info: list[torch.Tensor] = []
 
     # ------------------------------------------------------------ structure
     def __del__(self):
@@ -280,7 +296,7 @@ class Bucket:
         if not jobs:
             self.jobs, self.k0, self.Kb = [], [], 0
             for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "norm", "beta", "D",
-                         "ke", "ke_r", "n32", "V", "V64"):
+                         "ke", "ke_r", "n32", "V", "V64", "workspace"):
                 setattr(self, name, None)
             self._destroy_plan()
             return
@@ -293,11 +309,11 @@ class Bucket:
         eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
         deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
         if keep:
-            idx = torch.cat([torch.arange(self.k0[self.jobs.index(j)],
-                                          self.k0[self.jobs.index(j)] + j.K) for j in keep]).to(dev)
+            where = {id(j): a for j, a in zip(self.jobs, self.k0)}
+            idx = torch.cat([torch.arange(where[id(j)], where[id(j)] + j.K) for j in keep]).to(dev)
             g = min(Gb, self.Gb)
-            X[:Kkeep, :, :g] = self.X.index_select(0, idx)[:, :, :g]
-            HX[:Kkeep, :, :g] = self.HX.index_select(0, idx)[:, :, :g]
+            X[:Kkeep, :nb, :g] = self.X.index_select(0, idx)[:, :nb, :g]
+            HX[:Kkeep, :nb, :g] = self.HX.index_select(0, idx)[:, :nb, :g]
             eig[:Kkeep] = self.eig.index_select(0, idx)
             deig[:Kkeep] = self.deig.index_select(0, idx)
         k0, acc = [], 0
@@ -322,10 +338,8 @@ class Bucket:
         n = np.concatenate([j.n for j in jobs])
         self._destroy_plan()
         geom, keepalive = _lib.make_geom(self.shape, n, fi, v_slot)
-        handle = ct.c_void_p()
-        _lib.call("qmx_plan_create", ct.byref(geom), ct.byref(handle))
+        self._plan = _lib.create_plan(geom)
         del keepalive
-        self._plan = handle
         self.jobs, self.k0, self.Kb, self.Gb, self.Pb = jobs, k0, Kb, Gb, Pb
         self.X, self.HX, self.eig, self.deig = X, HX, eig, deig
         self.X2, self.HX2 = torch.empty_like(X), torch.empty_like(HX)
@@ -335,9 +349,8 @@ class Bucket:
         self.beta, self.D = beta, D
         self.V64 = V64
         self.V = V64 if dt == C128 else V64.to(self.rdtype)
-        self.rnorm = None
-        self.norm = None
-        need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], nb))
+        self.rnorm = self.norm = None
+        need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dt], 2 * nb))
         one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], 1))
         size = max(one, min(need, WORKSPACE_CAP))
         self.workspace = torch.empty(size, dtype=torch.uint8, device=dev)
@@ -345,8 +358,8 @@ class Bucket:
     # ------------------------------------------------------------ operators
     def hamiltonian(self, Cin: torch.Tensor, out: torch.Tensor, k0: int, nk: int,
                     beta: Optional[torch.Tensor] = None, D: Optional[torch.Tensor] = None) -> None:
-        """out = H Cin for the bucket k-points [k0, k0 + nk) (Cin, out
-        (nk, B, Gb) row-contiguous views, any precision)."""
+        """out = H Cin for the bucket k-points [k0, k0 + nk) (Cin, out:
+        (nk, B, Gb) views with unit stride along G, either precision)."""
         B = Cin.shape[1]
         dt = Cin.dtype
         V = self.V64 if dt == C128 else self.V
@@ -361,38 +374,56 @@ class Bucket:
             proj *= D[:, :, None].to(proj.real.dtype)
             out.baddbmm_(proj.transpose(-2, -1), beta)
 
+    def _orthonormalizer(self, O: torch.Tensor, tol: float, relative: bool = True):
+        """U (complex128) with U^H O U = 1 on the kept directions and zero
+        columns elsewhere (canonical orthogonalisation), and the kept mask:
+        overlap eigenvalues above tol (times the largest if `relative`)."""
+        lam, W, info = eigh_nosync(O)
+        if info is not None:
+            self._info.append(info)
+        keep = lam > (tol * lam[..., -1:].clamp_min(1e-300) if relative else tol)
+        s = torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))
+        return W * s[..., None, :].to(W.dtype), keep
+
+    def _subspace_eigh(self, H: torch.Tensor):
+        E, V, info = eigh_nosync(H)
+        if info is not None:
+            self._info.append(info)
+        return E, V
+
     # ------------------------------------------------------------ job start
     def _initialise(self, new: list[CellJob]) -> None:
-        """Random start, orthonormalisation and subspace rotation of the new
-        jobs' rows (_electrons.py:271-287, _davidson.py:164-169), in FP64:
-        bandwidth-limited random bands span their space with a condition
-        number far beyond float32, so the mixed form casts only the
-        resulting orthonormal Ritz vectors to complex64."""
+        """Start of the new jobs, whose rows close the stack: random bands in
+        FP64 (_electrons.py:271-287; bandwidth-limited random bands have a
+        condition number beyond float32), orthonormalised, then rotated to
+        the Ritz vectors of their span (_davidson.py:164-169)."""
+        if not new:
+            return
         nb = self.nb
-        for j, a in zip(self.jobs, self.k0):
-            if j not in new:
-                continue
-            C = torch.zeros((j.K, nb, self.Gb), dtype=C128, device=rc.device)
+        a0 = self.k0[len(self.jobs) - len(new)]
+        Kn = self.Kb - a0
+        C = torch.zeros((Kn, nb, self.Gb), dtype=C128, device=rc.device)
+        for j, a in zip(self.jobs[len(self.jobs) - len(new):], self.k0[len(self.jobs) - len(new):]):
+            Cj = C[a - a0 : a - a0 + j.K]
             # Seeds follow the job's own basis (n_max = its n_tot):
-            _lib.call("qmx_randomize", DTYPE_CODE[C128], _c(C), C.stride(1), C.stride(0), j.K,
+            _lib.call("qmx_randomize", DTYPE_CODE[C128], _c(Cj), Cj.stride(1), Cj.stride(0), j.K,
                       self.Gb, j.n_tot, 0, _c(self.n32[a:]), _c(self.ke[a:]), self.ke.stride(0),
                       j.seed, 0, nb, rc.stream_ptr())
-            C = gemm.rotate(ortho_matrix(_dot(C, C)), C)
-            HC = torch.empty_like(C)
-            if self.dtype == C128:
-                self.hamiltonian(C, HC, a, j.K)
-            else:
-                self.hamiltonian(C, HC, a, j.K, j.projectors(self.Gb) if j.P else None,
-                                 j.device_tables()["D"][None, :].expand(j.K, -1) if j.P else None)
-            E, Vr = eigh(_dot(C, HC))
-            self.X[a : a + j.K, :nb] = gemm.rotate(Vr, C)
-            self.HX[a : a + j.K, :nb] = gemm.rotate(Vr, HC)
-            self.eig[a : a + j.K] = E
             j.admitted = time.perf_counter()
+        U, _ = self._orthonormalizer(_dot(C, C), 1e-14)
+        C = gemm.rotate(U, C).to(self.dtype)
+        HC = self.HX[a0:, :nb]
+        self.hamiltonian(C, HC, a0, Kn)
+        E, Vr = self._subspace_eigh(_dot(C, HC).to(C128))
+        Vr = Vr.to(self.dtype)
+        torch.matmul(Vr.transpose(-2, -1), C, out=self.X[a0:, :nb])
+        self.HX[a0:, :nb] = torch.matmul(Vr.transpose(-2, -1), HC)
+        self.eig[a0:] = E
 
     # ------------------------------------------------------------ iteration
     def residuals(self) -> None:
-        """R = precondition(HC - C eig) into X[:, nb:] with norms (_davidson.py:182-184)."""
+        """R = precondition(HC - C eig) into X[:, nb:] with norms
+        (_davidson.py:182-184)."""
         nb = self.nb
         C, HC, R = self.X[:, :nb], self.HX[:, :nb], self.X[:, nb:]
         KEref = _band_ke(C, self.ke)
@@ -406,76 +437,83 @@ class Bucket:
         self.norm = norm2.sqrt()
         self.rnorm = rnorm2.sqrt()
 
-    def converged_jobs(self, rnorm_host: np.ndarray, deig_host: np.ndarray) -> list[CellJob]:
+    def pack(self) -> torch.Tensor:
+        """Convergence data for the host read: [rnorm, deig, number of
+        unconverged QL eigensolves since the last read]."""
+        fail = torch.zeros(2, dtype=torch.float64, device=rc.device)
+        for info in self._info:
+            fail[0] += (info != 0).sum()
+        self._info = []
+        fail[1] = (~torch.isfinite(self.eig)).sum() + (~torch.isfinite(self.rnorm)).sum()
+        return torch.cat([self.rnorm.flatten(), self.deig.flatten(), fail])
+
+    def converged_jobs(self, packed: np.ndarray) -> list[CellJob]:
+        nb = self.nb
+        if packed[-2] != 0:
+            raise RuntimeError(f"batched QL eigensolver did not converge ({int(packed[-2])} matrices)"
+                               f" in bucket {self.shape}, {nb} bands")
+        if packed[-1] != 0:
+            raise RuntimeError(f"non-finite eigenvalues or residuals ({int(packed[-1])}) in bucket"
+                               f" {self.shape}, {nb} bands")
+        rnorm = packed[: self.Kb * nb].reshape(self.Kb, nb)
+        deig = packed[self.Kb * nb : 2 * self.Kb * nb].reshape(self.Kb, nb)
         done = []
         for j, a in zip(self.jobs, self.k0):
             rows = slice(a, a + j.K)
             if self.mixed:
-                ok = rnorm_host[rows, : j.n_bands].max() < self.residual_threshold
+                ok = rnorm[rows, : j.n_bands].max() < self.residual_threshold
             else:
-                ok = deig_host[rows, : j.n_bands].max() < self.eig_threshold
+                ok = deig[rows, : j.n_bands].max() < self.eig_threshold
             if ok or j.iterations >= self.max_iterations:
                 j.converged = bool(ok)
                 done.append(j)
         return done
 
-    def _project_out(self, C: torch.Tensor, R: torch.Tensor, seed: int) -> torch.Tensor:
-        """R <- (1 - C C^H) R twice, re-drawing directions that vanish in
-        float32 (as Davidson._project_out); returns the new norms."""
-        for attempt in range(2):
-            for _ in range(2):
-                R.baddbmm_(_dot(C, R).transpose(-2, -1), C, alpha=-1)
-            norm = _band_norm(R)
-            low = norm < 1e-3
-            if attempt or not bool(low.any()):
-                break
-            self._redraw(R, low, seed)
-            _scale_constrain(R, 1.0 / _band_norm(R), self.n32)
-        return norm
-
-    def _redraw(self, R: torch.Tensor, mask: torch.Tensor, seed: int) -> None:
-        """Replace the selected (k, band) rows of R by bandwidth-limited random
-        vectors (the role of randomize_selected, _wavefunction_init.py:128-188)."""
-        i_k, i_b = torch.where(mask)
-        gen = torch.Generator(device=rc.device).manual_seed(int(seed))
-        G = self.Gb
-        re = torch.randn((len(i_k), G), generator=gen, device=rc.device, dtype=torch.float64)
-        im = torch.randn((len(i_k), G), generator=gen, device=rc.device, dtype=torch.float64)
-        ke = self.ke[i_k]
-        vals = torch.complex(re, im) / (1.0 + ((4.0 / 3) * ke) ** 6)
-        vals[torch.arange(G, device=rc.device)[None, :] >= self.n32[i_k].long()[:, None]] = 0.0
-        R[i_k, i_b] = vals.to(R.dtype)
+    def _eighg_chol(self, H: torch.Tensor, O: torch.Tensor):
+        """FP64 generalized eigenproblem H V = O V E for overlaps known to be
+        close to the identity (Cholesky without a host check)."""
+        L = torch.linalg.cholesky_ex(O)[0]
+        eye = torch.eye(O.shape[-1], dtype=O.dtype, device=O.device).expand_as(L)
+        Li = torch.linalg.solve_triangular(L, eye, upper=False)
+        Hh = Li @ H @ dagger(Li)
+        E, W = self._subspace_eigh(0.5 * (Hh + dagger(Hh)))
+        return E, dagger(Li) @ W
 
     def expand(self) -> None:
         """One Davidson step on every row (_davidson.py:180-233, no locking)."""
-        nb = self.nb
+        nb, dt = self.nb, self.dtype
         C, R = self.X[:, :nb], self.X[:, nb:]
         HC, HR = self.HX[:, :nb], self.HX[:, nb:]
+        # Normalised residuals; a null one (below 1e-30, where 1/norm would
+        # overflow float32) is zeroed and then dropped with the null space:
         norm = self.norm
-        low = norm < np.sqrt(self.Gb * 1e-15)  # _regularize, _davidson.py:56-58,101-115
-        if bool(low.any()):
-            self._redraw(R, low, 7919 * self.n_ticks + 1)
-            norm = torch.where(low, _band_norm(R), norm)
-        _scale_constrain(R, 1.0 / norm, self.n32)
-        if self.mixed:
-            norm = self._project_out(C, R, 104729 * self.n_ticks + 3)
-            _scale_constrain(R, 1.0 / norm.clamp_min(1e-300), self.n32)
+        _scale_constrain(R, torch.where(norm > 1e-30, 1.0 / norm.clamp_min(1e-30), 0.0), self.n32)
+        for _ in range(2):  # R <- (1 - C C^H) R, twice
+            R.baddbmm_(_dot(C, R).transpose(-2, -1), C, alpha=-1)
+        rtol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
+        U, keep = self._orthonormalizer(_dot(R, R).to(C128), rtol, relative=False)
+        R.copy_(torch.matmul(U.to(dt).transpose(-2, -1), R))
         self.hamiltonian(R, HR, 0, self.Kb)
-        XO = _dot(self.X, R).to(C128)  # (Kb, 2nb, nb): [C^H R; R^H R]
+        XO = _dot(self.X, R).to(C128)  # (Kb, 2 nb, nb): [C^H R; R^H R]
         XH = _dot(self.X, HR).to(C128)
         n2 = 2 * nb
         O = torch.zeros((self.Kb, n2, n2), dtype=C128, device=rc.device)
         H = torch.zeros_like(O)
         O.diagonal(dim1=-2, dim2=-1)[:, :nb] = 1.0
         H.diagonal(dim1=-2, dim2=-1)[:, :nb] = self.eig.to(C128)
-        O[:, :, nb:] = XO
-        H[:, :, nb:] = XH
+        O[:, :nb, nb:] = XO[:, :nb]
         O[:, nb:, :nb] = dagger(XO[:, :nb])
+        O[:, nb:, nb:] = 0.5 * (XO[:, nb:] + dagger(XO[:, nb:]))
+        H[:, :nb, nb:] = XH[:, :nb]
         H[:, nb:, :nb] = dagger(XH[:, :nb])
-        E, Vs = (eighg_robust if self.mixed else eighg)(H, O)
-        Vs = Vs[:, :, :nb].to(self.dtype)
-        torch.matmul(Vs.transpose(-2, -1), self.X, out=self.X2[:, :nb])
-        torch.matmul(Vs.transpose(-2, -1), self.HX, out=self.HX2[:, :nb])
+        H[:, nb:, nb:] = 0.5 * (XH[:, nb:] + dagger(XH[:, nb:]))
+        drop = ~keep
+        O.diagonal(dim1=-2, dim2=-1)[:, nb:] += drop.to(torch.float64)
+        H.diagonal(dim1=-2, dim2=-1)[:, nb:] += torch.where(drop, self.DROP_SHIFT, 0.0)
+        E, Vs = self._eighg_chol(H, O)
+        Vt = Vs[:, :, :nb].to(dt).transpose(-2, -1)
+        torch.matmul(Vt, self.X, out=self.X2[:, :nb])
+        torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
         self.HX, self.HX2 = self.HX2, self.HX
         E = E[:, :nb]
@@ -486,52 +524,66 @@ class Bucket:
         self.n_ticks += 1
 
     # ------------------------------------------------------------ results
-    def finish(self, job: CellJob) -> CellResult:
-        """Result of a converged (or exhausted) job; the mixed form first
-        recomputes its subspace in FP64 (complex128 H C and eighg)."""
+    def finish(self, job: CellJob) -> "_Pending":
+        """Device results of a converged (or exhausted) job; the mixed form
+        first recomputes its subspace in FP64 (complex128 H C, FP64 overlap
+        and eigenproblem, the final Rayleigh quotients). The packed result
+        ends with the count of unconverged QL eigensolves."""
         a = self.k0[self.jobs.index(job)]
         rows = slice(a, a + job.K)
-        nb = self.nb
+        nb, B = self.nb, job.n_bands
+        n_info = len(self._info)
         if self.mixed:
             C = self.X[rows, :nb].to(C128)
             HC = torch.empty_like(C)
             beta = job.projectors(self.Gb) if job.P else None
             D = job.device_tables()["D"][None, :].expand(job.K, -1) if job.P else None
             self.hamiltonian(C, HC, a, job.K, beta, D)
-            E, Vs = eighg(_dot(C, HC), _dot(C, C))
+            E, Vs = self._eighg_chol(_dot(C, HC), _dot(C, C))
+            Vs = Vs[:, :, :B]
             C = gemm.rotate(Vs, C)
             HC = gemm.rotate(Vs, HC)
-            resid = HC - C * E[:, :, None].to(C128)
-            rnorm = torch.linalg.vector_norm(resid, dim=-1)
+            eig = E[:, :B]
+            rnorm = torch.linalg.vector_norm(HC - C * eig[:, :, None].to(C128), dim=-1)
         else:
-            E = self.eig[rows]
-            rnorm = self.rnorm[rows] if self.rnorm is not None else torch.full_like(E, np.nan)
-        B = job.n_bands
-        eig = E[:, :B].cpu().numpy()
-        res = CellResult(
-            job_id=job.job_id, eig=eig, residual_norm=rnorm[:, :B].cpu().numpy(),
-            Eband=float(2.0 * (job.wk[:, None] * eig).sum()), iterations=job.iterations,
-            converged=getattr(job, "converged", False), arrival=job.arrival,
-            admitted=job.admitted, finished=time.perf_counter())
-        job.result = res
-        return res
+            eig = self.eig[rows, :B].clone()
+            rnorm = self.rnorm[rows, :B].clone()
+        fail = torch.zeros(1, dtype=torch.float64, device=rc.device)
+        for info in self._info[n_info:]:
+            fail += (info != 0).sum()
+        del self._info[n_info:]
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return _Pending(job, torch.cat([eig.flatten(), rnorm.flatten(), fail]), ev)
+
+
+@dataclass
+class _Pending:
+    job: CellJob
+    data: torch.Tensor
+    event: "torch.cuda.Event"
 
 
 # ---------------------------------------------------------------- solver
 class BatchSolver:
     """Runs band-solve jobs in buckets, admitting new jobs as others finish.
 
-    ``submit`` queues jobs; ``tick`` performs one iteration of every bucket:
-    residuals (one device-to-host read per bucket for the convergence
-    test), retirement of converged jobs, admission of queued jobs into the
-    freed capacity (HBM budget and per-bucket k-point limit) and one
-    Davidson step. ``run`` ticks until every submitted job is done.
+    ``submit`` queues jobs (arrival times relative to the solver's start);
+    ``tick`` performs one iteration of every bucket: residuals and one
+    device-to-host read for all buckets (convergence data and the previous
+    tick's results), retirement of converged jobs, admission of arrived jobs
+    into the freed capacity (HBM budget and per-bucket k-point limit) and one
+    Davidson step. Every launch goes to the caller's current stream: the
+    library routines behind torch.linalg and the batched GEMMs share one
+    handle (and its device workspace) per thread across streams, so buckets
+    are not spread over streams. ``run`` ticks until every submitted job is
+    done.
     """
 
     def __init__(self, dtype: torch.dtype = C64, *, residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
                  band_classes: Optional[Sequence[int]] = None, hbm_budget: float = 200e9,
-                 max_k_per_bucket: int = 2048, n_streams: int = 1):
+                 max_k_per_bucket: int = 2048):
         rc.require_gpu()
         self.dtype = dtype
         self.residual_threshold = residual_threshold
@@ -540,12 +592,22 @@ class BatchSolver:
         self.band_classes = sorted(band_classes) if band_classes else None
         self.hbm_budget = hbm_budget
         self.max_k_per_bucket = max_k_per_bucket
-        self.streams = [torch.cuda.Stream(device=rc.device) for _ in range(max(0, n_streams - 1))]
         self.buckets: dict = {}
         self.queue: list[CellJob] = []
         self.results: dict[int, CellResult] = {}
+        self.pending: list[_Pending] = []
         self.hbm_in_use = 0
         self._held: dict[int, int] = {}
+        self.n_ticks = 0
+        torch.cuda.synchronize()
+        self._t0_event = torch.cuda.Event(enable_timing=True)
+        self._t0_event.record()
+        torch.cuda.synchronize()
+        self.t0 = time.perf_counter()
+
+    def clock(self) -> float:
+        """Seconds since the solver started (the time base of arrivals)."""
+        return time.perf_counter() - self.t0
 
     def band_class(self, job: CellJob) -> int:
         if self.band_classes:
@@ -560,10 +622,8 @@ class BatchSolver:
     def _bucket(self, key) -> Bucket:
         b = self.buckets.get(key)
         if b is None:
-            stream = self.streams[len(self.buckets) % len(self.streams)] if self.streams else None
             b = Bucket(key[0], key[1], self.dtype, residual_threshold=self.residual_threshold,
-                       eig_threshold=self.eig_threshold, max_iterations=self.max_iterations,
-                       stream=stream)
+                       eig_threshold=self.eig_threshold, max_iterations=self.max_iterations)
             self.buckets[key] = b
         return b
 
@@ -589,62 +649,76 @@ class BatchSolver:
         self.queue = rest
         return taken
 
+    def _collect(self, pending: list, host: list) -> None:
+        for p, h in zip(pending, host):
+            job = p.job
+            K, B = job.K, job.n_bands
+            if h[-1] != 0:
+                raise RuntimeError(f"batched QL eigensolver did not converge (job {job.job_id})")
+            eig = h[: K * B].reshape(K, B).copy()
+            rnorm = h[K * B : 2 * K * B].reshape(K, B).copy()
+            res = CellResult(
+                job_id=job.job_id, eig=eig, residual_norm=rnorm,
+                Eband=float(2.0 * (job.wk[:, None] * eig).sum()), iterations=job.iterations,
+                converged=getattr(job, "converged", False), arrival=job.arrival,
+                admitted=(job.admitted - self.t0) if job.admitted else job.arrival,
+                finished=self._t0_event.elapsed_time(p.event) * 1e-3)
+            job.result = res
+            self.results[job.job_id] = res
+
     def tick(self, now: Optional[float] = None) -> list[CellResult]:
-        finished: list[CellResult] = []
+        """One iteration of every bucket; returns the results that became
+        available (jobs finished in the previous tick)."""
         active = [b for b in self.buckets.values() if b.jobs]
-        # 1. residuals of every active bucket, then one host read each.
+        # 1. convergence data of every bucket and the previous results, one
+        #    read (each copy queued on the stream that produced its data).
+        host_packs = []
         for b in active:
-            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
-                b.residuals()
-        retire: dict = {}
-        for b in active:
-            rn = b.rnorm.cpu().numpy()
-            de = b.deig.cpu().numpy()
-            retire[id(b)] = b.converged_jobs(rn, de)
-        # 2. retire, admit, restack, start the new jobs, one Davidson step.
-        for b in active:
-            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
-                for job in retire[id(b)]:
-                    finished.append(b.finish(job))
-        newcomers = self._free_and_admit(retire, now)
-        for key in sorted(set(newcomers) | {(b.shape, b.nb) for b in active}):
+            b.residuals()
+            host_packs.append(b.pack().to("cpu", non_blocking=True))
+        prev = self.pending
+        self.pending = []
+        host_prev = [p.data.to("cpu", non_blocking=True) for p in prev]
+        torch.cuda.current_stream().synchronize()
+        self._collect(prev, [h.numpy() for h in host_prev])
+        new_results = [self.results[p.job.job_id] for p in prev]
+        host = [h.numpy() for h in host_packs]
+        # 2. retire converged jobs (their FP64 results are read next tick).
+        retire = {}
+        for b, h in zip(active, host):
+            done = b.converged_jobs(h)
+            retire[id(b)] = done
+            for job in done:
+                self.pending.append(b.finish(job))
+            for job in done:
+                self.hbm_in_use -= self._held.pop(job.job_id, 0)
+        # 3. admission, restack, start of new jobs, one Davidson step.
+        newcomers = self._admit(now)
+        keys = sorted(set(newcomers) | {(b.shape, b.nb) for b in active})
+        for key in keys:
             b = self._bucket(key)
             new = newcomers.get(key, [])
             gone = retire.get(id(b), [])
-            with torch.cuda.stream(b.stream) if b.stream is not None else _null():
-                if new or gone:
-                    b._restack([j for j in b.jobs if j not in gone], new)
-                    if not b.jobs:
-                        continue
-                    b._initialise(new)
-                    b.residuals()
-                if b.jobs:
-                    b.expand()
-        for res in finished:
-            self.results[res.job_id] = res
-        return finished
-
-    def _free_and_admit(self, retire: dict, now) -> dict:
-        for jobs in retire.values():
-            for job in jobs:
-                self.hbm_in_use -= self._held.pop(job.job_id, 0)
-                job.release()
-        return self._admit(now)
+            if new or gone:
+                b._restack([j for j in b.jobs if j not in gone], new)
+                for job in gone:
+                    job.release()
+                if not b.jobs:
+                    continue
+                b._initialise(new)
+                b.residuals()
+            if b.jobs:
+                b.expand()
+        self.n_ticks += 1
+        return new_results
 
     @property
     def idle(self) -> bool:
-        return not self.queue and not any(b.jobs for b in self.buckets.values())
+        return (not self.queue and not self.pending
+                and not any(b.jobs for b in self.buckets.values()))
 
     def run(self) -> dict[int, CellResult]:
         while not self.idle:
             self.tick()
         torch.cuda.synchronize()
         return self.results
-
-
-class _null:
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *a):
-        return False

@@ -92,7 +92,8 @@ struct Radices {
   int n;
   int nr;
   int r[kMaxRadices];
-  FastDiv ns[kMaxRadices];
+  FastDiv ns[kMaxRadices];  // product of the radices before pass ip
+  FastDiv st[kMaxRadices];  // n / r[ip], the butterflies per line of pass ip
 };
 
 bool factorize(int n, Radices* out);

@@ -29,6 +29,17 @@ using namespace qmx;
 
 namespace {
 
+// Strict total order for sorting eigenvalues: ascending, NaN after every
+// number, equal values (and NaNs among themselves) by index. The ranks
+// #{j : sorts_before(x_j, j, x_i, i)} are then a permutation of 0..n-1 for
+// any input, so index arrays built from them are always fully written.
+__device__ __forceinline__ bool sorts_before(double a, int ia, double b, int ib) {
+  const bool na = a != a, nb = b != b;
+  if (na || nb) return (na && nb) ? ia < ib : nb;
+  return a < b || (a == b && ia < ib);
+}
+
+
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
@@ -414,10 +425,7 @@ __global__ void __launch_bounds__(kJacThreads)
   for (int i = tid; i < n; i += kJacThreads) {
     const double li = a[i * ld + i].x;
     int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double lj = a[j * ld + j].x;
-      rank += (lj < li) || (lj == li && j < i);
-    }
+    for (int j = 0; j < n; ++j) rank += sorts_before(a[j * ld + j].x, j, li, i);
     Eb[rank] = li;
     for (int k = 0; k < n; ++k) Vb[(int64_t)k * ldv + rank] = v[k * ld + i];
   }
@@ -475,6 +483,13 @@ constexpr int kSmallThreads = 256;
 constexpr int kSmallMaxN = 256;
 constexpr int kSmallLdsN = 128;
 
+// Workspace bytes of one matrix: the n x n complex working copy, plus the
+// n x n real QL accumulator when it does not fit in LDS; 256-byte aligned.
+__host__ __device__ inline int64_t small_ws_stride(int64_t n) {
+  const int64_t bytes = n * n * 16 + (n > kSmallLdsN ? n * n * 8 : 0);
+  return (bytes + 255) / 256 * 256;
+}
+
 template <int NW>
 __device__ __forceinline__ double block_sum_small(double x, double* red) {
   x = wave_sum_d(x);
@@ -511,7 +526,7 @@ __global__ void __launch_bounds__(kSmallThreads)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t b = blockIdx.x;
   const double2* Ab = A + b * stride_a;
-  double2* W = Wws + b * (int64_t)n * n * (ZLDS ? 1 : 2);
+  double2* W = reinterpret_cast<double2*>(reinterpret_cast<char*>(Wws) + b * small_ws_stride(n));
   // Global Z: n * n doubles right after this matrix's n * n complex W.
   double* Z = ZLDS ? Zl : reinterpret_cast<double*>(W + (int64_t)n * n);
 
@@ -696,10 +711,7 @@ __global__ void __launch_bounds__(kSmallThreads)
   for (int i = tid; i < n; i += kSmallThreads) {
     const double li_ = d[i];
     int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double lj = d[j];
-      rank += (lj < li_) || (lj == li_ && j < i);
-    }
+    for (int j = 0; j < n; ++j) rank += sorts_before(d[j], j, li_, i);
     perm[rank] = i;
     E[b * n + rank] = li_;
   }
@@ -740,8 +752,7 @@ __global__ void __launch_bounds__(kSmallThreads)
 
 extern "C" int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch) {
   if (n < 1 || batch < 0) return -1;
-  const int64_t per = n * n * (int64_t)sizeof(double2) + (n > kSmallLdsN ? n * n * 8 : 0);
-  return batch * per;
+  return batch * small_ws_stride(n);
 }
 
 extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride_a,

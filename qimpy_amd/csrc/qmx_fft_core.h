@@ -183,6 +183,108 @@ __device__ C2* fft_lds_wave(C2* a, C2* b, const C2* tw, const Radices& rad, cons
   return a;
 }
 
+// In-place Stockham pass of radix R over the nl lines of a 2D LDS array,
+// element (position p, line l) at buf[p * ps + off(l)], off(l) = l * ls, or
+// lines[l] * ls when INDIRECT. All NT threads of the block take part; each
+// holds at most NB butterflies in registers between the read half and the
+// write half, which a barrier separates, so the pass overwrites its own
+// input and one buffer suffices. LINE_FAST: consecutive threads take
+// consecutive lines (unit stride when ls == 1); otherwise consecutive
+// butterflies of one line (unit stride when ps == 1). Callers guarantee
+// stride * nl <= NB * NT.
+template <typename C2, int R, int S, int NT, int NB, bool LINE_FAST, bool INDIRECT>
+__device__ __forceinline__ void ip_pass(C2* buf, const C2* __restrict__ tw, int n, FastDiv fns,
+                                        FastDiv fst, int nl, FastDiv fnl, int ps, int ls,
+                                        const int* lines) {
+  const int Ns = fns.d, stride = fst.d, twstep = n / (Ns * R);
+  const int total = stride * nl;
+  C2 v[NB][R];
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (t < total) {
+      int j, l;
+      if (LINE_FAST) {
+        j = fdiv(t, fnl);
+        l = t - j * nl;
+      } else {
+        l = fdiv(t, fst);
+        j = t - l * stride;
+      }
+      const int lo = (INDIRECT ? lines[l] : l) * ls;
+      const int k = j - fdiv(j, fns) * Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[(j + r * stride) * ps + lo];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C2 w = tw[r * k * twstep];
+        if constexpr (S > 0) w.y = -w.y;
+        v[it][r] = cmul(v[it][r], w);
+      }
+      dft<C2, R, S>(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (t < total) {
+      int j, l;
+      if (LINE_FAST) {
+        j = fdiv(t, fnl);
+        l = t - j * nl;
+      } else {
+        l = fdiv(t, fst);
+        j = t - l * stride;
+      }
+      const int lo = (INDIRECT ? lines[l] : l) * ls;
+      const int k = j - fdiv(j, fns) * Ns;
+      const int idxD = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(idxD + r * Ns) * ps + lo] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-place transform of the nl lines described as for ip_pass; at
+// most MAXE complex values per thread in any pass. Ends with a barrier.
+template <typename C2, int S, int NT, int MAXE, bool LINE_FAST, bool INDIRECT>
+__device__ void fft_ip(C2* buf, const C2* tw, const Radices& rad, int nl, FastDiv fnl, int ps,
+                       int ls, const int* lines) {
+  for (int ip = 0; ip < rad.nr; ++ip) {
+    switch (rad.r[ip]) {
+      case 4:
+        ip_pass<C2, 4, S, NT, MAXE / 4, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],
+                                                              rad.st[ip], nl, fnl, ps, ls, lines);
+        break;
+      case 2:
+        ip_pass<C2, 2, S, NT, MAXE / 2, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],
+                                                              rad.st[ip], nl, fnl, ps, ls, lines);
+        break;
+      case 3:
+        ip_pass<C2, 3, S, NT, MAXE / 3, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],
+                                                              rad.st[ip], nl, fnl, ps, ls, lines);
+        break;
+      case 5:
+        ip_pass<C2, 5, S, NT, MAXE / 5, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],
+                                                              rad.st[ip], nl, fnl, ps, ls, lines);
+        break;
+    }
+  }
+}
+
+// Largest butterfly count per thread that fft_ip needs for nl lines of the
+// factorised length, with NT threads (host side, to check against MAXE / R).
+inline bool fft_ip_fits(const Radices& rad, int nl, int nt, int maxe) {
+  for (int ip = 0; ip < rad.nr; ++ip) {
+    const int R = rad.r[ip];
+    const long total = (long)(rad.n / R) * nl;
+    if ((total + nt - 1) / nt > maxe / R) return false;
+  }
+  return true;
+}
+
 // Stage the twiddle table of length n into LDS.
 template <typename C2>
 __device__ __forceinline__ void load_twiddles(C2* dst, const C2* __restrict__ src, int n) {

@@ -68,6 +68,7 @@ bool factorize(int n, Radices* out) {
       if (out->nr >= kMaxRadices) return false;
       out->r[out->nr] = r;
       out->ns[out->nr] = make_fastdiv(ns);
+      out->st[out->nr] = make_fastdiv(n / r);
       ++out->nr;
       ns *= r;
       m /= r;
@@ -78,6 +79,7 @@ bool factorize(int n, Radices* out) {
 
 constexpr int kThreads = 256;
 constexpr int kLdsBudget = 60 * 1024;
+constexpr int kLdsMax = 160 * 1024;  // LDS of one workgroup on gfx950
 
 // Largest line count M for lines of length n with two ping-pong buffers,
 // the twiddle table and `extra_per_line` bytes per line within the budget.
@@ -114,6 +116,8 @@ struct qmx_plan {
   int* d_t1idx;       // [nk][n_tot] col * n2 + z of each sphere point
   int* d_n;           // [nk]
   int* d_vslot;       // [nk] potential box of each k
+  int* d_col_ix;      // [ncol] x of each column
+  int local_path;     // 0 automatic, 1 separate passes B, C, D
   int64_t* h_nsum;    // [nk + 1] prefix sums of n[k] (host)
   int n_vslots;       // 1 + largest slot
   float2* d_tw32[3];
@@ -124,7 +128,7 @@ namespace {
 
 // ---------------------------------------------------------------- pass A
 // One workgroup: M consecutive columns of one band of one k.
-template <typename C2, typename R>
+template <typename C2, typename R, bool ZMAJOR>
 __global__ void __launch_bounds__(kThreads)
     k_sphere_to_cols(const C2* __restrict__ C, int64_t ldb, int64_t ldk,
                      C2* __restrict__ T1, const int* __restrict__ t1idx,
@@ -152,11 +156,20 @@ __global__ void __launch_bounds__(kThreads)
   }
   __syncthreads();
   C2* res = fft_lds<C2, +1>(a, b, tw, rz, fm, ld);
-  C2* out = T1 + ((int64_t)k * nb + ib) * ncol * nz + (int64_t)col0 * nz;
-  for (int i = threadIdx.x; i < mcols * nz; i += blockDim.x) {
-    const int l = i / nz;
-    const int p = i - l * nz;
-    out[i] = res[p * ld + l];
+  C2* slab = T1 + ((int64_t)k * nb + ib) * ncol * nz;
+  if (ZMAJOR) {  // T1[z][col]: runs of mcols columns per z
+    for (int i = threadIdx.x; i < M * nz; i += blockDim.x) {
+      const int p = fdiv(i, fm);
+      const int l = i - p * M;
+      if (l < mcols) slab[(int64_t)p * ncol + col0 + l] = res[p * ld + l];
+    }
+  } else {  // T1[col][z]
+    C2* out = slab + (int64_t)col0 * nz;
+    for (int i = threadIdx.x; i < mcols * nz; i += blockDim.x) {
+      const int l = i / nz;
+      const int p = i - l * nz;
+      out[i] = res[p * ld + l];
+    }
   }
 }
 
@@ -297,8 +310,77 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// ---------------------------------------------------------------- pass E
+// ------------------------------------------------------ plane pass (B-C-D)
+// When an x-y plane fits in LDS, passes B, C and D fuse into one kernel over
+// T1 in z-major layout (T1[k][band][z][col]). One workgroup: the plane z of
+// one band of one k, row stride odd_ld(ny). The active columns at this z
+// are scattered into the zeroed plane, then in place: inverse y-FFT along
+// the active x rows, inverse x-FFT along every y, x V(x, y, z) (V given
+// z-major, one box per potential slot), forward x-FFT, forward y-FFT along
+// the active rows; the active columns are gathered back to the same place.
+// HBM traffic per band is T1 in and out (V stays cached across bands).
+constexpr int kPlaneThreads = 512;
+constexpr int kPlaneMaxE = 24;  // complex values per thread in one FFT pass
+
 template <typename C2, typename R>
+__global__ void __launch_bounds__(kPlaneThreads)
+    k_plane_potential(C2* __restrict__ T1, const R* __restrict__ Vt,
+                      const int* __restrict__ vslot, const int* __restrict__ col_ix,
+                      const int* __restrict__ col_iy, const int* __restrict__ x_list,
+                      const C2* __restrict__ twx_g, const C2* __restrict__ twy_g, Radices rx,
+                      Radices ry, FastDiv fny, FastDiv fnxa, int ncol, int nxa, int nb) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nx = rx.n, ny = ry.n, ldp = odd_ld(ny), nz = gridDim.x;
+  C2* P = reinterpret_cast<C2*>(smem_raw);
+  C2* twx = P + nx * ldp;
+  C2* twy = twx + nx;
+  int* xl = reinterpret_cast<int*>(twy + ny);
+  const int z = blockIdx.x, ib = blockIdx.y, k = blockIdx.z;
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) twx[i] = twx_g[i];
+  for (int i = threadIdx.x; i < ny; i += blockDim.x) twy[i] = twy_g[i];
+  for (int i = threadIdx.x; i < nxa; i += blockDim.x) xl[i] = x_list[i];
+  zero_lds(P, nx * ldp);
+  __syncthreads();
+  C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) P[col_ix[c] * ldp + col_iy[c]] = row[c];
+  __syncthreads();
+  fft_ip<C2, +1, kPlaneThreads, kPlaneMaxE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
+  fft_ip<C2, +1, kPlaneThreads, kPlaneMaxE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
+  const R* Vz = Vt + ((int64_t)vslot[k] * nz + z) * nx * ny;
+  for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
+    const int x = fdiv(i, fny);
+    const int y = i - x * ny;
+    P[x * ldp + y] = cscale(P[x * ldp + y], Vz[i]);
+  }
+  __syncthreads();
+  fft_ip<C2, -1, kPlaneThreads, kPlaneMaxE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
+  fft_ip<C2, -1, kPlaneThreads, kPlaneMaxE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) row[c] = P[col_ix[c] * ldp + col_iy[c]];
+}
+
+// Vt[s][z][xy] = V[s][xy][z]: each potential box to z-major order (32 x 32
+// tiles through LDS, both sides coalesced).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_potential_zmajor(const R* __restrict__ V, R* __restrict__ Vt, int nxy, int nz) {
+  __shared__ R tile[32][33];
+  const int s = blockIdx.z, xy0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const R* Vs = V + (int64_t)s * nxy * nz;
+  R* Vts = Vt + (int64_t)s * nxy * nz;
+  for (int r = ty; r < 32; r += 8) {
+    const int xy = xy0 + r, zz = z0 + tx;
+    if (xy < nxy && zz < nz) tile[r][tx] = Vs[(int64_t)xy * nz + zz];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int zz = z0 + r, xy = xy0 + tx;
+    if (xy < nxy && zz < nz) Vts[(int64_t)zz * nxy + xy] = tile[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------- pass E
+template <typename C2, typename R, bool ZMAJOR>
 __global__ void __launch_bounds__(kThreads)
     k_cols_to_sphere(const C2* __restrict__ T1, const C2* C, int64_t ldb, int64_t ldk,
                      C2* out, int64_t oldb, int64_t oldk, const R* __restrict__ ke,
@@ -314,11 +396,20 @@ __global__ void __launch_bounds__(kThreads)
   const int col0 = blockIdx.x * M, ib = blockIdx.y, k = blockIdx.z;
   const int mcols = min(M, ncol - col0);
   load_twiddles(tw, tw_g, nz);
-  const C2* src = T1 + ((int64_t)k * nb + ib) * ncol * nz + (int64_t)col0 * nz;
-  for (int i = threadIdx.x; i < M * nz; i += blockDim.x) {
-    const int l = i / nz;
-    const int p = i - l * nz;
-    a[p * ld + l] = l < mcols ? src[i] : C2{0, 0};
+  const C2* slab = T1 + ((int64_t)k * nb + ib) * ncol * nz;
+  if (ZMAJOR) {
+    for (int i = threadIdx.x; i < M * nz; i += blockDim.x) {
+      const int p = fdiv(i, fm);
+      const int l = i - p * M;
+      a[p * ld + l] = l < mcols ? slab[(int64_t)p * ncol + col0 + l] : C2{0, 0};
+    }
+  } else {
+    const C2* src = slab + (int64_t)col0 * nz;
+    for (int i = threadIdx.x; i < M * nz; i += blockDim.x) {
+      const int l = i / nz;
+      const int p = i - l * nz;
+      a[p * ld + l] = l < mcols ? src[i] : C2{0, 0};
+    }
   }
   __syncthreads();
   C2* res = fft_lds<C2, -1>(a, b, tw, rz, fm, ld);
@@ -463,6 +554,44 @@ const double2* tw_of<double2>(const qmx_plan* p, int axis) {
   return p->d_tw64[axis];
 }
 
+// Plane path (fused B-C-D) when a plane fits in LDS and every FFT pass
+// fits the per-thread register budget; plan option 1 forces the separate
+// passes, 2 asks for the plane path where it fits.
+static size_t plane_lds_bytes(const qmx_plan* p, int cs) {
+  const int nx = p->shape[0], ny = p->shape[1];
+  return (size_t)nx * odd_ld(ny) * cs + (size_t)(nx + ny) * cs + (size_t)p->nxa * sizeof(int);
+}
+
+static bool plane_fits(const qmx_plan* p, int cs) {
+  return p->nxa > 0 && plane_lds_bytes(p, cs) <= (size_t)kLdsMax &&
+         fft_ip_fits(p->rad[1], p->nxa, kPlaneThreads, kPlaneMaxE) &&
+         fft_ip_fits(p->rad[0], p->shape[1], kPlaneThreads, kPlaneMaxE);
+}
+
+static bool use_plane_path(const qmx_plan* p, int cs) {
+  return p->local_path != 1 && plane_fits(p, cs);
+}
+
+static int64_t plane_aware_band_bytes(const qmx_plan* p, int cs, bool plane) {
+  const int64_t nz = p->shape[2], ny = p->shape[1];
+  return plane ? (int64_t)p->ncol * nz * cs : ((int64_t)p->ncol * nz + (int64_t)p->nxa * ny * nz) * cs;
+}
+
+static int64_t vt_bytes(const qmx_plan* p, int rs) {
+  return ((int64_t)p->n_vslots * p->N * rs + 255) / 256 * 256;
+}
+
+template <typename C2, typename R>
+static int set_plane_lds(size_t bytes) {
+  static size_t done = 0;
+  if (bytes > done) {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_potential<C2, R>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done = bytes;
+  }
+  return QMX_OK;
+}
+
 template <typename R>
 int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t ldb, int64_t ldk,
                      void* outv, int64_t oldb, int64_t oldk, const void* Vv, const void* kev,
@@ -475,8 +604,11 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
   const int* nvec = p->d_n + k0;
   const int* vslot = p->d_vslot + k0;
-  const int64_t per_band = ((int64_t)ncol * nz + (int64_t)nxa * ny * nz) * sizeof(C2);
-  int64_t blk = ws_bytes / (per_band * nk);
+  const bool plane = use_plane_path(p, sizeof(C2));
+  const int64_t per_band = plane_aware_band_bytes(p, sizeof(C2), plane);
+  const int64_t fixed = plane ? vt_bytes(p, sizeof(R)) : 0;
+  QMX_CHECK_ARG(ws_bytes > fixed, "qmx_apply_local: workspace smaller than the potential boxes");
+  int64_t blk = (ws_bytes - fixed) / (per_band * nk);
   QMX_CHECK_ARG(blk >= 1, "qmx_apply_local: workspace smaller than one band of every k");
   blk = std::min<int64_t>(blk, nb);
   blk = std::min<int64_t>(blk, 65535 / nk);  // grid.z of passes B and D is nk * blk
@@ -518,17 +650,49 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   const double bc = (double)cs, br = (double)sizeof(R);
   const double t1_elems = (double)nk * ncol * nz, t2_elems = (double)nk * nxa * ny * nz;
   hipEvent_t ev = nullptr;
+  const R* Vt = nullptr;
+  if (plane) {
+    // z-major copy of every potential box at the start of the workspace
+    R* vt = static_cast<R*>(ws);
+    const int nxy = nx * ny;
+    dim3 gT((nxy + 31) / 32, (nz + 31) / 32, p->n_vslots);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_potential_zmajor<R>), gT, dim3(256), 0, st, V, vt, nxy, nz);
+    QMX_LAUNCHED();
+    prof_end(st, ev, QMX_KERNEL_VZMAJOR, 2.0 * p->n_vslots * (double)p->N * br);
+    Vt = vt;
+  }
+  const size_t shP = plane_lds_bytes(p, sizeof(C2));
+  if (plane) {
+    const int rc = set_plane_lds<C2, R>(shP);
+    if (rc) return rc;
+  }
+  const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
   for (int64_t b0 = 0; b0 < nb; b0 += blk) {
     const int cur = (int)std::min<int64_t>(blk, nb - b0);
-    C2* T1 = static_cast<C2*>(ws);
+    C2* T1 = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
     C2* T2 = T1 + (int64_t)nk * cur * ncol * nz;
     dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
     prof_begin(st, &ev);
-    hipLaunchKernelGGL((k_sphere_to_cols<C2, R>), gA, dim3(kThreads), shA, st, C, ldb, ldk, T1,
-                       t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
-                       p->n_tot, (int)b0);
+    if (plane)
+      hipLaunchKernelGGL((k_sphere_to_cols<C2, R, true>), gA, dim3(kThreads), shA, st, C, ldb,
+                         ldk, T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                         p->n_tot, (int)b0);
+    else
+      hipLaunchKernelGGL((k_sphere_to_cols<C2, R, false>), gA, dim3(kThreads), shA, st, C, ldb,
+                         ldk, T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                         p->n_tot, (int)b0);
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_A, cur * (n_sphere * bc + t1_elems * bc));
+    if (plane) {
+      dim3 gP(nz, cur, nk);
+      prof_begin(st, &ev);
+      hipLaunchKernelGGL((k_plane_potential<C2, R>), gP, dim3(kPlaneThreads), shP, st, T1, Vt,
+                         vslot, p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
+                         tw_of<C2>(p, 1), p->rad[0], p->rad[1], fny, fnxa, ncol, nxa, cur);
+      QMX_LAUNCHED();
+      prof_end(st, ev, QMX_KERNEL_PLANE, cur * 2.0 * t1_elems * bc + (double)nk * p->N * br);
+    } else {
     dim3 gB((nz + My - 1) / My, nxa, nk * cur);
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_cols_to_planes<C2, R>), gB, dim3(kThreads), shB, st, T1, T2,
@@ -558,11 +722,18 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                        nxa, nz);
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
+    }
     prof_begin(st, &ev);
-    hipLaunchKernelGGL((k_cols_to_sphere<C2, R>), gA, dim3(kThreads), shA, st, T1, C, ldb, ldk,
-                       out, oldb, oldk, ke, ke_ldk, t1idx, cstart, nvec,
-                       tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
-                       accumulate);
+    if (plane)
+      hipLaunchKernelGGL((k_cols_to_sphere<C2, R, true>), gA, dim3(kThreads), shA, st, T1, C,
+                         ldb, ldk, out, oldb, oldk, ke, ke_ldk, t1idx, cstart, nvec,
+                         tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
+                         accumulate);
+    else
+      hipLaunchKernelGGL((k_cols_to_sphere<C2, R, false>), gA, dim3(kThreads), shA, st, T1, C,
+                         ldb, ldk, out, oldb, oldk, ke, ke_ldk, t1idx, cstart, nvec,
+                         tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
+                         accumulate);
     QMX_LAUNCHED();
     prof_end(st, ev, QMX_KERNEL_PASS_E,
              cur * (t1_elems * bc + n_sphere * bc * (1 + (ke ? 1 : 0) + (accumulate ? 1 : 0)) +
@@ -647,7 +818,7 @@ static int upload(T** dst, const std::vector<T>& src) {
 static void free_plan(qmx_plan* p) {
   if (!p) return;
   int* arrs[] = {p->d_x_list, p->d_pcol_start, p->d_col_iy, p->d_cstart,
-                 p->d_t1idx,  p->d_n,          p->d_vslot};
+                 p->d_t1idx,  p->d_n,          p->d_vslot,  p->d_col_ix};
   for (int* a : arrs)
     if (a) (void)hipFree(a);
   delete[] p->h_nsum;
@@ -796,10 +967,13 @@ int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
       p->n_vslots = std::max(p->n_vslots, vslot[k] + 1);
     }
   }
+  std::vector<int> col_ix(t.col_iy.size());
+  for (size_t xa = 0; xa < t.x_list.size(); ++xa)
+    for (int c = t.pcol_start[xa]; c < t.pcol_start[xa + 1]; ++c) col_ix[c] = t.x_list[xa];
   if ((rc = upload(&p->d_x_list, t.x_list)) || (rc = upload(&p->d_pcol_start, t.pcol_start)) ||
       (rc = upload(&p->d_col_iy, t.col_iy)) || (rc = upload(&p->d_cstart, t.cstart)) ||
       (rc = upload(&p->d_t1idx, t.t1idx)) || (rc = upload(&p->d_n, nvec)) ||
-      (rc = upload(&p->d_vslot, vslot))) {
+      (rc = upload(&p->d_vslot, vslot)) || (rc = upload(&p->d_col_ix, col_ix))) {
     free_plan(p);
     return rc;
   }
@@ -826,9 +1000,21 @@ int qmx_plan_destroy(qmx_plan* plan) {
 
 int64_t qmx_plan_workspace_bytes(const qmx_plan* p, int dtype, int64_t n_bands) {
   if (!p || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
-  const int64_t cs = dtype == QMX_C64 ? 8 : 16;
-  const int64_t nz = p->shape[2], ny = p->shape[1];
-  return ((int64_t)p->ncol * nz + (int64_t)p->nxa * ny * nz) * cs * p->nk * n_bands;
+  const int cs = dtype == QMX_C64 ? 8 : 16;
+  const bool plane = use_plane_path(p, cs);
+  const int64_t fixed = plane ? vt_bytes(p, cs / 2) : 0;
+  return fixed + plane_aware_band_bytes(p, cs, plane) * p->nk * n_bands;
+}
+
+int qmx_plan_set_local_path(qmx_plan* p, int path) {
+  QMX_CHECK_ARG(p && path >= 0 && path <= 2, "qmx_plan_set_local_path: path must be 0, 1 or 2");
+  p->local_path = path;
+  return QMX_OK;
+}
+
+int qmx_plan_local_path(const qmx_plan* p, int dtype) {
+  if (!p || (dtype != QMX_C64 && dtype != QMX_C128)) return -1;
+  return use_plane_path(p, dtype == QMX_C64 ? 8 : 16) ? 2 : 1;
 }
 
 int qmx_plan_info(const qmx_plan* p, int64_t* info) {

@@ -118,6 +118,24 @@ def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch
     return E.reshape(batch + (n,)), V.reshape(batch + (n, n))
 
 
+def eigh_nosync(H: torch.Tensor):
+    """(E, V, info) like eigh for complex128 (..., n, n) on the GPU, without
+    a host synchronisation: info is None or an int32 tensor per matrix,
+    nonzero where the QL iteration did not converge, for the caller to check
+    at its next synchronisation point."""
+    n = H.shape[-1]
+    n_mat = 1
+    for b in H.shape[:-2]:
+        n_mat *= int(b)
+    if H.is_cuda and 0 < n <= SMALL_QL_MAX_N:
+        method = small_method(n, n_mat)
+        if method != "library":
+            E, V = eigh_small(H, method)
+            return E, V, (eigh_small.last_info if method == "ql" else None)
+    E, V = eigh(H)
+    return E, V, None
+
+
 def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Hermitian eigendecomposition in FP64 (lower triangle, ascending), like
     torch.linalg.eigh. On the GPU: n <= SMALL_QL_MAX_N runs the whole batch

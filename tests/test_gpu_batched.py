@@ -87,3 +87,17 @@ def test_jobs_join_running_bucket(gpu, dense_refs):
     assert max(sizes) > min(s for s in sizes)  # membership changed while running
     for i, ref in enumerate(dense_refs):
         assert np.abs(solver.results[i].eig - ref).max() < TOL[C64]
+
+
+def test_served_stream_matches_dense_and_alone(gpu, dense_refs):
+    """Requests arriving over time (continuous batching): every result equals
+    the dense answer and its own result served alone, within tolerance."""
+    from qimpy_amd.serve import serve
+    specs = _specs()
+    arrivals = [0.0, 0.01, 0.02, 0.05, 0.08]
+    rep = serve(specs, arrivals, _solver(C64))
+    assert rep.n_requests == len(specs) and rep.p99_s >= rep.p50_s > 0
+    for i, ref in enumerate(dense_refs):
+        assert np.abs(rep.results[i].eig - ref).max() < TOL[C64]
+    alone = serve([specs[2]], [0.0], _solver(C64)).results[0]
+    assert np.abs(alone.eig - rep.results[2].eig).max() < TOL[C64]

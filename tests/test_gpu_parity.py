@@ -301,3 +301,21 @@ def test_eigh_dispatch(gpu, shape):
     scale = np.abs(ref).max()
     assert np.abs(E.numpy() - ref).max() < 1e-12 * scale
     assert (H @ V - V * E[:, None, :]).abs().max().item() < 1e-11 * scale
+
+
+def test_ql_large_batch_global_accumulator(gpu):
+    """n > 128 keeps the QL accumulator in the workspace, one region per
+    matrix: a batch of 48 at n = 144 (the largest subspace of 72-band
+    cells) must give every matrix's own eigenpairs."""
+    from qimpy_amd.linalg import eigh_small
+    n, B = 144, 48
+    g = torch.Generator().manual_seed(144)
+    A = torch.randn(B, n, n, dtype=C128, generator=g)
+    H = A + A.conj().transpose(-2, -1)
+    E, V = eigh_small(H.cuda(), "ql")
+    assert int(eigh_small.last_info.abs().sum().item()) == 0
+    E, V = E.cpu(), V.cpu()
+    ref = np.linalg.eigvalsh(H.numpy())
+    scale = np.abs(ref).max()
+    assert np.abs(E.numpy() - ref).max() < 1e-12 * scale
+    assert (H @ V - V * E[:, None, :]).abs().max().item() < 1e-11 * scale

@@ -96,3 +96,17 @@ def test_geom_errors():
         _lib.geom_tables((7, 4, 4), np.array([1]), np.array([[0]], dtype=np.int64))
     with pytest.raises(RuntimeError, match="n\\[k\\]"):
         _lib.geom_tables((4, 4, 4), np.array([5]), np.array([[0, 1, 2]], dtype=np.int64))
+
+
+@pytest.mark.parametrize("n", [1, 2, 37, 127, 128, 129, 143, 144, 200, 256])
+def test_heev_small_workspace_contract(n):
+    """Workspace of the batched QL eigensolver: per matrix at least the complex
+    working copy (16 n^2 bytes) plus, above n = 128, the real QL accumulator
+    (8 n^2 bytes); linear in the batch, 256-byte aligned per matrix (the
+    kernel offsets matrix b by b times the one-matrix size)."""
+    lib = _lib.lib()
+    one = int(lib.qmx_heev_small_workspace_bytes(n, 1))
+    assert one >= 16 * n * n + (8 * n * n if n > 128 else 0)
+    assert one % 256 == 0
+    for batch in (0, 3, 100):
+        assert int(lib.qmx_heev_small_workspace_bytes(n, batch)) == batch * one
