@@ -201,6 +201,30 @@ int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int64_t stride
                            int64_t batch, double* E, void* V, int64_t ldv, int64_t stride_v,
                            int* info, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Many Hermitian matrices of different sizes (n <= 256) in one launch of
+ * the qmx_heev_small_batched algorithm, one workgroup per descriptor; the
+ * launch costs about the slowest matrix instead of the sum. `desc` is a
+ * DEVICE array of `count` descriptors, every n <= n_max. Each matrix needs
+ * its own work area of qmx_heev_small_work_bytes(n) bytes (256-byte
+ * aligned). A is read (lower triangle, row stride lda), E gets the n
+ * ascending eigenvalues, V the eigenvectors in columns (row stride ldv),
+ * *info (if not NULL) is nonzero when QL did not converge. Used by the
+ * batched cell solver (no counterpart in the reference, which solves one
+ * cell at a time). */
+typedef struct qmx_heev_desc {
+  const void* A;
+  double* E;
+  void* V;
+  int32_t* info;
+  void* work;
+  int64_t lda;
+  int64_t ldv;
+  int32_t n;
+  int32_t reserved;
+} qmx_heev_desc;
+int64_t qmx_heev_small_work_bytes(int64_t n);
+int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
+
 /* Kernel ids reported by the profiler. */
 enum {
   QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */

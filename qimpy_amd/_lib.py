@@ -69,6 +69,8 @@ SIGNATURES = {
     "qmx_heevj_batched": (i32, [vp, i64, i64, i64, i64, vp, vp, i64, i64, vp, i64, i32, vp]),
     "qmx_heev_small_workspace_bytes": (i64, [i64, i64]),
     "qmx_heev_small_batched": (i32, [vp, i64, i64, i64, i64, vp, vp, i64, i64, vp, vp, i64, vp]),
+    "qmx_heev_small_work_bytes": (i64, [i64]),
+    "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_profile_enable": (i32, [i32]),
     "qmx_profile_read": (i32, [vp, vp, vp, i32]),
     "qmx_last_error": (ct.c_char_p, []),

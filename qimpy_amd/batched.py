@@ -54,7 +54,7 @@ import torch
 from . import _lib, gemm, rc
 from .basis import DTYPE_CODE, REAL_OF
 from .ions import build_projectors
-from .linalg import dagger, eigh_nosync
+from .linalg import dagger, eigh_grouped, eigh_nosync
 from .systems import signed_freqs
 
 C64, C128 = torch.complex64, torch.complex128
@@ -469,29 +469,41 @@ class Bucket:
                 done.append(j)
         return done
 
-    def _eighg_chol(self, H: torch.Tensor, O: torch.Tensor):
-        """FP64 generalized eigenproblem H V = O V E for overlaps known to be
-        close to the identity (Cholesky without a host check)."""
+    def _chol_reduce(self, H: torch.Tensor, O: torch.Tensor):
+        """Li and Li H Li^H for H V = O V E with O close to the identity
+        (Cholesky without a host check); then V = Li^H W."""
         L = torch.linalg.cholesky_ex(O)[0]
         eye = torch.eye(O.shape[-1], dtype=O.dtype, device=O.device).expand_as(L)
         Li = torch.linalg.solve_triangular(L, eye, upper=False)
         Hh = Li @ H @ dagger(Li)
-        E, W = self._subspace_eigh(0.5 * (Hh + dagger(Hh)))
+        return Li, 0.5 * (Hh + dagger(Hh))
+
+    def _eighg_chol(self, H: torch.Tensor, O: torch.Tensor):
+        """FP64 generalized eigenproblem H V = O V E, O close to the identity."""
+        Li, Hh = self._chol_reduce(H, O)
+        E, W = self._subspace_eigh(Hh)
         return E, dagger(Li) @ W
 
-    def expand(self) -> None:
-        """One Davidson step on every row (_davidson.py:180-233, no locking)."""
+    def expand_steps(self):
+        """One Davidson step on every row (_davidson.py:180-233, no locking),
+        as a generator that yields its two FP64 Hermitian eigenproblems
+        (residual-block overlap, then the projected H) and receives their
+        (eigenvalues, eigenvectors), so that a solver can run every bucket's
+        eigenproblems in one launch."""
         nb, dt = self.nb, self.dtype
         C, R = self.X[:, :nb], self.X[:, nb:]
-        HC, HR = self.HX[:, :nb], self.HX[:, nb:]
+        HR = self.HX[:, nb:]
         # Normalised residuals; a null one (below 1e-30, where 1/norm would
         # overflow float32) is zeroed and then dropped with the null space:
         norm = self.norm
         _scale_constrain(R, torch.where(norm > 1e-30, 1.0 / norm.clamp_min(1e-30), 0.0), self.n32)
         for _ in range(2):  # R <- (1 - C C^H) R, twice
             R.baddbmm_(_dot(C, R).transpose(-2, -1), C, alpha=-1)
-        rtol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
-        U, keep = self._orthonormalizer(_dot(R, R).to(C128), rtol, relative=False)
+        # Canonical orthonormalisation of R (drops near-dependent directions):
+        lam, W = yield _dot(R, R).to(C128)
+        tol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
+        keep = lam > tol
+        U = W * torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))[..., None, :]
         R.copy_(torch.matmul(U.to(dt).transpose(-2, -1), R))
         self.hamiltonian(R, HR, 0, self.Kb)
         XO = _dot(self.X, R).to(C128)  # (Kb, 2 nb, nb): [C^H R; R^H R]
@@ -510,8 +522,9 @@ class Bucket:
         drop = ~keep
         O.diagonal(dim1=-2, dim2=-1)[:, nb:] += drop.to(torch.float64)
         H.diagonal(dim1=-2, dim2=-1)[:, nb:] += torch.where(drop, self.DROP_SHIFT, 0.0)
-        E, Vs = self._eighg_chol(H, O)
-        Vt = Vs[:, :, :nb].to(dt).transpose(-2, -1)
+        Li, Hh = self._chol_reduce(H, O)
+        E, W2 = yield Hh
+        Vt = (dagger(Li) @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
         torch.matmul(Vt, self.X, out=self.X2[:, :nb])
         torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
@@ -522,6 +535,16 @@ class Bucket:
         for j in self.jobs:
             j.iterations += 1
         self.n_ticks += 1
+
+    def expand(self) -> None:
+        """expand_steps driven with this bucket's own eigensolves."""
+        steps = self.expand_steps()
+        mat = next(steps)
+        try:
+            while True:
+                mat = steps.send(self._subspace_eigh(mat))
+        except StopIteration:
+            pass
 
     # ------------------------------------------------------------ results
     def finish(self, job: CellJob) -> "_Pending":
@@ -598,6 +621,7 @@ class BatchSolver:
         self.pending: list[_Pending] = []
         self.hbm_in_use = 0
         self._held: dict[int, int] = {}
+        self._info: list[torch.Tensor] = []
         self.n_ticks = 0
         torch.cuda.synchronize()
         self._t0_event = torch.cuda.Event(enable_timing=True)
@@ -679,7 +703,13 @@ class BatchSolver:
         prev = self.pending
         self.pending = []
         host_prev = [p.data.to("cpu", non_blocking=True) for p in prev]
+        fails = None
+        if self._info:
+            fails = torch.stack([(i != 0).sum() for i in self._info]).sum().to("cpu", non_blocking=True)
+            self._info = []
         torch.cuda.current_stream().synchronize()
+        if fails is not None and int(fails) != 0:
+            raise RuntimeError(f"grouped QL eigensolver did not converge ({int(fails)} matrices)")
         self._collect(prev, [h.numpy() for h in host_prev])
         new_results = [self.results[p.job.job_id] for p in prev]
         host = [h.numpy() for h in host_packs]
@@ -707,8 +737,20 @@ class BatchSolver:
                     continue
                 b._initialise(new)
                 b.residuals()
-            if b.jobs:
-                b.expand()
+        # One Davidson step of every bucket; their eigenproblems run grouped.
+        steps = [b.expand_steps() for b in self.buckets.values() if b.jobs]
+        mats = [next(g) for g in steps]
+        while steps:
+            results, info = eigh_grouped(mats)
+            self._info.append(info)
+            nxt_steps, nxt_mats = [], []
+            for g, r in zip(steps, results):
+                try:
+                    nxt_mats.append(g.send(r))
+                    nxt_steps.append(g)
+                except StopIteration:
+                    pass
+            steps, mats = nxt_steps, nxt_mats
         self.n_ticks += 1
         return new_results
 

@@ -503,11 +503,12 @@ __device__ __forceinline__ double block_sum_small(double x, double* red) {
   return s;
 }
 
+// One matrix per workgroup: A (row stride lda, lower triangle read) ->
+// E ascending, V (row stride ldv), work = small_ws_stride(n) bytes.
 template <bool ZLDS>
-__global__ void __launch_bounds__(kSmallThreads)
-    k_heev_small(const double2* __restrict__ A, int n, int64_t lda, int64_t stride_a,
-                 double* __restrict__ E, double2* __restrict__ Vout, int64_t ldv,
-                 int64_t stride_v, double2* __restrict__ Wws, int max_iter, int* __restrict__ info) {
+__device__ void heev_small_one(const double2* __restrict__ Ab, int n, int64_t lda,
+                               double* __restrict__ Eb, double2* __restrict__ Vb, int64_t ldv,
+                               double2* __restrict__ W, int max_iter, int* __restrict__ infob) {
   constexpr int NW = kSmallThreads / 64;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int ldz = ZLDS ? n + 1 : n;
@@ -524,9 +525,6 @@ __global__ void __launch_bounds__(kSmallThreads)
   int* perm = li + n;
   int* ctl = perm + n;  // [0] log length, [1] done flag, [2] failure
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t b = blockIdx.x;
-  const double2* Ab = A + b * stride_a;
-  double2* W = reinterpret_cast<double2*>(reinterpret_cast<char*>(Wws) + b * small_ws_stride(n));
   // Global Z: n * n doubles right after this matrix's n * n complex W.
   double* Z = ZLDS ? Zl : reinterpret_cast<double*>(W + (int64_t)n * n);
 
@@ -713,11 +711,10 @@ __global__ void __launch_bounds__(kSmallThreads)
     int rank = 0;
     for (int j = 0; j < n; ++j) rank += sorts_before(d[j], j, li_, i);
     perm[rank] = i;
-    E[b * n + rank] = li_;
+    Eb[rank] = li_;
   }
-  if (tid == 0 && info) info[b] = ctl[2];
+  if (tid == 0 && infob) *infob = ctl[2];
   __syncthreads();
-  double2* Vb = Vout + b * stride_v;
   for (int j = tid; j < n; j += kSmallThreads) {
     const int pj = perm[j];
     for (int i = 0; i < n; ++i) Vb[(int64_t)i * ldv + j] = double2{Z[(int64_t)i * ldz + pj], 0.0};
@@ -748,6 +745,34 @@ __global__ void __launch_bounds__(kSmallThreads)
   }
 }
 
+template <bool ZLDS>
+__global__ void __launch_bounds__(kSmallThreads)
+    k_heev_small(const double2* __restrict__ A, int n, int64_t lda, int64_t stride_a,
+                 double* __restrict__ E, double2* __restrict__ Vout, int64_t ldv,
+                 int64_t stride_v, double2* __restrict__ Wws, int max_iter, int* __restrict__ info) {
+  const int64_t b = blockIdx.x;
+  heev_small_one<ZLDS>(A + b * stride_a, n, lda, E + b * n, Vout + b * stride_v, ldv,
+                       reinterpret_cast<double2*>(reinterpret_cast<char*>(Wws) + b * small_ws_stride(n)),
+                       max_iter, info ? info + b : nullptr);
+}
+
+// Matrices of different sizes in one launch, one descriptor each.
+template <bool ZLDS>
+__global__ void __launch_bounds__(kSmallThreads)
+    k_heev_small_grouped(const qmx_heev_desc* __restrict__ desc, int max_iter) {
+  const qmx_heev_desc d = desc[blockIdx.x];
+  heev_small_one<ZLDS>(static_cast<const double2*>(d.A), d.n, d.lda, d.E,
+                       static_cast<double2*>(d.V), d.ldv, static_cast<double2*>(d.work), max_iter,
+                       d.info);
+}
+
+static size_t small_lds_bytes(int64_t n) {
+  const bool zlds = n <= kSmallLdsN;
+  return (size_t)3 * n * sizeof(double2) + (zlds ? (size_t)n * (n + 1) * sizeof(double) : 0) +
+         (size_t)4 * n * sizeof(double) + 4 * sizeof(double) + (size_t)2 * n * sizeof(int) +
+         4 * sizeof(int);
+}
+
 }  // namespace
 
 extern "C" int64_t qmx_heev_small_workspace_bytes(int64_t n, int64_t batch) {
@@ -767,10 +792,7 @@ extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int
                 "qmx_heev_small_batched: workspace too small");
   if (batch == 0) return QMX_OK;
   const bool zlds = n <= kSmallLdsN;
-  const size_t sh = (size_t)3 * n * sizeof(double2) +
-                    (zlds ? (size_t)n * (n + 1) * sizeof(double) : 0) +
-                    (size_t)4 * n * sizeof(double) + 4 * sizeof(double) +
-                    (size_t)2 * n * sizeof(int) + 4 * sizeof(int);
+  const size_t sh = small_lds_bytes(n);
   const void* fn = zlds ? (const void*)k_heev_small<true> : (const void*)k_heev_small<false>;
   QMX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -784,6 +806,32 @@ extern "C" int qmx_heev_small_batched(const void* A, int64_t n, int64_t lda, int
                        static_cast<const double2*>(A), (int)n, lda, stride_a, E,
                        static_cast<double2*>(V), ldv, stride_v, static_cast<double2*>(workspace),
                        60, info);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
+extern "C" int64_t qmx_heev_small_work_bytes(int64_t n) {
+  if (n < 1 || n > kSmallMaxN) return -1;
+  return (n * n * 24 + 255) / 256 * 256;  // complex copy + real accumulator, any launch variant
+}
+
+extern "C" int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max,
+                                      void* stream) {
+  QMX_CHECK_ARG(desc || count == 0, "qmx_heev_small_grouped: null descriptors");
+  QMX_CHECK_ARG(count >= 0 && count < (1ll << 31), "qmx_heev_small_grouped: bad count");
+  QMX_CHECK_ARG(n_max >= 1 && n_max <= kSmallMaxN, "qmx_heev_small_grouped: n_max must be in 1..256");
+  if (count == 0) return QMX_OK;
+  const bool zlds = n_max <= kSmallLdsN;
+  const size_t sh = small_lds_bytes(n_max);
+  const void* fn = zlds ? (const void*)k_heev_small_grouped<true> : (const void*)k_heev_small_grouped<false>;
+  QMX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (zlds)
+    hipLaunchKernelGGL(k_heev_small_grouped<true>, dim3((unsigned)count), dim3(kSmallThreads), sh,
+                       st, desc, 60);
+  else
+    hipLaunchKernelGGL(k_heev_small_grouped<false>, dim3((unsigned)count), dim3(kSmallThreads),
+                       sh, st, desc, 60);
   QMX_LAUNCHED();
   return QMX_OK;
 }

@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as ct
 from typing import Protocol, TypeVar, Generic
 
+import numpy as np
 import torch
 
 from . import _lib, rc
@@ -116,6 +117,60 @@ def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch
         # QL non-convergence is checked lazily by callers that synchronise anyway.
         eigh_small.last_info = info
     return E.reshape(batch + (n,)), V.reshape(batch + (n, n))
+
+
+_HEEV_DESC = np.dtype([("A", np.uint64), ("E", np.uint64), ("V", np.uint64), ("info", np.uint64),
+                       ("work", np.uint64), ("lda", np.int64), ("ldv", np.int64), ("n", np.int32),
+                       ("reserved", np.int32)])
+
+
+def eigh_grouped(mats: list) -> tuple[list, torch.Tensor]:
+    """Eigendecompositions of several batches of Hermitian complex128
+    matrices (each (..., n_i, n_i), n_i <= SMALL_QL_MAX_N, on the GPU) in one
+    launch per accumulator placement (n <= 128 in LDS, above in the
+    workspace), without a host synchronisation. Returns [(E, V)] in the
+    input order and an int32 tensor, nonzero where QL did not converge."""
+    dev = rc.device
+    flat = [_fp64(H).reshape((-1,) + tuple(H.shape[-2:])).contiguous() for H in mats]
+    outs = []
+    for H in flat:
+        B, n = H.shape[0], H.shape[-1]
+        if n > SMALL_QL_MAX_N:
+            raise ValueError(f"eigh_grouped: n = {n} > {SMALL_QL_MAX_N}")
+        outs.append((torch.empty((B, n), dtype=torch.float64, device=dev),
+                     torch.empty((B, n, n), dtype=torch.complex128, device=dev)))
+    count = sum(H.shape[0] for H in flat)
+    info = torch.zeros(max(count, 1), dtype=torch.int32, device=dev)
+    if count == 0:
+        return [(E.reshape(H.shape[:-1]), V.reshape(H.shape)) for H, (E, V) in zip(mats, outs)], info
+    work_of = {}
+    total = 0
+    for H in flat:
+        n = H.shape[-1]
+        if n not in work_of:
+            work_of[n] = int(_lib.lib().qmx_heev_small_work_bytes(n))
+        total += work_of[n] * H.shape[0]
+    ws = torch.empty(total, dtype=torch.uint8, device=dev)
+    groups = {True: [], False: []}
+    off, i_info = 0, 0
+    for H, (E, V) in zip(flat, outs):
+        B, n = H.shape[0], H.shape[-1]
+        for b in range(B):
+            groups[n <= 128].append((H.data_ptr() + b * n * n * 16, E.data_ptr() + b * n * 8,
+                                     V.data_ptr() + b * n * n * 16, info.data_ptr() + 4 * i_info,
+                                     ws.data_ptr() + off, n, n, n, 0))
+            off += work_of[n]
+            i_info += 1
+    keep = []
+    for small, rows in groups.items():
+        if not rows:
+            continue
+        arr = np.array(rows, dtype=_HEEV_DESC)
+        d = torch.from_numpy(arr.view(np.uint8)).pin_memory().to(dev, non_blocking=True)
+        keep.append(d)
+        n_max = int(arr["n"].max())
+        _lib.call("qmx_heev_small_grouped", d.data_ptr(), len(rows), n_max, rc.stream_ptr())
+    return [(E.reshape(H.shape[:-1]), V.reshape(H.shape)) for H, (E, V) in zip(mats, outs)], info
 
 
 def eigh_nosync(H: torch.Tensor):

@@ -319,3 +319,25 @@ def test_ql_large_batch_global_accumulator(gpu):
     scale = np.abs(ref).max()
     assert np.abs(E.numpy() - ref).max() < 1e-12 * scale
     assert (H @ V - V * E[:, None, :]).abs().max().item() < 1e-11 * scale
+
+
+def test_eigh_grouped_mixed_sizes(gpu):
+    """One grouped launch over batches of different n (both accumulator
+    placements): every matrix matches LAPACK; an empty batch is allowed."""
+    from qimpy_amd.linalg import eigh_grouped
+    g = torch.Generator().manual_seed(7)
+    mats = []
+    for B, n in ((3, 5), (2, 37), (4, 128), (2, 144), (0, 20), (1, 1)):
+        A = torch.randn(B, n, n, dtype=C128, generator=g)
+        mats.append(A + A.conj().transpose(-2, -1))
+    results, info = eigh_grouped([H.cuda() for H in mats])
+    assert int(info.abs().sum().item()) == 0
+    for H, (E, V) in zip(mats, results):
+        if H.shape[0] == 0:
+            assert E.shape == (0, H.shape[-1])
+            continue
+        E, V = E.cpu(), V.cpu()
+        ref = np.linalg.eigvalsh(H.numpy())
+        scale = np.abs(ref).max()
+        assert np.abs(E.numpy() - ref).max() < 1e-12 * scale
+        assert (H @ V - V * E[:, None, :]).abs().max().item() < 1e-11 * scale
