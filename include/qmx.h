@@ -225,6 +225,19 @@ typedef struct qmx_heev_desc {
 int64_t qmx_heev_small_work_bytes(int64_t n);
 int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
 
+/* Batched overlap product over the basis axis on the matrix cores:
+ *   out[b][m][n] = (accumulate ? out : 0) + sum_g conj(A[b][m][g]) B[b][n][g]
+ * (_dot, dft/electrons/_wavefunction_dot.py:66-84: C1^H C2 per spin and k).
+ * A: batch x M x G (row stride lda, batch stride sa, elements), B likewise
+ * (N rows); out: complex128, row stride ldc, batch stride sc. QMX_C64
+ * inputs run on FP32 MFMA, accumulated in FP32 within chunks of the basis
+ * axis and in FP64 across them; QMX_C128 on FP64 MFMA. Workspace:
+ * qmx_dot_workspace_bytes(dtype, batch, M, N, G) bytes. */
+int64_t qmx_dot_workspace_bytes(int dtype, int64_t batch, int64_t M, int64_t N, int64_t G);
+int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
+            int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc,
+            int64_t sc, int accumulate, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Kernel ids reported by the profiler. */
 enum {
   QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */

@@ -69,6 +69,9 @@ SIGNATURES = {
     "qmx_heevj_batched": (i32, [vp, i64, i64, i64, i64, vp, vp, i64, i64, vp, i64, i32, vp]),
     "qmx_heev_small_workspace_bytes": (i64, [i64, i64]),
     "qmx_heev_small_batched": (i32, [vp, i64, i64, i64, i64, vp, vp, i64, i64, vp, vp, i64, vp]),
+    "qmx_dot_workspace_bytes": (i64, [i32, i64, i64, i64, i64]),
+    "qmx_dot": (i32, [i32, vp, i64, i64, vp, i64, i64, i64, i64, i64, i64, vp, i64, i64, i32,
+                      vp, i64, vp]),
     "qmx_heev_small_work_bytes": (i64, [i64]),
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_profile_enable": (i32, [i32]),

@@ -241,8 +241,8 @@ def _scale_constrain(X: torch.Tensor, scale: Optional[torch.Tensor], n32: torch.
 
 
 def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
-    """conj(A) @ B^T over the basis axis, batched over k."""
-    return torch.matmul(A.conj(), B.transpose(-2, -1))
+    """conj(A) @ B^T over the basis axis, batched over k, complex128."""
+    return gemm.dot128(A, B)
 
 
 # ---------------------------------------------------------------- buckets
@@ -372,7 +372,7 @@ class Bucket:
                 beta, D = self.beta[k0 : k0 + nk], self.D[k0 : k0 + nk]
             proj = _dot(beta, Cin)  # (nk, P, B)
             proj *= D[:, :, None].to(proj.real.dtype)
-            out.baddbmm_(proj.transpose(-2, -1), beta)
+            out.baddbmm_(proj.to(dt).transpose(-2, -1), beta)
 
     def _orthonormalizer(self, O: torch.Tensor, tol: float, relative: bool = True):
         """U (complex128) with U^H O U = 1 on the kept directions and zero
@@ -498,7 +498,7 @@ class Bucket:
         norm = self.norm
         _scale_constrain(R, torch.where(norm > 1e-30, 1.0 / norm.clamp_min(1e-30), 0.0), self.n32)
         for _ in range(2):  # R <- (1 - C C^H) R, twice
-            R.baddbmm_(_dot(C, R).transpose(-2, -1), C, alpha=-1)
+            R.baddbmm_(_dot(C, R).to(dt).transpose(-2, -1), C, alpha=-1)
         # Canonical orthonormalisation of R (drops near-dependent directions):
         lam, W = yield _dot(R, R).to(C128)
         tol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL

@@ -250,7 +250,7 @@ __device__ __forceinline__ void ip_pass(C2* buf, const C2* __restrict__ tw, int 
 // Full in-place transform of the nl lines described as for ip_pass; at
 // most MAXE complex values per thread in any pass. Ends with a barrier.
 template <typename C2, int S, int NT, int MAXE, bool LINE_FAST, bool INDIRECT>
-__device__ void fft_ip(C2* buf, const C2* tw, const Radices& rad, int nl, FastDiv fnl, int ps,
+__device__ __forceinline__ void fft_ip(C2* buf, const C2* tw, const Radices& rad, int nl, FastDiv fnl, int ps,
                        int ls, const int* lines) {
   for (int ip = 0; ip < rad.nr; ++ip) {
     switch (rad.r[ip]) {

@@ -2,39 +2,73 @@
 
 Two shapes cover the path (reference: _wavefunction_dot.py):
 - ``dot(A, B)``: (..., M, N) = conj(A)(..., M, G) @ B(..., N, G)^T — overlaps
-  C^H C, C^H HC and projections beta^H C (_dot, :66-84);
+  C^H C, C^H HC and projections beta^H C (_dot, :66-84). Long basis axis,
+  small output: ``dot128`` runs it on the library's split-K MFMA kernel
+  (qmx_dot) with an FP64 result;
 - ``rotate(M, C)``: (..., Nout, G) = M(..., Nin, Nout)^T @ C(..., Nin, G) —
-  subspace rotations C V and the nonlocal beta (D P) (_matmul, :168-185).
-Both operands keep the basis index G contiguous, so every call is a plain
-batched library GEMM on the device, in the precision of its inputs.
+  subspace rotations C V and the nonlocal beta (D P) (_matmul, :168-185), a
+  bandwidth-bound batched library GEMM in the precision of C.
+Both operands keep the basis index G contiguous.
 """
 from __future__ import annotations
 
 import torch
 
+from . import _lib, rc
 
-#: Target length of each K-chunk of the split-K overlap GEMM.
-SPLIT_K_CHUNK = 512
+_DTYPE = {torch.complex64: 0, torch.complex128: 1}
+
+
+def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
+           accumulate: bool = False) -> torch.Tensor:
+    """conj(A) @ B^T over the last axis, batched over the leading axes, as
+    complex128 (qmx_dot). A: (..., M, G), B: (..., N, G), both with unit
+    stride along G and one uniform stride per batch (any leading shape that
+    flattens to a single batch axis)."""
+    if A.dtype != B.dtype or A.dtype not in _DTYPE:
+        raise TypeError(f"dot128: complex64 or complex128 operands of one type, not {A.dtype}/{B.dtype}")
+    batch = torch.broadcast_shapes(A.shape[:-2], B.shape[:-2])
+    M, G = A.shape[-2:]
+    N = B.shape[-2]
+    if B.shape[-1] != G:
+        raise ValueError("dot128: basis lengths differ")
+    A3 = _as_batched(A.expand(batch + A.shape[-2:]))
+    B3 = _as_batched(B.expand(batch + B.shape[-2:]))
+    nb = A3.shape[0]
+    if out is None:
+        out = torch.empty(batch + (M, N), dtype=torch.complex128, device=A.device)
+        accumulate = False
+    o3 = out.reshape(nb, M, N)
+    if o3.data_ptr() != out.data_ptr() or o3.stride(-1) != 1:
+        raise ValueError("dot128: out must be a contiguous (..., M, N) complex128 tensor")
+    code = _DTYPE[A.dtype]
+    need = int(_lib.lib().qmx_dot_workspace_bytes(code, nb, M, N, G))
+    ws = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
+    _lib.call("qmx_dot", code, A3.data_ptr(), A3.stride(1), A3.stride(0), B3.data_ptr(),
+              B3.stride(1), B3.stride(0), nb, M, N, G, o3.data_ptr(), o3.stride(1), o3.stride(0),
+              int(accumulate), ws.data_ptr(), need, rc.stream_ptr())
+    return out
+
+
+def _as_batched(X: torch.Tensor) -> torch.Tensor:
+    """(..., M, G) -> (batch, M, G) view with unit G stride, copying only
+    when the leading axes do not fold into one uniformly strided axis."""
+    if X.stride(-1) != 1:
+        X = X.contiguous()
+    M, G = X.shape[-2:]
+    lead = X.shape[:-2]
+    if len(lead) == 0:
+        return X.reshape(1, M, G) if X.is_contiguous() else X.unsqueeze(0)
+    try:
+        return X.view(-1, M, G)
+    except RuntimeError:
+        return X.contiguous().view(-1, M, G)
 
 
 def dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
-    """conj(A) @ B^T with the long basis axis split into chunks that run as one
-    batched GEMM and are then summed (a plain GEMM with a (<=1024)^2 output
-    and K = 10^4..10^5 leaves most of the device idle)."""
-    G = A.shape[-1]
-    s = G // SPLIT_K_CHUNK
-    if s < 2:
-        return torch.matmul(A.conj(), B.transpose(-2, -1))
-    Gs = (G // s) * s
-    batch = torch.broadcast_shapes(A.shape[:-2], B.shape[:-2])
-    A = A.expand(batch + A.shape[-2:])
-    B = B.expand(batch + B.shape[-2:])
-    a = A[..., :Gs].unflatten(-1, (s, Gs // s)).transpose(-3, -2)  # (..., s, M, Gs/s)
-    b = B[..., :Gs].unflatten(-1, (s, Gs // s)).transpose(-3, -2)
-    out = torch.matmul(a.conj(), b.transpose(-2, -1)).sum(dim=-3)
-    if Gs < G:
-        out += torch.matmul(A[..., Gs:].conj(), B[..., Gs:].transpose(-2, -1))
-    return out
+    """conj(A) @ B^T in the precision of the operands (the reference's
+    _dot returns the wavefunction dtype), computed by dot128."""
+    return dot128(A, B).to(A.dtype)
 
 
 def rotate(M: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
