@@ -518,12 +518,11 @@ __device__ void heev_small_one(const double2* __restrict__ Ab, int n, int64_t ld
   double* Zl = reinterpret_cast<double*>(tau + n);
   double* d = Zl + (ZLDS ? n * ldz : 0);
   double* e = d + n;
-  double* lc = e + n;
-  double* ls = lc + n;
-  double* red = ls + n;
-  int* li = reinterpret_cast<int*>(red + NW);
-  int* perm = li + n;
-  int* ctl = perm + n;  // [0] log length, [1] done flag, [2] failure
+  double* lc = e + n;   // [2][n] rotation cosines, double-buffered log
+  double* ls = lc + 2 * n;  // [2][n] rotation sines
+  double* red = ls + 2 * n;
+  int* perm = reinterpret_cast<int*>(red + NW);
+  int* ctl = perm + n;  // [0..1] log lengths, [2] failure, [3..4] last-log flags, [5..6] top plane
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // Global Z: n * n doubles right after this matrix's n * n complex W.
   double* Z = ZLDS ? Zl : reinterpret_cast<double*>(W + (int64_t)n * n);
@@ -580,18 +579,17 @@ __device__ void heev_small_one(const double2* __restrict__ Ab, int n, int64_t ld
     }
     __syncthreads();
     if (tk.x == 0.0 && tk.y == 0.0) continue;
-    // y = W[k+1:, k+1:] v, one wave per row
-    for (int i = k + 1 + wave; i < n; i += NW) {
-      const double2* Wi = W + (int64_t)i * n;
+    // y = W[k+1:, k+1:] v, one thread per entry: W is Hermitian, so
+    // y_i = sum_j conj(W[j][i]) v_j reads column i (coalesced over i).
+    for (int i = k + 1 + tid; i < n; i += kSmallThreads) {
       double yr = 0, yi = 0;
-      for (int j = k + 1 + lane; j < n; j += 64) {
-        const double2 a = Wi[j], bb = v[j];
-        yr += a.x * bb.x - a.y * bb.y;
-        yi += a.x * bb.y + a.y * bb.x;
+#pragma unroll 16
+      for (int j = k + 1; j < n; ++j) {
+        const double2 a = W[(int64_t)j * n + i], bb = v[j];
+        yr += a.x * bb.x + a.y * bb.y;
+        yi += a.x * bb.y - a.y * bb.x;
       }
-      yr = wave_sum_d(yr);
-      yi = wave_sum_d(yi);
-      if (lane == 0) w[i] = double2{yr, yi};
+      w[i] = double2{yr, yi};
     }
     __syncthreads();
     double part = 0;
@@ -630,78 +628,130 @@ __device__ void heev_small_one(const double2* __restrict__ Ab, int n, int64_t ld
   if (!ZLDS) __threadfence_block();  // global Z written by other threads of the group
   __syncthreads();
 
-  // 2. Implicit-shift QL; thread 0 chases, everyone rotates its row of Z.
-  const double eps = 2.220446049250313e-16;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    for (;;) {
-      if (tid == 0) {
+  // 2. Implicit-shift QL (tql2 order: eigenvalue l = 0, 1, ... deflates
+  // when e[l] is negligible). Lane 0 of wave 0 runs the bulge chase of one
+  // QL iteration into a rotation log while waves 1-3 apply the previous log
+  // to their rows of Z (double-buffered logs, one barrier per iteration).
+  // A chase rotates the consecutive planes (ihi, ihi + 1), (ihi - 1, ihi),
+  // ...; each row applies them in blocks of 8 with the reads hoisted, the
+  // element shared by successive rotations carried in a register.
+  {
+    const double eps = 2.220446049250313e-16;
+    int l = 0, iter = 0;  // chase state, lane 0 of wave 0 only
+    // Produce the next log into buffer `bf`: ctl[bf] rotations, ctl[3 + bf]
+    // set when no rotations remain.
+    auto produce = [&](int bf) {
+      double* cl = lc + bf * n;
+      double* sl = ls + bf * n;
+      for (;;) {
+        if (l >= n) {
+          ctl[bf] = 0;
+          ctl[3 + bf] = 1;
+          return;
+        }
         int m = l;
         while (m < n - 1) {
           const double dd = fabs(d[m]) + fabs(d[m + 1]);
           if (fabs(e[m]) <= eps * dd) break;
           ++m;
         }
+        if (m == l) {
+          ++l;
+          iter = 0;
+          continue;
+        }
+        if (++iter > max_iter) {
+          ctl[2] = 1;
+          ++l;
+          iter = 0;
+          continue;
+        }
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = sqrt(g * g + 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+        double s = 1.0, c = 1.0, p = 0.0;
         int nlog = 0;
-        if (m == l || ++iter > max_iter) {
-          if (m != l) ctl[2] = 1;
-          ctl[1] = 1;
-        } else {
-          ctl[1] = 0;
-          double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-          double r = hypot(g, 1.0);
-          g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
-          double s = 1.0, c = 1.0, p = 0.0;
-          bool underflow = false;
-          for (int i = m - 1; i >= l; --i) {
-            const double f = s * e[i], bb = c * e[i];
-            r = hypot(f, g);
-            e[i + 1] = r;
-            if (r == 0.0) {
-              d[i + 1] -= p;
-              e[m] = 0.0;
-              underflow = true;
-              break;
-            }
-            s = f / r;
-            c = g / r;
-            g = d[i + 1] - p;
-            r = (d[i] - g) * s + 2.0 * c * bb;
-            p = s * r;
-            d[i + 1] = g + p;
-            g = c * r - bb;
-            li[nlog] = i;
-            lc[nlog] = c;
-            ls[nlog] = s;
-            ++nlog;
-          }
-          if (!underflow) {
-            d[l] -= p;
-            e[l] = g;
+        bool underflow = false;
+        // Step i reads d[i], e[i] and d[i + 1] before any step writes them
+        // (step i writes d[i + 1] and e[i + 1] only, after step i + 1 read
+        // d[i + 1] as its d[i]), so they stream through registers.
+        double di = d[m - 1], ei = e[m - 1], dip1 = d[m];
+        for (int i = m - 1; i >= l; --i) {
+          const double dn = i > l ? d[i - 1] : 0.0, en = i > l ? e[i - 1] : 0.0;
+          const double f = s * ei, bb = c * ei;
+          const double x = f * f + g * g;
+          if (x == 0.0) {
+            e[i + 1] = 0.0;
+            d[i + 1] = dip1 - p;
             e[m] = 0.0;
+            underflow = true;
+            break;
           }
+          // 1/sqrt(x): hardware estimate and two Newton steps (full FP64)
+          double ri = __builtin_amdgcn_rsq(x);
+          ri = ri * (1.5 - 0.5 * x * ri * ri);
+          ri = ri * (1.5 - 0.5 * x * ri * ri);
+          r = x * ri;
+          e[i + 1] = r;
+          s = f * ri;
+          c = g * ri;
+          g = dip1 - p;
+          r = (di - g) * s + 2.0 * c * bb;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - bb;
+          cl[nlog] = c;
+          sl[nlog] = s;
+          ++nlog;
+          dip1 = di;
+          di = dn;
+          ei = en;
         }
-        ctl[0] = nlog;
+        if (!underflow) {
+          d[l] -= p;
+          e[l] = g;
+          e[m] = 0.0;
+        }
+        ctl[bf] = nlog;
+        ctl[3 + bf] = 0;
+        ctl[5 + bf] = m - 1;
+        return;
+      }
+    };
+    if (tid == 0) produce(0);
+    __syncthreads();
+    for (int step = 0;; ++step) {
+      const int cur = step & 1;
+      const int nlog = ctl[cur];
+      if (ctl[3 + cur]) break;  // no log left (every row is up to date)
+      if (wave == 0) {
+        if (lane == 0) produce(cur ^ 1);
+      } else if (nlog > 0) {
+        const double* cl = lc + cur * n;
+        const double* sl = ls + cur * n;
+        const int ihi = ctl[5 + cur];
+        for (int row = tid - 64; row < n; row += kSmallThreads - 64) {
+          double* zr = Z + (int64_t)row * ldz;
+          double carry = zr[ihi + 1];
+          for (int t0 = 0; t0 < nlog; t0 += 8) {
+            const int cnt = min(8, nlog - t0);
+            double zz[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              if (q < cnt) zz[q] = zr[ihi - t0 - q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              if (q < cnt) {
+                const double c = cl[t0 + q], s = sl[t0 + q];
+                zr[ihi - t0 - q + 1] = s * zz[q] + c * carry;
+                carry = c * zz[q] - s * carry;
+              }
+            }
+          }
+          zr[ihi - nlog + 1] = carry;
+        }
       }
       __syncthreads();
-      const int nlog = ctl[0];
-      const int done = ctl[1];
-      if (tid < n && nlog > 0) {
-        // Rotation t acts on the planes (i, i + 1), i = li[0] - t.
-        double* zr = Z + (int64_t)tid * ldz;
-        const int ihi = li[0];
-        double carry = zr[ihi + 1];
-        for (int t = 0; t < nlog; ++t) {
-          const int i = ihi - t;
-          const double c = lc[t], s = ls[t];
-          const double zi = zr[i];
-          zr[i + 1] = s * zi + c * carry;
-          carry = c * zi - s * carry;
-        }
-        zr[ihi - nlog + 1] = carry;
-      }
-      __syncthreads();
-      if (done) break;
     }
   }
 
@@ -769,8 +819,8 @@ __global__ void __launch_bounds__(kSmallThreads)
 static size_t small_lds_bytes(int64_t n) {
   const bool zlds = n <= kSmallLdsN;
   return (size_t)3 * n * sizeof(double2) + (zlds ? (size_t)n * (n + 1) * sizeof(double) : 0) +
-         (size_t)4 * n * sizeof(double) + 4 * sizeof(double) + (size_t)2 * n * sizeof(int) +
-         4 * sizeof(int);
+         (size_t)6 * n * sizeof(double) + 4 * sizeof(double) + (size_t)n * sizeof(int) +
+         8 * sizeof(int);
 }
 
 }  // namespace
