@@ -245,6 +245,17 @@ def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     return gemm.dot128(A, B)
 
 
+def _lowdin(O: torch.Tensor) -> torch.Tensor:
+    """O^(-1/2) for Hermitian overlaps within ~1e-3 of the identity:
+    I - E/2 + 3 E^2/8 with E = O - I (error O(|E|^3)); no factorisation,
+    no host synchronisation."""
+    E = 0.5 * (O + dagger(O))
+    E.diagonal(dim1=-2, dim2=-1).sub_(1.0)
+    S = (0.375 * E) @ E - 0.5 * E
+    S.diagonal(dim1=-2, dim2=-1).add_(1.0)
+    return S
+
+
 # ---------------------------------------------------------------- buckets
 class Bucket:
     """Jobs sharing an FFT box and a band class, iterated in lockstep."""
@@ -374,31 +385,14 @@ class Bucket:
             proj *= D[:, :, None].to(proj.real.dtype)
             out.baddbmm_(proj.to(dt).transpose(-2, -1), beta)
 
-    def _orthonormalizer(self, O: torch.Tensor, tol: float, relative: bool = True):
-        """U (complex128) with U^H O U = 1 on the kept directions and zero
-        columns elsewhere (canonical orthogonalisation), and the kept mask:
-        overlap eigenvalues above tol (times the largest if `relative`)."""
-        lam, W, info = eigh_nosync(O)
-        if info is not None:
-            self._info.append(info)
-        keep = lam > (tol * lam[..., -1:].clamp_min(1e-300) if relative else tol)
-        s = torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))
-        return W * s[..., None, :].to(W.dtype), keep
-
-    def _subspace_eigh(self, H: torch.Tensor):
-        E, V, info = eigh_nosync(H)
-        if info is not None:
-            self._info.append(info)
-        return E, V
-
     # ------------------------------------------------------------ job start
-    def _initialise(self, new: list[CellJob]) -> None:
+    def init_steps(self, new: list[CellJob]):
         """Start of the new jobs, whose rows close the stack: random bands in
         FP64 (_electrons.py:271-287; bandwidth-limited random bands have a
-        condition number beyond float32), orthonormalised, then rotated to
-        the Ritz vectors of their span (_davidson.py:164-169)."""
-        if not new:
-            return
+        condition number beyond float32), orthonormalised (canonical, on the
+        FP64 eigenvectors of their overlap), then rotated to the Ritz vectors
+        of their span (_davidson.py:164-169). A generator that yields its two
+        Hermitian eigenproblems (see expand_steps)."""
         nb = self.nb
         a0 = self.k0[len(self.jobs) - len(new)]
         Kn = self.Kb - a0
@@ -410,11 +404,15 @@ class Bucket:
                       self.Gb, j.n_tot, 0, _c(self.n32[a:]), _c(self.ke[a:]), self.ke.stride(0),
                       j.seed, 0, nb, rc.stream_ptr())
             j.admitted = time.perf_counter()
-        U, _ = self._orthonormalizer(_dot(C, C), 1e-14)
+        lam, W = yield _dot(C, C)
+        keep = lam > 1e-14 * lam[..., -1:].clamp_min(1e-300)
+        U = W * torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))[..., None, :]
         C = gemm.rotate(U, C).to(self.dtype)
         HC = self.HX[a0:, :nb]
         self.hamiltonian(C, HC, a0, Kn)
-        E, Vr = self._subspace_eigh(_dot(C, HC).to(C128))
+        Hs = _dot(C, HC)
+        Hs.diagonal(dim1=-2, dim2=-1).add_(torch.where(keep, 0.0, self.DROP_SHIFT))
+        E, Vr = yield 0.5 * (Hs + dagger(Hs))
         Vr = Vr.to(self.dtype)
         torch.matmul(Vr.transpose(-2, -1), C, out=self.X[a0:, :nb])
         self.HX[a0:, :nb] = torch.matmul(Vr.transpose(-2, -1), HC)
@@ -469,21 +467,6 @@ class Bucket:
                 done.append(j)
         return done
 
-    def _chol_reduce(self, H: torch.Tensor, O: torch.Tensor):
-        """Li and Li H Li^H for H V = O V E with O close to the identity
-        (Cholesky without a host check); then V = Li^H W."""
-        L = torch.linalg.cholesky_ex(O)[0]
-        eye = torch.eye(O.shape[-1], dtype=O.dtype, device=O.device).expand_as(L)
-        Li = torch.linalg.solve_triangular(L, eye, upper=False)
-        Hh = Li @ H @ dagger(Li)
-        return Li, 0.5 * (Hh + dagger(Hh))
-
-    def _eighg_chol(self, H: torch.Tensor, O: torch.Tensor):
-        """FP64 generalized eigenproblem H V = O V E, O close to the identity."""
-        Li, Hh = self._chol_reduce(H, O)
-        E, W = self._subspace_eigh(Hh)
-        return E, dagger(Li) @ W
-
     def expand_steps(self):
         """One Davidson step on every row (_davidson.py:180-233, no locking),
         as a generator that yields its two FP64 Hermitian eigenproblems
@@ -522,9 +505,10 @@ class Bucket:
         drop = ~keep
         O.diagonal(dim1=-2, dim2=-1)[:, nb:] += drop.to(torch.float64)
         H.diagonal(dim1=-2, dim2=-1)[:, nb:] += torch.where(drop, self.DROP_SHIFT, 0.0)
-        Li, Hh = self._chol_reduce(H, O)
-        E, W2 = yield Hh
-        Vt = (dagger(Li) @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
+        S = _lowdin(O)
+        Hh = S @ H @ S
+        E, W2 = yield 0.5 * (Hh + dagger(Hh))
+        Vt = (S @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
         torch.matmul(Vt, self.X, out=self.X2[:, :nb])
         torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
@@ -536,34 +520,27 @@ class Bucket:
             j.iterations += 1
         self.n_ticks += 1
 
-    def expand(self) -> None:
-        """expand_steps driven with this bucket's own eigensolves."""
-        steps = self.expand_steps()
-        mat = next(steps)
-        try:
-            while True:
-                mat = steps.send(self._subspace_eigh(mat))
-        except StopIteration:
-            pass
-
     # ------------------------------------------------------------ results
-    def finish(self, job: CellJob) -> "_Pending":
-        """Device results of a converged (or exhausted) job; the mixed form
-        first recomputes its subspace in FP64 (complex128 H C, FP64 overlap
-        and eigenproblem, the final Rayleigh quotients). The packed result
-        ends with the count of unconverged QL eigensolves."""
+    def finish_steps(self, job: CellJob):
+        """Device results of a converged (or exhausted) job, as a generator
+        that yields one Hermitian eigenproblem and returns a _Pending. The
+        mixed form recomputes the job's subspace in FP64 first (complex128
+        H C, FP64 overlap and eigenproblem, the final Rayleigh quotients).
+        The packed result ends with the count of unconverged QL eigensolves
+        (filled in by the solver)."""
         a = self.k0[self.jobs.index(job)]
         rows = slice(a, a + job.K)
         nb, B = self.nb, job.n_bands
-        n_info = len(self._info)
         if self.mixed:
             C = self.X[rows, :nb].to(C128)
             HC = torch.empty_like(C)
             beta = job.projectors(self.Gb) if job.P else None
             D = job.device_tables()["D"][None, :].expand(job.K, -1) if job.P else None
             self.hamiltonian(C, HC, a, job.K, beta, D)
-            E, Vs = self._eighg_chol(_dot(C, HC), _dot(C, C))
-            Vs = Vs[:, :, :B]
+            S = _lowdin(_dot(C, C))  # C is orthonormal to float32 rounding
+            Hh = S @ _dot(C, HC) @ S
+            E, W = yield 0.5 * (Hh + dagger(Hh))
+            Vs = S @ W[:, :, :B]
             C = gemm.rotate(Vs, C)
             HC = gemm.rotate(Vs, HC)
             eig = E[:, :B]
@@ -571,13 +548,10 @@ class Bucket:
         else:
             eig = self.eig[rows, :B].clone()
             rnorm = self.rnorm[rows, :B].clone()
-        fail = torch.zeros(1, dtype=torch.float64, device=rc.device)
-        for info in self._info[n_info:]:
-            fail += (info != 0).sum()
-        del self._info[n_info:]
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
-        return _Pending(job, torch.cat([eig.flatten(), rnorm.flatten(), fail]), ev)
+        return _Pending(job, torch.cat([eig.flatten(), rnorm.flatten(),
+                                        torch.zeros(1, dtype=torch.float64, device=rc.device)]), ev)
 
 
 @dataclass
@@ -605,7 +579,7 @@ class BatchSolver:
 
     def __init__(self, dtype: torch.dtype = C64, *, residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
-                 band_classes: Optional[Sequence[int]] = None, hbm_budget: float = 200e9,
+                 band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
                  max_k_per_bucket: int = 2048):
         rc.require_gpu()
         self.dtype = dtype
@@ -634,11 +608,17 @@ class BatchSolver:
         return time.perf_counter() - self.t0
 
     def band_class(self, job: CellJob) -> int:
+        """Smallest configured class that holds the job's n_bands +
+        n_bands_extra and keeps the expanded subspace (twice the class)
+        below its smallest basis (_davidson.py:159-163); else its own size
+        rounded up to a multiple of 8 within that limit."""
+        limit = (int(job.n.min()) - 1) // 2
         if self.band_classes:
             for c in self.band_classes:
-                if c >= job.n_bands_max:
+                if job.n_bands_max <= c <= limit:
                     return c
-        return int(np.ceil(job.n_bands_max / 8) * 8)
+        c = int(np.ceil(job.n_bands_max / 8) * 8)
+        return c if c <= limit else job.n_bands_max
 
     def submit(self, jobs: Sequence[CellJob]) -> None:
         self.queue.extend(jobs)
@@ -713,18 +693,19 @@ class BatchSolver:
         self._collect(prev, [h.numpy() for h in host_prev])
         new_results = [self.results[p.job.job_id] for p in prev]
         host = [h.numpy() for h in host_packs]
-        # 2. retire converged jobs (their FP64 results are read next tick).
-        retire = {}
+        # 2. retire converged jobs (their FP64 results are read next tick);
+        #    each finish runs up to its eigenproblem before the restack.
+        retire, gens, mats = {}, [], []
         for b, h in zip(active, host):
             done = b.converged_jobs(h)
             retire[id(b)] = done
             for job in done:
-                self.pending.append(b.finish(job))
-            for job in done:
+                self._start(b.finish_steps(job), gens, mats)
                 self.hbm_in_use -= self._held.pop(job.job_id, 0)
-        # 3. admission, restack, start of new jobs, one Davidson step.
+        # 3. admission, restack, start of new jobs.
         newcomers = self._admit(now)
         keys = sorted(set(newcomers) | {(b.shape, b.nb) for b in active})
+        changed = []
         for key in keys:
             b = self._bucket(key)
             new = newcomers.get(key, [])
@@ -735,24 +716,50 @@ class BatchSolver:
                     job.release()
                 if not b.jobs:
                     continue
-                b._initialise(new)
-                b.residuals()
-        # One Davidson step of every bucket; their eigenproblems run grouped.
-        steps = [b.expand_steps() for b in self.buckets.values() if b.jobs]
-        mats = [next(g) for g in steps]
-        while steps:
-            results, info = eigh_grouped(mats)
-            self._info.append(info)
-            nxt_steps, nxt_mats = [], []
-            for g, r in zip(steps, results):
-                try:
-                    nxt_mats.append(g.send(r))
-                    nxt_steps.append(g)
-                except StopIteration:
-                    pass
-            steps, mats = nxt_steps, nxt_mats
+                changed.append(b)
+                if new:
+                    self._start(b.init_steps(new), gens, mats)
+        self.pending.extend(self._drive(gens, mats))
+        for b in changed:
+            b.residuals()
+        # 4. one Davidson step of every bucket.
+        steps, mats = [], []
+        for b in self.buckets.values():
+            if b.jobs:
+                self._start(b.expand_steps(), steps, mats)
+        self._drive(steps, mats)
         self.n_ticks += 1
         return new_results
+
+    def _start(self, gen, gens: list, mats: list) -> None:
+        """Run `gen` to its first eigenproblem (queued in gens/mats); a
+        generator that returns before yielding delivers its value to the
+        pending results."""
+        try:
+            mats.append(next(gen))
+            gens.append(gen)
+        except StopIteration as stop:
+            if stop.value is not None:
+                self.pending.append(stop.value)
+
+    def _drive(self, gens: list, mats: list) -> list:
+        """Run generators that yield Hermitian eigenproblems to completion,
+        all pending eigenproblems of a round in one grouped launch; returns
+        the generators' return values (those that are not None)."""
+        out = []
+        while gens:
+            results, info = eigh_grouped(mats)
+            self._info.append(info)
+            nxt_gens, nxt_mats = [], []
+            for g, r in zip(gens, results):
+                try:
+                    nxt_mats.append(g.send(r))
+                    nxt_gens.append(g)
+                except StopIteration as stop:
+                    if stop.value is not None:
+                        out.append(stop.value)
+            gens, mats = nxt_gens, nxt_mats
+        return out
 
     @property
     def idle(self) -> bool:
