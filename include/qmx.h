@@ -220,10 +220,25 @@ typedef struct qmx_heev_desc {
   int64_t lda;
   int64_t ldv;
   int32_t n;
-  int32_t reserved;
+  int32_t n_want; /* eigenvectors wanted (qmx_heevx_grouped; 0 = all n) */
 } qmx_heev_desc;
 int64_t qmx_heev_small_work_bytes(int64_t n);
 int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
+
+/* The same grouped eigenproblems (n <= 256, descriptors as above) with a
+ * parallel tridiagonal stage: Householder reduction, all n eigenvalues by
+ * bisection (one lane each, division-free Sturm counts), the eigenvectors
+ * of the n_want lowest by inverse iteration (one lane each, re-orthogonalised
+ * inside clusters of close eigenvalues) and the back-transformation. Replaces
+ * the serial QL iteration of qmx_heev_small_grouped for eigh inside eighg
+ * (math/_linalg.py:181) and the Davidson subspace solves
+ * (_davidson.py:165,199-206). E gets all n eigenvalues ascending, V the
+ * eigenvectors of the n_want lowest in columns 0..n_want-1 (the other columns
+ * are not written). Work area per matrix: qmx_heevx_work_bytes(n, n_want)
+ * bytes (256-byte aligned). *info is set to 0, or 1 for a non-finite input
+ * (E and V are then NaN). */
+int64_t qmx_heevx_work_bytes(int64_t n, int64_t n_want);
+int qmx_heevx_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
 
 /* Batched overlap product over the basis axis on the matrix cores:
  *   out[b][m][n] = (accumulate ? out : 0) + sum_g conj(A[b][m][g]) B[b][n][g]

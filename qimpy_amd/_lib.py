@@ -74,6 +74,8 @@ SIGNATURES = {
                       vp, i64, vp]),
     "qmx_heev_small_work_bytes": (i64, [i64]),
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
+    "qmx_heevx_work_bytes": (i64, [i64, i64]),
+    "qmx_heevx_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_profile_enable": (i32, [i32]),
     "qmx_profile_read": (i32, [vp, vp, vp, i32]),
     "qmx_last_error": (ct.c_char_p, []),

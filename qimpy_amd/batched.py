@@ -507,7 +507,7 @@ class Bucket:
         H.diagonal(dim1=-2, dim2=-1)[:, nb:] += torch.where(drop, self.DROP_SHIFT, 0.0)
         S = _lowdin(O)
         Hh = S @ H @ S
-        E, W2 = yield 0.5 * (Hh + dagger(Hh))
+        E, W2 = yield 0.5 * (Hh + dagger(Hh)), nb  # only the nb lowest eigenvectors
         Vt = (S @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
         torch.matmul(Vt, self.X, out=self.X2[:, :nb])
         torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
@@ -539,7 +539,7 @@ class Bucket:
             self.hamiltonian(C, HC, a, job.K, beta, D)
             S = _lowdin(_dot(C, C))  # C is orthonormal to float32 rounding
             Hh = S @ _dot(C, HC) @ S
-            E, W = yield 0.5 * (Hh + dagger(Hh))
+            E, W = yield 0.5 * (Hh + dagger(Hh)), B
             Vs = S @ W[:, :, :B]
             C = gemm.rotate(Vs, C)
             HC = gemm.rotate(Vs, HC)
@@ -745,10 +745,14 @@ class BatchSolver:
     def _drive(self, gens: list, mats: list) -> list:
         """Run generators that yield Hermitian eigenproblems to completion,
         all pending eigenproblems of a round in one grouped launch; returns
-        the generators' return values (those that are not None)."""
+        the generators' return values (those that are not None). A generator
+        yields a matrix batch, or (matrix batch, n_want) when it needs only
+        the eigenvectors of the n_want lowest eigenvalues."""
         out = []
         while gens:
-            results, info = eigh_grouped(mats)
+            want = [m[1] if isinstance(m, tuple) else None for m in mats]
+            mats = [m[0] if isinstance(m, tuple) else m for m in mats]
+            results, info = eigh_grouped(mats, want)
             self._info.append(info)
             nxt_gens, nxt_mats = [], []
             for g, r in zip(gens, results):

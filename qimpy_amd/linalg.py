@@ -8,7 +8,7 @@ these small dense problems always run in complex128 on the device.
 from __future__ import annotations
 
 import ctypes as ct
-from typing import Protocol, TypeVar, Generic
+from typing import Generic, Optional, Protocol, Sequence, TypeVar
 
 import numpy as np
 import torch
@@ -84,21 +84,32 @@ def _cost_ql(n: int) -> float:
     return 1.1 * (n / 36.0) ** 1.95 if n <= SMALL_QL_MAX_N else float("inf")
 
 
+def _cost_bisect(n: int) -> float:
+    return 0.5 * (n / 36.0) ** 1.6 if n <= SMALL_QL_MAX_N else float("inf")
+
+
 def _cost_library(n: int, n_mat: int) -> float:
     return n_mat * (0.35 + 0.02 * n)
 
 
 def small_method(n: int, n_mat: int) -> str:
-    """'jacobi', 'ql' or 'library' for a batch of n_mat matrices of size n."""
-    costs = {"jacobi": _cost_jacobi(n), "ql": _cost_ql(n), "library": _cost_library(n, n_mat)}
+    """'bisect', 'jacobi', 'ql' or 'library' for a batch of n_mat matrices of
+    size n."""
+    costs = {"bisect": _cost_bisect(n), "jacobi": _cost_jacobi(n), "ql": _cost_ql(n),
+             "library": _cost_library(n, n_mat)}
     return min(costs, key=costs.get)
 
 
 def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch.Tensor]:
     """Batched on-chip eigensolve of (..., n, n) complex128 on the GPU:
-    method "ql" (Householder + QL, n <= 256) or "jacobi" (n <= 96)."""
+    method "bisect" (Householder + bisection + inverse iteration, n <= 256),
+    "ql" (Householder + QL, n <= 256) or "jacobi" (n <= 96)."""
     n = H.shape[-1]
     batch = H.shape[:-2]
+    if method == "bisect":
+        (res,), info = eigh_grouped([H])
+        eigh_small.last_info = info
+        return res
     Hf = _fp64(H).reshape((-1, n, n)).contiguous()
     B = Hf.shape[0]
     E = torch.empty((B, n), dtype=torch.float64, device=H.device)
@@ -121,17 +132,26 @@ def eigh_small(H: torch.Tensor, method: str = "ql") -> tuple[torch.Tensor, torch
 
 _HEEV_DESC = np.dtype([("A", np.uint64), ("E", np.uint64), ("V", np.uint64), ("info", np.uint64),
                        ("work", np.uint64), ("lda", np.int64), ("ldv", np.int64), ("n", np.int32),
-                       ("reserved", np.int32)])
+                       ("n_want", np.int32)])
 
 
-def eigh_grouped(mats: list) -> tuple[list, torch.Tensor]:
+def eigh_grouped(mats: list, n_want: Optional[Sequence[int]] = None,
+                 method: str = "bisect") -> tuple[list, torch.Tensor]:
     """Eigendecompositions of several batches of Hermitian complex128
-    matrices (each (..., n_i, n_i), n_i <= SMALL_QL_MAX_N, on the GPU) in one
-    launch per accumulator placement (n <= 128 in LDS, above in the
-    workspace), without a host synchronisation. Returns [(E, V)] in the
-    input order and an int32 tensor, nonzero where QL did not converge."""
+    matrices (each (..., n_i, n_i), n_i <= SMALL_QL_MAX_N, on the GPU)
+    without a host synchronisation. Returns [(E, V)] in the input order and
+    an int32 tensor per matrix, nonzero where the eigensolver failed.
+
+    method "bisect" (qmx_heevx_grouped): one launch for all matrices;
+    n_want[i] (default all) eigenvectors of the lowest eigenvalues of batch
+    i are computed, in the first n_want[i] columns of V (the others are left
+    unwritten); E holds every eigenvalue. method "ql"
+    (qmx_heev_small_grouped): all eigenvectors, one launch per accumulator
+    placement (n <= 128 in LDS, above in the workspace)."""
     dev = rc.device
     flat = [_fp64(H).reshape((-1,) + tuple(H.shape[-2:])).contiguous() for H in mats]
+    want = [int(H.shape[-1]) for H in flat] if n_want is None else [
+        int(w) if w and 0 < int(w) <= H.shape[-1] else int(H.shape[-1]) for w, H in zip(n_want, flat)]
     outs = []
     for H in flat:
         B, n = H.shape[0], H.shape[-1]
@@ -143,23 +163,26 @@ def eigh_grouped(mats: list) -> tuple[list, torch.Tensor]:
     info = torch.zeros(max(count, 1), dtype=torch.int32, device=dev)
     if count == 0:
         return [(E.reshape(H.shape[:-1]), V.reshape(H.shape)) for H, (E, V) in zip(mats, outs)], info
+    bisect = method == "bisect"
     work_of = {}
     total = 0
-    for H in flat:
+    for H, w in zip(flat, want):
         n = H.shape[-1]
-        if n not in work_of:
-            work_of[n] = int(_lib.lib().qmx_heev_small_work_bytes(n))
-        total += work_of[n] * H.shape[0]
+        if (n, w) not in work_of:
+            work_of[(n, w)] = int(_lib.lib().qmx_heevx_work_bytes(n, w) if bisect
+                                  else _lib.lib().qmx_heev_small_work_bytes(n))
+        total += work_of[(n, w)] * H.shape[0]
     ws = torch.empty(total, dtype=torch.uint8, device=dev)
     groups = {True: [], False: []}
     off, i_info = 0, 0
-    for H, (E, V) in zip(flat, outs):
+    for H, (E, V), w in zip(flat, outs, want):
         B, n = H.shape[0], H.shape[-1]
         for b in range(B):
-            groups[n <= 128].append((H.data_ptr() + b * n * n * 16, E.data_ptr() + b * n * 8,
-                                     V.data_ptr() + b * n * n * 16, info.data_ptr() + 4 * i_info,
-                                     ws.data_ptr() + off, n, n, n, 0))
-            off += work_of[n]
+            groups[bisect or n <= 128].append((H.data_ptr() + b * n * n * 16, E.data_ptr() + b * n * 8,
+                                               V.data_ptr() + b * n * n * 16,
+                                               info.data_ptr() + 4 * i_info, ws.data_ptr() + off,
+                                               n, n, n, w))
+            off += work_of[(n, w)]
             i_info += 1
     keep = []
     for small, rows in groups.items():
@@ -169,7 +192,8 @@ def eigh_grouped(mats: list) -> tuple[list, torch.Tensor]:
         d = torch.from_numpy(arr.view(np.uint8)).pin_memory().to(dev, non_blocking=True)
         keep.append(d)
         n_max = int(arr["n"].max())
-        _lib.call("qmx_heev_small_grouped", d.data_ptr(), len(rows), n_max, rc.stream_ptr())
+        _lib.call("qmx_heevx_grouped" if bisect else "qmx_heev_small_grouped", d.data_ptr(),
+                  len(rows), n_max, rc.stream_ptr())
     return [(E.reshape(H.shape[:-1]), V.reshape(H.shape)) for H, (E, V) in zip(mats, outs)], info
 
 
@@ -186,7 +210,7 @@ def eigh_nosync(H: torch.Tensor):
         method = small_method(n, n_mat)
         if method != "library":
             E, V = eigh_small(H, method)
-            return E, V, (eigh_small.last_info if method == "ql" else None)
+            return E, V, (eigh_small.last_info if method in ("ql", "bisect") else None)
     E, V = eigh(H)
     return E, V, None
 
@@ -212,8 +236,8 @@ def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         method = small_method(n, n_mat)
         if method != "library":
             E, V = eigh_small(H, method)
-            if method == "ql" and bool((eigh_small.last_info != 0).any()):
-                return torch.linalg.eigh(H)  # QL did not converge on some matrix
+            if method in ("ql", "bisect") and bool((eigh_small.last_info != 0).any()):
+                return torch.linalg.eigh(H)  # QL did not converge / non-finite input
             return E, V
     if not (HETRD_MIN_N <= n <= HETRD_MAX_N):
         return torch.linalg.eigh(H)

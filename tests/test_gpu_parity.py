@@ -343,6 +343,67 @@ def test_eigh_grouped_mixed_sizes(gpu):
         assert (H @ V - V * E[:, None, :]).abs().max().item() < 1e-11 * scale
 
 
+def _unitary(n, rng):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return Q
+
+
+def _bisect_cases():
+    """(name, H (1, n, n) complex128, n_want) for the bisection eigensolver:
+    random spectra, exactly degenerate clusters (multiplicity 2, 3, 8, 16),
+    decoupled rows shifted to 1e6 (the dropped directions of the batched
+    Davidson step), the zero matrix, the identity and a diagonal matrix."""
+    rng = np.random.default_rng(11)
+    out = []
+    for n, w in ((1, 1), (2, 2), (3, 1), (36, 36), (64, 32), (144, 72), (200, 200), (256, 40)):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        out.append((f"random{n}", A + A.conj().T, w))
+    for n, mult in ((12, 3), (72, 8), (144, 2), (144, 3), (144, 16)):
+        ev = np.repeat(rng.standard_normal((n + mult - 1) // mult), mult)[:n]
+        Q = _unitary(n, rng)
+        out.append((f"degenerate{n}x{mult}", (Q * ev) @ Q.conj().T, n // 2))
+    A = rng.standard_normal((144, 144)) + 1j * rng.standard_normal((144, 144))
+    H = A + A.conj().T
+    H[134:, :] = 0
+    H[:, 134:] = 0
+    H[np.arange(134, 144), np.arange(134, 144)] = 1e6
+    out.append(("shifted", H, 72))
+    out.append(("zero", np.zeros((10, 10), complex), 10))
+    out.append(("identity", np.eye(20, dtype=complex), 20))
+    out.append(("diagonal", np.diag(np.arange(30.0)).astype(complex), 15))
+    return [(name, torch.tensor(H)[None], w) for name, H, w in out]
+
+
+@pytest.mark.parametrize("case", _bisect_cases(), ids=lambda c: c[0])
+def test_eigh_bisect_cases(gpu, case):
+    """Bisection + inverse-iteration eigensolver (qmx_heevx_grouped) against
+    LAPACK: all eigenvalues to 1e-13 of the spectral radius; the n_want
+    lowest eigenvectors with residual |HV - VE| < 1e-12 |H| and
+    orthonormality to 1e-11 (inverse iteration leaves about eps |H| / gap
+    between neighbours outside a cluster, removed inside clusters)."""
+    from qimpy_amd.linalg import eigh_grouped
+    name, H, w = case
+    (res,), info = eigh_grouped([H.cuda()], [w])
+    E, V = res
+    assert int(info.abs().sum().item()) == 0
+    E, V = E.cpu()[0], V.cpu()[0, :, :w]
+    ref = np.linalg.eigvalsh(H[0].numpy())
+    scale = max(np.abs(ref).max(), 1e-300)
+    assert np.abs(E.numpy() - ref).max() <= 1e-13 * scale
+    assert (H[0] @ V - V * E[None, :w]).abs().max().item() <= 1e-12 * scale
+    assert (V.conj().T @ V - torch.eye(w, dtype=C128)).abs().max().item() < 1e-11
+
+
+def test_eigh_bisect_nonfinite(gpu):
+    """A NaN in the input gives info = 1 and NaN eigenvalues, no hang."""
+    from qimpy_amd.linalg import eigh_grouped
+    H = torch.eye(8, dtype=C128)[None].clone()
+    H[0, 3, 2] = float("nan")
+    (res,), info = eigh_grouped([H.cuda()])
+    assert int(info[0].item()) == 1
+    assert torch.isnan(res[0]).all()
+
+
 @pytest.mark.parametrize("dtype", [C64, C128])
 @pytest.mark.parametrize("shape", [(1, 1, 1, 1), (3, 5, 7, 13), (2, 70, 33, 1000), (4, 144, 72, 4017),
                                    (1, 130, 65, 40000)])

@@ -98,6 +98,22 @@ def test_geom_errors():
         _lib.geom_tables((4, 4, 4), np.array([5]), np.array([[0, 1, 2]], dtype=np.int64))
 
 
+@pytest.mark.parametrize("n,n_want", [(1, 1), (2, 1), (37, 37), (144, 72), (144, 144), (256, 3)])
+def test_heevx_work_contract(n, n_want):
+    """Work area of the bisection eigensolver per matrix: the complex working
+    copy (16 n^2), four n x n_want real inverse-iteration arrays and d, e, tau
+    (32 n), 256-byte aligned; n_want <= 0 or > n means all n; n outside
+    1..256 is refused (-1)."""
+    lib = _lib.lib()
+    w = int(lib.qmx_heevx_work_bytes(n, n_want))
+    assert w >= 16 * n * n + 32 * n * n_want + 32 * n
+    assert w % 256 == 0
+    assert int(lib.qmx_heevx_work_bytes(n, 0)) == int(lib.qmx_heevx_work_bytes(n, n))
+    assert int(lib.qmx_heevx_work_bytes(n, n + 5)) == int(lib.qmx_heevx_work_bytes(n, n))
+    assert int(lib.qmx_heevx_work_bytes(0, 1)) == -1
+    assert int(lib.qmx_heevx_work_bytes(257, 1)) == -1
+
+
 @pytest.mark.parametrize("n", [1, 2, 37, 127, 128, 129, 143, 144, 200, 256])
 def test_heev_small_workspace_contract(n):
     """Workspace of the batched QL eigensolver: per matrix at least the complex
