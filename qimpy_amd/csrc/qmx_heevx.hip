@@ -53,6 +53,7 @@ constexpr int kXMaxN = 256;
 constexpr int kXIters = 3;                // inverse-iteration solves per eigenvector
 constexpr int kBtCols = 4;                // back-transformation columns per wave pass
 constexpr int kBtRows = kXMaxN / 64;      // rows per lane in the back-transformation
+constexpr int kBtR = 36;                  // rows per wave, row-block back-transformation
 constexpr int kStage = 8;                 // steps per register-staged block in phase 3
 constexpr double kEps = 2.220446049250313e-16;
 constexpr double kClusterRtol = 1e-6;  // final MGS inside clusters (T scaled to norm ~1)
@@ -60,9 +61,10 @@ constexpr double kTightTol = 1e-12;    // numerically degenerate eigenvalues (sc
 
 // Work area of one matrix: W (n x n complex: the working copy, then the
 // reflectors in its rows), X, U1, U2, U3 (n x n_want real each, inverse
-// iteration), d, e (n real) and tau (n complex) of the tridiagonal form.
+// iteration), d, e (n real) and tau (n complex) of the tridiagonal form, and
+// a status word (nonzero: non-finite input).
 __host__ __device__ inline int64_t x_ws_stride(int64_t n, int64_t n_want) {
-  const int64_t bytes = n * n * 16 + 4 * n * n_want * 8 + n * 32;
+  const int64_t bytes = n * n * 16 + 4 * n * n_want * 8 + n * 32 + 16;
   return (bytes + 255) / 256 * 256;
 }
 
@@ -75,6 +77,7 @@ struct XWork {
   gd* d;
   gd* e;
   gd2* tau;
+  QMX_GLOBAL int* status;
 };
 
 __device__ __forceinline__ XWork x_work(unsigned char* work, int n, int nw) {
@@ -87,14 +90,20 @@ __device__ __forceinline__ XWork x_work(unsigned char* work, int n, int nw) {
   w.d = w.U3 + (int64_t)n * nw;
   w.e = w.d + n;
   w.tau = (gd2*)(w.e + n);
+  w.status = (QMX_GLOBAL int*)(w.tau + n);
   return w;
 }
 
 // LDS of the reduction kernel: v, w (n complex), ypart (kXThreads complex),
 // red (8 doubles), then the packed lower triangle of the trailing block.
+// Reduction kernel threads: 512 for n_max > kRBigN (one matrix per CU when
+// the packed block fills the LDS), 256 below.
+constexpr int kRMaxThreads = 512;
+constexpr int kRBigN = 96;
 __host__ __device__ inline size_t r_lds_head(int64_t n) {
-  return (size_t)(2 * n + kXThreads) * 16 + 8 * 8;
+  return (size_t)(2 * n + kRMaxThreads) * 16 + 8 * 8;
 }
+
 constexpr size_t kLdsMax = 163840;
 
 __device__ __forceinline__ double xwave_sum(double v) {
@@ -112,6 +121,19 @@ __device__ __forceinline__ double xblock_sum(double x, double* red) {
   double s = 0;
 #pragma unroll
   for (int i = 0; i < kXWaves; ++i) s += red[i];
+  return s;
+}
+
+template <int kRWaves>
+__device__ __forceinline__ double rblock_sum(double x, double* red) {
+  x = xwave_sum(x);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = x;
+  __syncthreads();
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < kRWaves; ++i) s += red[i];
   return s;
 }
 
@@ -362,6 +384,7 @@ __device__ void group_orthonormalise(gd* X, int n, int nw, int s, int j) {
 // LDS (row a = i - ks at offset a (a + 1) / 2), and only the reflectors go
 // back to global memory. ks is the first step whose trailing block fits in
 // the `cap` complex elements of LDS left after the vectors.
+template <int kRThreads>
 __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned char* work,
                                  int nw, int64_t cap) {
   extern __shared__ __align__(16) unsigned char smem_r[];
@@ -374,8 +397,9 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
   L.v = reinterpret_cast<double2*>(smem_r);
   L.w = L.v + n;
   L.ypart = L.w + n;
-  L.red = reinterpret_cast<double*>(L.ypart + kXThreads);
+  L.red = reinterpret_cast<double*>(L.ypart + kRMaxThreads);
   double2* P = reinterpret_cast<double2*>(smem_r + r_lds_head(n));
+  constexpr int kRWaves = kRThreads / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const gd2* A = (const gd2*)A_;
   const XWork G = x_work(work, n, nw);
@@ -387,7 +411,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
   // 0. Hermitian working copy from the lower triangle (global steps), or
   // the packed lower triangle straight into LDS.
   if (ks > 0) {
-    for (int idx = tid; idx < n * n; idx += kXThreads) {
+    for (int idx = tid; idx < n * n; idx += kRThreads) {
       const int i = idx / n, j = idx - i * n;
       double2 x;
       if (j < i) {
@@ -410,11 +434,11 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     gd2* rowk = W + (int64_t)k * n;
     const double2 a1 = ld2(rowk + k + 1);  // W[k][k+1] = conj(alpha), read before it is overwritten
     double s = 0;
-    for (int i = k + 2 + tid; i < n; i += kXThreads) {
+    for (int i = k + 2 + tid; i < n; i += kRThreads) {
       const double2 x = ld2(rowk + i);
       s += x.x * x.x + x.y * x.y;
     }
-    const double sigma = xblock_sum(s, L.red);
+    const double sigma = rblock_sum<kRWaves>(s, L.red);
     const double2 alpha = double2{a1.x, -a1.y};
     double beta, sr = 0, si = 0;
     double2 tk;
@@ -430,7 +454,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       si = -ai / den;
     }
     // v_i = x_i / (alpha - beta) with x_i = conj(W[k][i]); kept in LDS and in row k.
-    for (int i = k + 2 + tid; i < n; i += kXThreads) {
+    for (int i = k + 2 + tid; i < n; i += kRThreads) {
       const double2 xr = ld2(rowk + i);
       const double xx = xr.x, xy = -xr.y;
       const double2 vv = double2{xx * sr - xy * si, xx * si + xy * sr};
@@ -447,7 +471,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     __syncthreads();
     if (tk.x == 0.0 && tk.y == 0.0) continue;
     // y = A' v over the trailing block, A' Hermitian: y_i = sum_j conj(W[j][i]) v_j.
-    int P = kXThreads / m;
+    int P = kRThreads / m;
     P = P > 8 ? 8 : (P < 1 ? 1 : P);
     const int chunk = (m + P - 1) / P;
     if (tid < P * m) {
@@ -476,7 +500,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     }
     __syncthreads();
     double part = 0;
-    for (int ii = tid; ii < m; ii += kXThreads) {
+    for (int ii = tid; ii < m; ii += kRThreads) {
       double2 y = L.ypart[ii];
       for (int p = 1; p < P; ++p) {
         const double2 t = L.ypart[p * m + ii];
@@ -487,9 +511,9 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       const double2 vi = L.v[k + 1 + ii];
       part += vi.x * y.x + vi.y * y.y;
     }
-    const double vy = xblock_sum(part, L.red);
+    const double vy = rblock_sum<kRWaves>(part, L.red);
     const double t2 = tk.x * tk.x + tk.y * tk.y;
-    for (int i = k + 1 + tid; i < n; i += kXThreads) {
+    for (int i = k + 1 + tid; i < n; i += kRThreads) {
       const double2 y = L.w[i], vi = L.v[i];
       L.w[i] = double2{tk.x * y.x - tk.y * y.y - 0.5 * t2 * vy * vi.x,
                        tk.x * y.y + tk.y * y.x - 0.5 * t2 * vy * vi.y};
@@ -504,8 +528,8 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
         vj[c] = j < n ? L.v[j] : double2{0.0, 0.0};
         wj[c] = j < n ? L.w[j] : double2{0.0, 0.0};
       }
-      for (int i0 = k + 1 + wave; i0 < n; i0 += 2 * kXWaves) {
-        const int i1 = i0 + kXWaves;
+      for (int i0 = k + 1 + wave; i0 < n; i0 += 2 * kRWaves) {
+        const int i1 = i0 + kRWaves;
         double2 a0[4], a1v[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -549,7 +573,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
 
   // 1b. The trailing block packed in LDS; steps ks..n-2 there.
   auto pk = [ks](int i, int j) { return (i - ks) * (i - ks + 1) / 2 + (j - ks); };
-  for (int i = ks + wave; i < n; i += kXWaves) {
+  for (int i = ks + wave; i < n; i += kRWaves) {
     for (int j = ks + lane; j <= i; j += 64) {
       double2 x;
       if (ks > 0) {
@@ -567,11 +591,11 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     gd2* rowk = W + (int64_t)k * n;
     const double2 alpha = P[pk(k + 1, k)];  // x_i = A[i][k], i > k
     double s = 0;
-    for (int i = k + 2 + tid; i < n; i += kXThreads) {
+    for (int i = k + 2 + tid; i < n; i += kRThreads) {
       const double2 x = P[pk(i, k)];
       s += x.x * x.x + x.y * x.y;
     }
-    const double sigma = xblock_sum(s, L.red);
+    const double sigma = rblock_sum<kRWaves>(s, L.red);
     double beta, sr = 0, si = 0;
     double2 tk;
     if (sigma == 0.0 && alpha.y == 0.0) {
@@ -585,7 +609,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       sr = ar / den;
       si = -ai / den;
     }
-    for (int i = k + 2 + tid; i < n; i += kXThreads) {
+    for (int i = k + 2 + tid; i < n; i += kRThreads) {
       const double2 x = P[pk(i, k)];
       const double2 vv = double2{x.x * sr - x.y * si, x.x * si + x.y * sr};
       L.v[i] = vv;
@@ -602,7 +626,7 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     if (tk.x == 0.0 && tk.y == 0.0) continue;
     // y = A' v with A' Hermitian from its lower triangle: A_ij = P(i, j)
     // for j <= i, conj(P(j, i)) for j > i.
-    int Pp = kXThreads / m;
+    int Pp = kRThreads / m;
     Pp = Pp > 8 ? 8 : (Pp < 1 ? 1 : Pp);
     const int chunk = (m + Pp - 1) / Pp;
     if (tid < Pp * m) {
@@ -612,21 +636,59 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       double yr = 0, yi = 0;
       const int jm = min(j1, i + 1);
       const int rb = (i - ks) * (i - ks + 1) / 2 - ks;  // row i: P[rb + j]
-      for (int j = j0; j < jm; ++j) {
+      double yr1 = 0, yi1 = 0;
+      int j = j0;
+      for (; j + 4 <= jm; j += 4) {  // row i, j <= i: A_ij = P(i, j)
+        double2 a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] = P[rb + j + q];
+          b[q] = L.v[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q += 2) {
+          yr += a[q].x * b[q].x - a[q].y * b[q].y;
+          yi += a[q].x * b[q].y + a[q].y * b[q].x;
+          yr1 += a[q + 1].x * b[q + 1].x - a[q + 1].y * b[q + 1].y;
+          yi1 += a[q + 1].x * b[q + 1].y + a[q + 1].y * b[q + 1].x;
+        }
+      }
+      for (; j < jm; ++j) {
         const double2 a = P[rb + j], b = L.v[j];
         yr += a.x * b.x - a.y * b.y;
         yi += a.x * b.y + a.y * b.x;
       }
-      for (int j = max(j0, i + 1); j < j1; ++j) {
-        const double2 a = P[pk(j, i)], b = L.v[j];  // conj(a) * b
+      // column i below the diagonal, j > i: A_ij = conj(P(j, i)); P(j, i) at
+      // (j - ks)(j - ks + 1) / 2 + i - ks, the offset growing by j - ks + 1.
+      j = max(j0, i + 1);
+      int off = pk(j, i);
+      for (; j + 4 <= j1; j += 4) {
+        double2 a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] = P[off];
+          off += j + q - ks + 1;
+          b[q] = L.v[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q += 2) {
+          yr += a[q].x * b[q].x + a[q].y * b[q].y;
+          yi += a[q].x * b[q].y - a[q].y * b[q].x;
+          yr1 += a[q + 1].x * b[q + 1].x + a[q + 1].y * b[q + 1].y;
+          yi1 += a[q + 1].x * b[q + 1].y - a[q + 1].y * b[q + 1].x;
+        }
+      }
+      for (; j < j1; ++j) {
+        const double2 a = P[off], b = L.v[j];  // conj(a) * b
+        off += j - ks + 1;
         yr += a.x * b.x + a.y * b.y;
         yi += a.x * b.y - a.y * b.x;
       }
-      L.ypart[tid] = double2{yr, yi};
+      L.ypart[tid] = double2{yr + yr1, yi + yi1};
     }
     __syncthreads();
     double part = 0;
-    for (int ii = tid; ii < m; ii += kXThreads) {
+    for (int ii = tid; ii < m; ii += kRThreads) {
       double2 y = L.ypart[ii];
       for (int p = 1; p < Pp; ++p) {
         const double2 t = L.ypart[p * m + ii];
@@ -637,24 +699,54 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       const double2 vi = L.v[k + 1 + ii];
       part += vi.x * y.x + vi.y * y.y;
     }
-    const double vy = xblock_sum(part, L.red);
+    const double vy = rblock_sum<kRWaves>(part, L.red);
     const double t2 = tk.x * tk.x + tk.y * tk.y;
-    for (int i = k + 1 + tid; i < n; i += kXThreads) {
+    for (int i = k + 1 + tid; i < n; i += kRThreads) {
       const double2 y = L.w[i], vi = L.v[i];
       L.w[i] = double2{tk.x * y.x - tk.y * y.y - 0.5 * t2 * vy * vi.x,
                        tk.x * y.y + tk.y * y.x - 0.5 * t2 * vy * vi.y};
     }
     __syncthreads();
     // Lower triangle of A' -= v w^H + w v^H: rows to waves, columns to lanes.
-    for (int i = k + 1 + wave; i < n; i += kXWaves) {
-      const double2 vi = L.v[i], wi = L.w[i];
-      const int rb = (i - ks) * (i - ks + 1) / 2 - ks;
-      for (int j = k + 1 + lane; j <= i; j += 64) {
-        const double2 vj = L.v[j], wj = L.w[j];
-        double2 a = P[rb + j];
-        a.x -= vi.x * wj.x + vi.y * wj.y + wi.x * vj.x + wi.y * vj.y;
-        a.y -= vi.y * wj.x - vi.x * wj.y + wi.y * vj.x - wi.x * vj.y;
-        P[rb + j] = a;
+    {
+      // rows i = k + 1 + wave + 4 t, four at a time (all loads first), lanes
+      // along the row j = k + 1 .. i (conflict-free).
+      double2 vj[3], wj[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int j = k + 1 + lane + 64 * c;
+        vj[c] = j < n ? L.v[j] : double2{0.0, 0.0};
+        wj[c] = j < n ? L.w[j] : double2{0.0, 0.0};
+      }
+      for (int i0 = k + 1 + wave; i0 < n; i0 += 4 * kRWaves) {
+        double2 a[4][3];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int i = i0 + rr * kRWaves;
+          const int rb = (i - ks) * (i - ks + 1) / 2 - ks;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int j = k + 1 + lane + 64 * c;
+            a[rr][c] = (i < n && j <= i) ? P[rb + j] : double2{0.0, 0.0};
+          }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int i = i0 + rr * kRWaves;
+          if (i >= n) break;
+          const double2 vi = L.v[i], wi = L.w[i];
+          const int rb = (i - ks) * (i - ks + 1) / 2 - ks;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int j = k + 1 + lane + 64 * c;
+            if (j <= i) {
+              double2 b = a[rr][c];
+              b.x -= vi.x * wj[c].x + vi.y * wj[c].y + wi.x * vj[c].x + wi.y * vj[c].y;
+              b.y -= vi.y * wj[c].x - vi.x * wj[c].y + wi.y * vj[c].x - wi.x * vj[c].y;
+              P[rb + j] = b;
+            }
+          }
+        }
       }
     }
     __syncthreads();
@@ -705,7 +797,10 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
       const int i = idx / nw, j = idx - i * nw;
       st2(V + (int64_t)i * ldv + j, double2{__builtin_nan(""), 0.0});
     }
-    if (tid == 0 && info) *info = 1;
+    if (tid == 0) {
+      if (info) *info = 1;
+      *G.status = 1;
+    }
     return;
   }
   int ex = 0;
@@ -849,7 +944,101 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
   __threadfence_block();
   __syncthreads();
 
-  // 4. V = Q X = H_0 H_1 ... H_{n-2} X, kBtCols columns per wave pass.
+  if (tid == 0) {
+    if (info) *info = 0;
+    *G.status = 0;
+  }
+}
+
+// Phase 4 of one matrix: V = Q X from the reflectors (rows of W), tau and the
+// tridiagonal eigenvectors X in its work area (nothing when phase 2 found a
+// non-finite input).
+__device__ void heevx_back_one(int n, double2* V_, int64_t ldv, unsigned char* work, int nw) {
+  extern __shared__ __align__(16) unsigned char smem_b[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  gd2* V = (gd2*)V_;
+  const XWork G = x_work(work, n, nw);
+  if (*G.status != 0) return;
+  gd2* W = G.W;
+  gd* X = G.X;
+  struct {
+    double2* tau;
+    double2* vb;
+    double2* sp;
+  } L;
+  L.tau = reinterpret_cast<double2*>(smem_b);
+  L.vb = L.tau + n;
+  L.sp = L.vb + 3 * kXWaves * kBtR;
+  for (int i = tid; i < n; i += kXThreads) L.tau[i] = ld2(G.tau + i);
+  __syncthreads();
+  // 4. V = Q X = H_0 H_1 ... H_{n-2} X.
+  if (n <= kXWaves * kBtR) {
+    // Waves own blocks of kBtR rows, lanes own columns (64 at a time): each
+    // reflector is a partial dot per wave, one barrier, a sum over the waves
+    // and the update. Reflectors go through a triple LDS buffer padded with
+    // zeros to kXWaves * kBtR rows (v_{k-1} is stored while v_k is used,
+    // v_{k-2} loaded into a register), the partial dots through a double one.
+    constexpr int kPad = kXWaves * kBtR;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int r0 = wv * kBtR;
+    for (int g0 = 0; g0 < nw; g0 += 64) {
+      const int j = g0 + lane;
+      const bool jok = j < nw;
+      double2 vr[kBtR];
+#pragma unroll
+      for (int r = 0; r < kBtR; ++r)
+        vr[r] = double2{(r0 + r < n && jok) ? X[(int64_t)(r0 + r) * nw + j] : 0.0, 0.0};
+      if (tid < kPad) {
+        const int k = n - 2;
+        L.vb[((k + 3) % 3) * kPad + tid] =
+            (k >= 0 && tid > k && tid < n) ? ld2(W + (int64_t)k * n + tid) : double2{0.0, 0.0};
+      }
+      double2 vreg = (n >= 3 && tid < n && tid > n - 3) ? ld2(W + (int64_t)(n - 3) * n + tid)
+                                                        : double2{0.0, 0.0};
+      __syncthreads();
+      for (int k = n - 2; k >= 0; --k) {
+        const double2* vk = L.vb + (k % 3) * kPad + r0;
+        if (k > 0 && tid < kPad) L.vb[((k - 1) % 3) * kPad + tid] = vreg;  // v_{k-1}
+        vreg = (k > 1 && tid < n && tid > k - 2) ? ld2(W + (int64_t)(k - 2) * n + tid)
+                                                 : double2{0.0, 0.0};  // v_{k-2}
+        const bool live = r0 + kBtR - 1 > k;  // this wave has rows below k
+        double pr = 0, pi = 0;
+        if (live) {
+#pragma unroll
+          for (int r = 0; r < kBtR; ++r) {
+            const double2 v = vk[r];
+            pr += v.x * vr[r].x + v.y * vr[r].y;
+            pi += v.x * vr[r].y - v.y * vr[r].x;
+          }
+        }
+        L.sp[(k & 1) * (kXWaves * 64) + wv * 64 + lane] = double2{pr, pi};
+        __syncthreads();
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int w = 0; w < kXWaves; ++w) {
+          const double2 t = L.sp[(k & 1) * (kXWaves * 64) + w * 64 + lane];
+          sr += t.x;
+          si += t.y;
+        }
+        const double2 tk = L.tau[k];
+        const double tr = tk.x * sr - tk.y * si, ti = tk.x * si + tk.y * sr;
+        if (live) {
+#pragma unroll
+          for (int r = 0; r < kBtR; ++r) {
+            const double2 v = vk[r];
+            vr[r].x -= v.x * tr - v.y * ti;
+            vr[r].y -= v.x * ti + v.y * tr;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kBtR; ++r)
+        if (r0 + r < n && jok) st2(V + (int64_t)(r0 + r) * ldv + j, vr[r]);
+      __syncthreads();
+    }
+    return;
+  }
+  // Larger n: 4 columns per wave pass, lanes along the rows, wave reductions.
   for (int c0 = kBtCols * wave; c0 < nw; c0 += kBtCols * kXWaves) {
     double2 vr[kBtRows][kBtCols];
 #pragma unroll
@@ -917,15 +1106,21 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
       }
     }
   }
-  if (tid == 0 && info) *info = 0;
 }
 
-__global__ void __launch_bounds__(kXThreads)
+template <int NT>
+__global__ void __launch_bounds__(NT)
     k_heevx_reduce(const qmx_heev_desc* __restrict__ desc, int64_t cap) {
   const qmx_heev_desc d = desc[blockIdx.x];
   const int nw = d.n_want > 0 && d.n_want < d.n ? d.n_want : d.n;
-  heevx_reduce_one(static_cast<const double2*>(d.A), d.n, d.lda, static_cast<unsigned char*>(d.work),
-                   nw, cap);
+  heevx_reduce_one<NT>(static_cast<const double2*>(d.A), d.n, d.lda,
+                       static_cast<unsigned char*>(d.work), nw, cap);
+}
+
+__global__ void __launch_bounds__(kXThreads) k_heevx_back(const qmx_heev_desc* __restrict__ desc) {
+  const qmx_heev_desc d = desc[blockIdx.x];
+  const int nw = d.n_want > 0 && d.n_want < d.n ? d.n_want : d.n;
+  heevx_back_one(d.n, static_cast<double2*>(d.V), d.ldv, static_cast<unsigned char*>(d.work), nw);
 }
 
 __global__ void __launch_bounds__(kXThreads) k_heevx_tail(const qmx_heev_desc* __restrict__ desc) {
@@ -954,14 +1149,27 @@ extern "C" int qmx_heevx_grouped(const qmx_heev_desc* desc, int64_t count, int64
   const size_t want = r_lds_head(n_max) + (size_t)16 * n_max * (n_max + 1) / 2;
   const size_t shr = want < kLdsMax ? want : kLdsMax;
   const int64_t cap = (int64_t)((shr - r_lds_head(n_max)) / 16);
-  QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_reduce,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
-  hipLaunchKernelGGL(k_heevx_reduce, dim3((unsigned)count), dim3(kXThreads), shr, st, desc, cap);
+  if (n_max > kRBigN) {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_reduce<kRMaxThreads>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+    hipLaunchKernelGGL(k_heevx_reduce<kRMaxThreads>, dim3((unsigned)count), dim3(kRMaxThreads),
+                       shr, st, desc, cap);
+  } else {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_reduce<kXThreads>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+    hipLaunchKernelGGL(k_heevx_reduce<kXThreads>, dim3((unsigned)count), dim3(kXThreads), shr,
+                       st, desc, cap);
+  }
   QMX_LAUNCHED();
   const size_t sh = x_lds_bytes(n_max);
   QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_tail,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   hipLaunchKernelGGL(k_heevx_tail, dim3((unsigned)count), dim3(kXThreads), sh, st, desc);
+  QMX_LAUNCHED();
+  const size_t shb = (size_t)(n_max + 3 * kXWaves * kBtR + 2 * kXWaves * 64) * 16;
+  QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_back,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb));
+  hipLaunchKernelGGL(k_heevx_back, dim3((unsigned)count), dim3(kXThreads), shb, st, desc);
   QMX_LAUNCHED();
   return QMX_OK;
 }
