@@ -240,6 +240,30 @@ int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_m
 int64_t qmx_heevx_work_bytes(int64_t n, int64_t n_want);
 int qmx_heevx_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
 
+/* Fused steps of the batched Davidson iteration (no counterpart as kernels
+ * in the reference, which runs them as tensor expressions):
+ * - qmx_normalize_constrain: X[k][b] /= norm[k][b] (0 where norm <= 1e-30),
+ *   zero padding g >= n[k] (_davidson.py:186-187 + constrain,
+ *   _wavefunction.py:264-275); norm: double [nk][nb] on the device;
+ * - qmx_canonical_basis: Ut[b][j][i] = W[b][i][j] / sqrt(lam[b][j]) where
+ *   lam > tol, else 0 (W: complex128 eigenvectors in columns, m x m per
+ *   batch; Ut in dtype): the canonical orthonormalisation of the residual
+ *   block, transposed for the rotation Ut @ R;
+ * - qmx_subspace_assemble: E = O - I and H of the expanded subspace
+ *   [C, R] (2 nb x 2 nb per batch, complex128) from XO = [C^H R; R^H R] and
+ *   XH = [C^H HR; R^H HR] (2 nb x nb each), eig (current Ritz values) and lam
+ *   (residual-overlap eigenvalues: directions with lam <= tol are dropped,
+ *   their O diagonal set to 1 and H diagonal shifted by `shift`); the
+ *   TileExpansion layout of _davidson.py:196-206,250-264 with C^H C = I and
+ *   C^H H C = diag(eig). */
+int qmx_normalize_constrain(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk, int64_t nb,
+                            int64_t n_tot, const double* norm, const int32_t* n, void* stream);
+int qmx_canonical_basis(int dtype, const void* W, const double* lam, int64_t batch, int64_t m,
+                        double tol, void* Ut, void* stream);
+int qmx_subspace_assemble(const void* XO, const void* XH, const double* eig, const double* lam,
+                          int64_t batch, int64_t nb, double tol, double shift, void* E, void* H,
+                          void* stream);
+
 /* Batched overlap product over the basis axis on the matrix cores:
  *   out[b][m][n] = (accumulate ? out : 0) + sum_g conj(A[b][m][g]) B[b][n][g]
  * (_dot, dft/electrons/_wavefunction_dot.py:66-84: C1^H C2 per spin and k).

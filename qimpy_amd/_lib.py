@@ -76,6 +76,10 @@ SIGNATURES = {
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_heevx_work_bytes": (i64, [i64, i64]),
     "qmx_heevx_grouped": (i32, [vp, i64, i64, vp]),
+    "qmx_normalize_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
+    "qmx_canonical_basis": (i32, [i32, vp, vp, i64, i64, ct.c_double, vp, vp]),
+    "qmx_subspace_assemble": (i32, [vp, vp, vp, vp, i64, i64, ct.c_double, ct.c_double, vp, vp,
+                                    vp]),
     "qmx_profile_enable": (i32, [i32]),
     "qmx_profile_read": (i32, [vp, vp, vp, i32]),
     "qmx_last_error": (ct.c_char_p, []),

@@ -240,6 +240,12 @@ def _scale_constrain(X: torch.Tensor, scale: Optional[torch.Tensor], n32: torch.
               G, None if s is None else _c(s), _c(n32), rc.stream_ptr())
 
 
+def _normalize_constrain(X: torch.Tensor, norm: torch.Tensor, n32: torch.Tensor) -> None:
+    K, B, G = X.shape
+    _lib.call("qmx_normalize_constrain", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K,
+              B, G, _c(norm), _c(n32), rc.stream_ptr())
+
+
 def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """conj(A) @ B^T over the basis axis, batched over k, complex128."""
     return gemm.dot128(A, B)
@@ -478,36 +484,29 @@ class Bucket:
         HR = self.HX[:, nb:]
         # Normalised residuals; a null one (below 1e-30, where 1/norm would
         # overflow float32) is zeroed and then dropped with the null space:
-        norm = self.norm
-        _scale_constrain(R, torch.where(norm > 1e-30, 1.0 / norm.clamp_min(1e-30), 0.0), self.n32)
+        _normalize_constrain(R, self.norm, self.n32)
         for _ in range(2):  # R <- (1 - C C^H) R, twice
             R.baddbmm_(_dot(C, R).to(dt).transpose(-2, -1), C, alpha=-1)
         # Canonical orthonormalisation of R (drops near-dependent directions):
-        lam, W = yield _dot(R, R).to(C128)
+        lam, W = yield _dot(R, R)
         tol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
-        keep = lam > tol
-        U = W * torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))[..., None, :]
-        R.copy_(torch.matmul(U.to(dt).transpose(-2, -1), R))
+        Ut = torch.empty((self.Kb, nb, nb), dtype=dt, device=rc.device)
+        _lib.call("qmx_canonical_basis", DTYPE_CODE[dt], _c(W), _c(lam), self.Kb, nb, tol, _c(Ut),
+                  rc.stream_ptr())
+        R.copy_(torch.matmul(Ut, R))
         self.hamiltonian(R, HR, 0, self.Kb)
-        XO = _dot(self.X, R).to(C128)  # (Kb, 2 nb, nb): [C^H R; R^H R]
-        XH = _dot(self.X, HR).to(C128)
+        XO = _dot(self.X, R)  # (Kb, 2 nb, nb): [C^H R; R^H R]
+        XH = _dot(self.X, HR)
         n2 = 2 * nb
-        O = torch.zeros((self.Kb, n2, n2), dtype=C128, device=rc.device)
-        H = torch.zeros_like(O)
-        O.diagonal(dim1=-2, dim2=-1)[:, :nb] = 1.0
-        H.diagonal(dim1=-2, dim2=-1)[:, :nb] = self.eig.to(C128)
-        O[:, :nb, nb:] = XO[:, :nb]
-        O[:, nb:, :nb] = dagger(XO[:, :nb])
-        O[:, nb:, nb:] = 0.5 * (XO[:, nb:] + dagger(XO[:, nb:]))
-        H[:, :nb, nb:] = XH[:, :nb]
-        H[:, nb:, :nb] = dagger(XH[:, :nb])
-        H[:, nb:, nb:] = 0.5 * (XH[:, nb:] + dagger(XH[:, nb:]))
-        drop = ~keep
-        O.diagonal(dim1=-2, dim2=-1)[:, nb:] += drop.to(torch.float64)
-        H.diagonal(dim1=-2, dim2=-1)[:, nb:] += torch.where(drop, self.DROP_SHIFT, 0.0)
-        S = _lowdin(O)
-        Hh = S @ H @ S
-        E, W2 = yield 0.5 * (Hh + dagger(Hh)), nb  # only the nb lowest eigenvectors
+        E = torch.empty((self.Kb, n2, n2), dtype=C128, device=rc.device)
+        H = torch.empty_like(E)
+        _lib.call("qmx_subspace_assemble", _c(XO), _c(XH), _c(self.eig), _c(lam), self.Kb, nb, tol,
+                  self.DROP_SHIFT, _c(E), _c(H), rc.stream_ptr())
+        # Loewdin factor of O = I + E: I - E/2 + 3 E^2 / 8 (error O(|E|^3)).
+        S = torch.baddbmm(E, E, E, beta=-0.5, alpha=0.375)
+        S.diagonal(dim1=-2, dim2=-1).add_(1.0)
+        # The eigensolver reads the lower triangle of S H S.
+        E, W2 = yield S @ H @ S, nb  # only the nb lowest eigenvectors
         Vt = (S @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
         torch.matmul(Vt, self.X, out=self.X2[:, :nb])
         torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
