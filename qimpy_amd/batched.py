@@ -191,16 +191,22 @@ class CellJob:
 
     def release(self) -> None:
         self._dev = None
+        self._beta = None
 
     def projectors(self, G: int) -> torch.Tensor:
-        """beta (K, P, G) complex128, zero beyond each k's sphere."""
-        t = self.device_tables()
-        iG = t["iG"]
+        """beta (K, P, G) complex128, zero beyond each k's sphere; built once
+        per job (on its own basis) and kept on the device while it is
+        resident."""
+        beta = getattr(self, "_beta", None)
+        if beta is None:
+            t = self.device_tables()
+            beta = build_projectors(t["pos"], self.spec.proj_rc, self.spec.proj_l, t["iG"], t["k"],
+                                    self.spec.Gbasis, self.spec.volume,
+                                    torch.tensor(self.n, device=rc.device))
+            self._beta = beta
         if G > self.n_tot:
-            iG = torch.nn.functional.pad(iG, (0, 0, 0, G - self.n_tot))
-        return build_projectors(t["pos"], self.spec.proj_rc, self.spec.proj_l, iG[:, :G], t["k"],
-                                self.spec.Gbasis, self.spec.volume,
-                                torch.tensor(self.n, device=rc.device))
+            return torch.nn.functional.pad(beta, (0, G - self.n_tot))
+        return beta[..., :G]
 
     def hbm_bytes(self, nb: int, dtype: torch.dtype, G: Optional[int] = None) -> int:
         """Device bytes this job holds inside a bucket of band class nb."""
@@ -347,7 +353,7 @@ class Bucket:
             t = j.device_tables()
             ke[a : a + j.K, : j.n_tot] = t["ke"]
             if j.P:
-                beta[a : a + j.K, : j.P] = j.projectors(Gb).to(dt)
+                beta[a : a + j.K, : j.P, : j.n_tot] = j.projectors(j.n_tot)
                 D[a : a + j.K, : j.P] = t["D"].to(self.rdtype)
             V64[i] = t["V"]
             fi[a : a + j.K, : j.n_tot] = j.fft_index

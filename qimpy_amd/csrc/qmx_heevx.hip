@@ -23,6 +23,8 @@
 //      reflectors with wave reductions only (no block barriers).
 // Same input and output conventions as qmx_heev_small_grouped; E gets all n
 // eigenvalues (ascending), V the first n_want eigenvectors.
+#include <mutex>
+
 #include "qmx_common.h"
 
 using namespace qmx;
@@ -1132,6 +1134,21 @@ __global__ void __launch_bounds__(kXThreads) k_heevx_tail(const qmx_heev_desc* _
 
 }  // namespace
 
+namespace {
+// Dynamic LDS limits already set for this process (raised only when a larger
+// launch needs it; the attribute call costs more than a launch).
+size_t g_lds_reduce = 0, g_lds_reduce_big = 0, g_lds_tail = 0, g_lds_back = 0;
+std::mutex g_lds_mu;
+
+hipError_t set_lds_limit(const void* fn, size_t bytes, size_t* done) {
+  std::lock_guard<std::mutex> lock(g_lds_mu);
+  if (bytes <= *done) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) *done = bytes;
+  return e;
+}
+}  // namespace
+
 extern "C" int64_t qmx_heevx_work_bytes(int64_t n, int64_t n_want) {
   if (n < 1 || n > kXMaxN) return -1;
   if (n_want <= 0 || n_want > n) n_want = n;
@@ -1150,25 +1167,21 @@ extern "C" int qmx_heevx_grouped(const qmx_heev_desc* desc, int64_t count, int64
   const size_t shr = want < kLdsMax ? want : kLdsMax;
   const int64_t cap = (int64_t)((shr - r_lds_head(n_max)) / 16);
   if (n_max > kRBigN) {
-    QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_reduce<kRMaxThreads>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+    QMX_HIP(set_lds_limit((const void*)k_heevx_reduce<kRMaxThreads>, shr, &g_lds_reduce_big));
     hipLaunchKernelGGL(k_heevx_reduce<kRMaxThreads>, dim3((unsigned)count), dim3(kRMaxThreads),
                        shr, st, desc, cap);
   } else {
-    QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_reduce<kXThreads>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+    QMX_HIP(set_lds_limit((const void*)k_heevx_reduce<kXThreads>, shr, &g_lds_reduce));
     hipLaunchKernelGGL(k_heevx_reduce<kXThreads>, dim3((unsigned)count), dim3(kXThreads), shr,
                        st, desc, cap);
   }
   QMX_LAUNCHED();
   const size_t sh = x_lds_bytes(n_max);
-  QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_tail,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  QMX_HIP(set_lds_limit((const void*)k_heevx_tail, sh, &g_lds_tail));
   hipLaunchKernelGGL(k_heevx_tail, dim3((unsigned)count), dim3(kXThreads), sh, st, desc);
   QMX_LAUNCHED();
   const size_t shb = (size_t)(n_max + 3 * kXWaves * kBtR + 2 * kXWaves * 64) * 16;
-  QMX_HIP(hipFuncSetAttribute((const void*)k_heevx_back,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb));
+  QMX_HIP(set_lds_limit((const void*)k_heevx_back, shb, &g_lds_back));
   hipLaunchKernelGGL(k_heevx_back, dim3((unsigned)count), dim3(kXThreads), shb, st, desc);
   QMX_LAUNCHED();
   return QMX_OK;

@@ -13,6 +13,8 @@ of dft/electrons/_davidson.py:141-235 for one cell.
 from __future__ import annotations
 
 import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Callable, Optional, Sequence
 
@@ -38,30 +40,40 @@ class ServeReport:
 
 def serve(specs: Sequence, arrivals: Sequence[float], solver: BatchSolver, *,
           on_result: Optional[Callable[[CellResult], None]] = None,
-          progress: Optional[Callable[[str], None]] = None) -> ServeReport:
+          progress: Optional[Callable[[str], None]] = None, build_workers: int = 4) -> ServeReport:
     """Serve requests `specs[i]` arriving at `arrivals[i]` seconds after the
-    solver's clock start (ascending). Jobs are built from their specs when
-    they arrive (sphere tables on the host), then admitted by the solver."""
+    solver's clock start (ascending). A request's job (sphere tables, on the
+    host) is built by a pool of `build_workers` threads from its arrival on,
+    overlapping the solver's device work; built jobs enter the solver in
+    arrival order."""
     order = np.argsort(np.asarray(arrivals, dtype=np.float64), kind="stable")
     n = len(order)
     nxt = 0
+    building: deque = deque()
     t_last_report = time.perf_counter()
-    while nxt < n or not solver.idle:
-        now = solver.clock()
-        while nxt < n and arrivals[order[nxt]] <= now:
-            i = int(order[nxt])
-            solver.submit([CellJob(specs[i], job_id=i, arrival=float(arrivals[i]))])
-            nxt += 1
-        if solver.idle and nxt < n:  # nothing resident: wait for the next arrival
-            time.sleep(max(0.0, float(arrivals[order[nxt]]) - solver.clock()))
-            continue
-        for res in solver.tick(now=now):
-            if on_result is not None:
-                on_result(res)
-        if progress is not None and time.perf_counter() - t_last_report > 5.0:
-            t_last_report = time.perf_counter()
-            progress(f"t={solver.clock():.1f}s arrived {nxt}/{n} done {len(solver.results)}"
-                     f" queued {len(solver.queue)}")
+    with ThreadPoolExecutor(max_workers=max(1, build_workers)) as pool:
+        while nxt < n or building or not solver.idle:
+            now = solver.clock()
+            while nxt < n and arrivals[order[nxt]] <= now:
+                i = int(order[nxt])
+                building.append(pool.submit(CellJob, specs[i], job_id=i,
+                                            arrival=float(arrivals[i])))
+                nxt += 1
+            while building and building[0].done():
+                solver.submit([building.popleft().result()])
+            if solver.idle:
+                if building:  # nothing resident: wait for the next job to be built
+                    building[0].result()
+                elif nxt < n:  # or for the next arrival
+                    time.sleep(max(0.0, float(arrivals[order[nxt]]) - solver.clock()))
+                continue
+            for res in solver.tick(now=now):
+                if on_result is not None:
+                    on_result(res)
+            if progress is not None and time.perf_counter() - t_last_report > 5.0:
+                t_last_report = time.perf_counter()
+                progress(f"t={solver.clock():.1f}s arrived {nxt}/{n} done {len(solver.results)}"
+                         f" queued {len(solver.queue)}")
     torch.cuda.synchronize()
     res = solver.results
     lat = np.array([r.latency for r in res.values()])
