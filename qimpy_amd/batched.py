@@ -177,16 +177,17 @@ class CellJob:
         """iG, k, KE and the potential on the device (built once)."""
         if self._dev is None:
             dev = rc.device
-            k = torch.tensor(np.asarray(self.spec.k, dtype=np.float64).reshape(-1, 3), device=dev)
-            iG = torch.tensor(self.iG_np, device=dev)
-            Gk = (iG.to(torch.float64) + k[:, None, :]) @ torch.tensor(self.spec.Gbasis.T, device=dev)
+            k = _to_device(np.asarray(self.spec.k, dtype=np.float64).reshape(-1, 3))
+            iG = _to_device(self.iG_np)
+            Gk = (iG.to(torch.float64) + k[:, None, :]) @ _to_device(
+                np.asarray(self.spec.Gbasis, dtype=np.float64).T)
             ke = 0.5 * Gk.square().sum(-1)
-            ke[torch.arange(self.n_tot, device=dev)[None, :] >= torch.tensor(self.n, device=dev)[:, None]] = 0.0
-            V = torch.tensor(np.asarray(self.spec.V, dtype=np.float64).reshape(-1), device=dev)
-            self._dev = dict(k=k, iG=iG, ke=ke, V=V,
-                             pos=torch.tensor(np.asarray(self.spec.positions, dtype=np.float64),
-                                              device=dev),
-                             D=torch.tensor(np.asarray(self.spec.D, dtype=np.float64), device=dev))
+            n_dev = _to_device(self.n)
+            ke[torch.arange(self.n_tot, device=dev)[None, :] >= n_dev[:, None]] = 0.0
+            V = _to_device(np.asarray(self.spec.V, dtype=np.float64).reshape(-1))
+            self._dev = dict(k=k, iG=iG, ke=ke, V=V, n=n_dev,
+                             pos=_to_device(np.asarray(self.spec.positions, dtype=np.float64)),
+                             D=_to_device(np.asarray(self.spec.D, dtype=np.float64)))
         return self._dev
 
     def release(self) -> None:
@@ -201,8 +202,7 @@ class CellJob:
         if beta is None:
             t = self.device_tables()
             beta = build_projectors(t["pos"], self.spec.proj_rc, self.spec.proj_l, t["iG"], t["k"],
-                                    self.spec.Gbasis, self.spec.volume,
-                                    torch.tensor(self.n, device=rc.device))
+                                    self.spec.Gbasis, self.spec.volume, t["n"])
             self._beta = beta
         if G > self.n_tot:
             return torch.nn.functional.pad(beta, (0, G - self.n_tot))
@@ -220,6 +220,29 @@ class CellJob:
 # ---------------------------------------------------------------- raw kernels
 def _c(x: torch.Tensor) -> int:
     return x.data_ptr()
+
+
+def _to_device(a, dtype=None) -> torch.Tensor:
+    """Host array -> device tensor through pinned memory without waiting for
+    the work queued on the stream (a pageable copy would)."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.pin_memory().to(rc.device, non_blocking=True)
+
+
+#: Plans of restacked buckets, destroyed at the next host synchronisation
+#: (freeing device memory waits for the whole device).
+_retired_plans: list = []
+
+
+def flush_retired_plans() -> None:
+    while _retired_plans:
+        plan = _retired_plans.pop()
+        try:
+            _lib.lib().qmx_plan_destroy(plan)
+        except Exception:  # pragma: no cover - interpreter shutdown
+            pass
 
 
 def _band_ke(X: torch.Tensor, ke: torch.Tensor) -> torch.Tensor:
@@ -304,10 +327,7 @@ class Bucket:
     def _destroy_plan(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:
-            try:
-                _lib.lib().qmx_plan_destroy(plan)
-            except Exception:  # pragma: no cover - interpreter shutdown
-                pass
+            _retired_plans.append(plan)  # destroyed at the next synchronisation
         self._plan = None
 
     def _restack(self, keep: list[CellJob], new: list[CellJob]) -> None:
@@ -333,7 +353,7 @@ class Bucket:
         deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
         if keep:
             where = {id(j): a for j, a in zip(self.jobs, self.k0)}
-            idx = torch.cat([torch.arange(where[id(j)], where[id(j)] + j.K) for j in keep]).to(dev)
+            idx = torch.cat([torch.arange(where[id(j)], where[id(j)] + j.K, device=dev) for j in keep])
             g = min(Gb, self.Gb)
             X[:Kkeep, :nb, :g] = self.X.index_select(0, idx)[:, :nb, :g]
             HX[:Kkeep, :nb, :g] = self.HX.index_select(0, idx)[:, :nb, :g]
@@ -368,7 +388,7 @@ class Bucket:
         self.X2, self.HX2 = torch.empty_like(X), torch.empty_like(HX)
         self.ke = ke
         self.ke_r = ke.to(self.rdtype)
-        self.n32 = torch.tensor(n, dtype=torch.int32, device=dev)
+        self.n32 = _to_device(n.astype(np.int32))
         self.beta, self.D = beta, D
         self.V64 = V64
         self.V = V64 if dt == C128 else V64.to(self.rdtype)
@@ -693,6 +713,7 @@ class BatchSolver:
             fails = torch.stack([(i != 0).sum() for i in self._info]).sum().to("cpu", non_blocking=True)
             self._info = []
         torch.cuda.current_stream().synchronize()
+        flush_retired_plans()
         if fails is not None and int(fails) != 0:
             raise RuntimeError(f"grouped QL eigensolver did not converge ({int(fails)} matrices)")
         self._collect(prev, [h.numpy() for h in host_prev])
@@ -779,4 +800,5 @@ class BatchSolver:
         while not self.idle:
             self.tick()
         torch.cuda.synchronize()
+        flush_retired_plans()
         return self.results

@@ -122,6 +122,8 @@ struct qmx_plan {
   int n_vslots;       // 1 + largest slot
   float2* d_tw32[3];
   double2* d_tw64[3];
+  void* d_block;  // device block holding the integer tables
+  void* h_stage;  // pinned staging copy of the block
 };
 
 namespace {
@@ -806,21 +808,38 @@ int fft3d_impl(const int32_t* shape, void* datav, int64_t batch, int direction, 
   return QMX_OK;
 }
 
-template <typename T>
-static int upload(T** dst, const std::vector<T>& src) {
-  *dst = nullptr;
-  if (src.empty()) return QMX_OK;
-  QMX_HIP(hipMalloc(dst, sizeof(T) * src.size()));
-  QMX_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+// The integer tables of a plan go to the device as one block: staged in
+// pinned host memory and copied asynchronously on the null stream (which
+// orders with every blocking stream), so creating a plan does not wait for
+// the work already queued on the device. The staging buffer lives as long as
+// the plan.
+static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>& src,
+                         const std::vector<int**>& dst) {
+  std::vector<size_t> off(src.size());
+  size_t total = 0;
+  for (size_t i = 0; i < src.size(); ++i) {
+    off[i] = total;
+    total += (src[i]->size() * sizeof(int) + 255) / 256 * 256;
+  }
+  for (int** d : dst) *d = nullptr;
+  if (total == 0) return QMX_OK;
+  QMX_HIP(hipHostMalloc(&p->h_stage, total, hipHostMallocDefault));
+  QMX_HIP(hipMalloc(&p->d_block, total));
+  auto* h = static_cast<char*>(p->h_stage);
+  auto* d = static_cast<char*>(p->d_block);
+  for (size_t i = 0; i < src.size(); ++i) {
+    if (src[i]->empty()) continue;
+    std::memcpy(h + off[i], src[i]->data(), src[i]->size() * sizeof(int));
+    *dst[i] = reinterpret_cast<int*>(d + off[i]);
+  }
+  QMX_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, nullptr));
   return QMX_OK;
 }
 
 static void free_plan(qmx_plan* p) {
   if (!p) return;
-  int* arrs[] = {p->d_x_list, p->d_pcol_start, p->d_col_iy, p->d_cstart,
-                 p->d_t1idx,  p->d_n,          p->d_vslot,  p->d_col_ix};
-  for (int* a : arrs)
-    if (a) (void)hipFree(a);
+  if (p->d_block) (void)hipFree(p->d_block);
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
   delete[] p->h_nsum;
   delete p;
 }
@@ -970,10 +989,10 @@ int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
   std::vector<int> col_ix(t.col_iy.size());
   for (size_t xa = 0; xa < t.x_list.size(); ++xa)
     for (int c = t.pcol_start[xa]; c < t.pcol_start[xa + 1]; ++c) col_ix[c] = t.x_list[xa];
-  if ((rc = upload(&p->d_x_list, t.x_list)) || (rc = upload(&p->d_pcol_start, t.pcol_start)) ||
-      (rc = upload(&p->d_col_iy, t.col_iy)) || (rc = upload(&p->d_cstart, t.cstart)) ||
-      (rc = upload(&p->d_t1idx, t.t1idx)) || (rc = upload(&p->d_n, nvec)) ||
-      (rc = upload(&p->d_vslot, vslot)) || (rc = upload(&p->d_col_ix, col_ix))) {
+  if ((rc = upload_tables(p, {&t.x_list, &t.pcol_start, &t.col_iy, &t.cstart, &t.t1idx, &nvec,
+                              &vslot, &col_ix},
+                          {&p->d_x_list, &p->d_pcol_start, &p->d_col_iy, &p->d_cstart,
+                           &p->d_t1idx, &p->d_n, &p->d_vslot, &p->d_col_ix}))) {
     free_plan(p);
     return rc;
   }

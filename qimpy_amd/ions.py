@@ -33,7 +33,8 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
     if n_atoms == 0 or not proj_rc or not proj_l:
         return torch.zeros((K, 0, G), dtype=torch.complex128, device=dev)
     iGk = iG.to(torch.float64) + k[:, None, :]
-    Gk = iGk @ torch.tensor(Gbasis.T, device=dev)
+    Gb = torch.from_numpy(np.ascontiguousarray(np.asarray(Gbasis, dtype=np.float64).T))
+    Gk = iGk @ (Gb.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else Gb)
     Gmag2 = Gk.square().sum(-1)
     Y = []
     for l in proj_l:
@@ -42,7 +43,9 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
         else:  # i^l r Y_1m, m order (y, z, x)
             Y.extend(1j * Y1 * Gk[..., a] for a in (1, 2, 0))
     Y = torch.stack(Y, dim=-1)  # (K, G, n_lm)
-    f = torch.exp(-0.5 * torch.tensor(proj_rc, device=dev, dtype=torch.float64) ** 2
+    rc_t = torch.tensor(proj_rc, dtype=torch.float64)
+    rc_t = rc_t.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else rc_t
+    f = torch.exp(-0.5 * rc_t ** 2
                   * Gmag2[..., None])  # (K, G, n_rc)
     # exp(-2 pi i (iG + k) . tau) / sqrt(V), atoms last (_ions.py:304-311):
     phase = torch.polar(torch.full((), 1.0 / np.sqrt(volume), device=dev, dtype=torch.float64),
