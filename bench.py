@@ -42,8 +42,9 @@ MIXED_RESIDUAL = 3e-4
 PLAIN_THRESHOLD = 1e-8
 #: Library kernels that are bandwidth (not latency) bound; the roofline
 #: reports the one with the largest total time.
-BANDWIDTH_KERNELS = {"k_sphere_to_cols", "k_cols_to_planes", "k_xpass_potential",
-                     "k_planes_to_cols", "k_cols_to_sphere", "k_plane_potential", "k_gemm"}
+BANDWIDTH_KERNELS = {f"{k}<{t}>" for t in ("c64", "c128") for k in (
+    "k_sphere_to_cols", "k_cols_to_planes", "k_xpass_potential", "k_planes_to_cols",
+    "k_cols_to_sphere", "k_plane_potential", "k_gemm")}
 
 
 def parse():
@@ -103,6 +104,26 @@ def cpu_baseline(specs, gpu_iterations, budget_s: float = 12.0) -> dict:
                    f"threads of {cpu_model()}), extrapolated to the iterations of their FP64 GPU "
                    f"solves"),
     }
+
+
+def measured_traffic(kernel: str, n_requests: int):
+    """HBM traffic of `kernel` relative to its compulsory bytes on this served
+    stream, from the two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE, each
+    scaled by the same counter on a 1 GiB copy) recorded in
+    profiles/pmc_traffic.json. The stream's batching depends on timing, so
+    the ratio (measured over the profiled runs' dispatches) is applied to
+    this run's bytes per launch. None when the file has no entry."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in table.get("entries", []):
+        if e.get("kernel") == kernel and e.get("requests") == n_requests:
+            return {"ratio": e["traffic_over_compulsory"],
+                    "source": f"{e['profile']} via profiles/pmc_traffic.json"}
+    return None
 
 
 def served(specs, arrivals, dtype, profile=False):
@@ -196,12 +217,14 @@ def main():
     eband_err = max(abs(mixed.results[i].Eband - plain.results[i].Eband) for i in mixed.results)
 
     # The same request alone gives the same result as in the mix:
-    ident = 0.0
-    for i in (0, len(specs) // 2):
-        alone = BatchSolver(C64, residual_threshold=MIXED_RESIDUAL, max_iterations=200)
-        alone.submit([CellJob(specs[i], job_id=i)])
-        r = alone.run()[i]
-        ident = max(ident, float(np.abs(r.eig - mixed.results[i].eig).max()))
+    ident = None
+    if not args.no_extra:
+        ident = 0.0
+        for i in (0, len(specs) // 2):
+            alone = BatchSolver(C64, residual_threshold=MIXED_RESIDUAL, max_iterations=200)
+            alone.submit([CellJob(specs[i], job_id=i)])
+            r = alone.run()[i]
+            ident = max(ident, float(np.abs(r.eig - mixed.results[i].eig).max()))
 
     # Roofline of the dominant bandwidth kernel over the timed mixed stream:
     names = {i: _lib.KERNEL_NAMES.get(i, str(i)) for i in prof}
@@ -254,6 +277,11 @@ def main():
         },
         "kernels": kernels,
     }
+    traffic = measured_traffic(names[kid], len(specs))
+    if traffic is not None:
+        out["roofline"]["traffic"] = round(traffic["ratio"] * by / cnt)
+        out["roofline"]["traffic_over_compulsory"] = traffic["ratio"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_extra:
         cells = systems.make_cfg2_cells(64)
         vm, rm = batched(cells, C64)

@@ -288,7 +288,8 @@ enum {
   QMX_KERNEL_GEMM = 6,
   QMX_KERNEL_PLANE = 7,   /* fused plane pass: inverse y, x, * V, forward x, y */
   QMX_KERNEL_VZMAJOR = 8, /* potential boxes to z-major order */
-  QMX_N_KERNEL_IDS = 16
+  QMX_KERNEL_C128 = 16,   /* added to the id of a complex128 launch */
+  QMX_N_KERNEL_IDS = 32
 };
 
 /* Event-based profiling of this library's launches (no counterpart in the

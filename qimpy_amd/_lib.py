@@ -176,12 +176,15 @@ def geom_tables(shape, n: np.ndarray, fft_index: np.ndarray) -> dict:
     return out
 
 
-KERNEL_NAMES = {
+_BASE_NAMES = {
     0: "k_sphere_to_cols", 1: "k_cols_to_planes", 2: "k_xpass_potential",
     3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
     7: "k_plane_potential", 8: "k_potential_zmajor",
 }
-N_KERNEL_IDS = 16
+#: Profile ids: the kernel id for complex64 launches, + 16 for complex128.
+KERNEL_NAMES = {**{i: f"{n}<c64>" for i, n in _BASE_NAMES.items()},
+                **{16 + i: f"{n}<c128>" for i, n in _BASE_NAMES.items()}}
+N_KERNEL_IDS = 32
 
 
 def profile_enable(on: bool) -> None:

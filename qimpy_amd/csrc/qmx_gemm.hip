@@ -236,7 +236,7 @@ int dot_impl(const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
                      static_cast<const C2*>(ws), batch, (int)M, (int)N, p.nchunks,
                      static_cast<double2*>(out), ldc, sc, accumulate);
   QMX_LAUNCHED();
-  prof_end(st, ev, QMX_KERNEL_GEMM, (double)batch * (M + N) * G * sizeof(C2));
+  prof_end(st, ev, QMX_KERNEL_GEMM + (sizeof(T) == 8 ? QMX_KERNEL_C128 : 0), (double)batch * (M + N) * G * sizeof(C2));
   return QMX_OK;
 }
 

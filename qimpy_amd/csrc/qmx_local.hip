@@ -600,6 +600,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                      int64_t ke_ldk, int64_t nb, int accumulate, void* ws, int64_t ws_bytes,
                      hipStream_t st) {
   using C2 = typename Cplx<R>::T;
+  const int idoff = sizeof(R) == 8 ? QMX_KERNEL_C128 : 0;  // profile ids per precision
   const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
   const int ncol = p->ncol, nxa = p->nxa;
   const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
@@ -661,7 +662,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_potential_zmajor<R>), gT, dim3(256), 0, st, V, vt, nxy, nz);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_VZMAJOR, 2.0 * p->n_vslots * (double)p->N * br);
+    prof_end(st, ev, idoff + QMX_KERNEL_VZMAJOR, 2.0 * p->n_vslots * (double)p->N * br);
     Vt = vt;
   }
   const size_t shP = plane_lds_bytes(p, sizeof(C2));
@@ -685,7 +686,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                          ldk, T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
                          p->n_tot, (int)b0);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_PASS_A, cur * (n_sphere * bc + t1_elems * bc));
+    prof_end(st, ev, idoff + QMX_KERNEL_PASS_A, cur * (n_sphere * bc + t1_elems * bc));
     if (plane) {
       dim3 gP(nz, cur, nk);
       prof_begin(st, &ev);
@@ -693,7 +694,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                          vslot, p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
                          tw_of<C2>(p, 1), p->rad[0], p->rad[1], fny, fnxa, ncol, nxa, cur);
       QMX_LAUNCHED();
-      prof_end(st, ev, QMX_KERNEL_PLANE, cur * 2.0 * t1_elems * bc + (double)nk * p->N * br);
+      prof_end(st, ev, idoff + QMX_KERNEL_PLANE, cur * 2.0 * t1_elems * bc + (double)nk * p->N * br);
     } else {
     dim3 gB((nz + My - 1) / My, nxa, nk * cur);
     prof_begin(st, &ev);
@@ -701,7 +702,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                        p->d_pcol_start, p->d_col_iy, tw_of<C2>(p, 1), p->rad[1], fy, ncol,
                        nxa, nz);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_PASS_B, cur * (t1_elems + t2_elems) * bc);
+    prof_end(st, ev, idoff + QMX_KERNEL_PASS_B, cur * (t1_elems + t2_elems) * bc);
     // Bands per chunk: enough workgroups to fill the chip, V reused within.
     const int64_t wg_base = (int64_t)((nz + Mzc - 1) / Mzc) * ny * nk;
     int n_chunks = (int)std::min<int64_t>((cur + nbg - 1) / nbg,
@@ -716,14 +717,14 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                        p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fxw, fxM, zshift, nbg, ny, nz,
                        nxa, cur, bpc, n_chunks);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_PASS_C,
+    prof_end(st, ev, idoff + QMX_KERNEL_PASS_C,
              cur * 2.0 * t2_elems * bc + (double)nk * n_chunks * (double)p->N * br);
     prof_begin(st, &ev);
     hipLaunchKernelGGL((k_planes_to_cols<C2, R>), gB, dim3(kThreads), shB, st, T2, T1,
                        p->d_pcol_start, p->d_col_iy, tw_of<C2>(p, 1), p->rad[1], fy, ncol,
                        nxa, nz);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
+    prof_end(st, ev, idoff + QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
     }
     prof_begin(st, &ev);
     if (plane)
@@ -737,7 +738,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
                          tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
                          accumulate);
     QMX_LAUNCHED();
-    prof_end(st, ev, QMX_KERNEL_PASS_E,
+    prof_end(st, ev, idoff + QMX_KERNEL_PASS_E,
              cur * (t1_elems * bc + n_sphere * bc * (1 + (ke ? 1 : 0) + (accumulate ? 1 : 0)) +
                     (ke ? n_sphere * br : 0.0)));
   }
