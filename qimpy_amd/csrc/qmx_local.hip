@@ -321,11 +321,20 @@ __global__ void __launch_bounds__(kThreads)
 // z-major, one box per potential slot), forward x-FFT, forward y-FFT along
 // the active rows; the active columns are gathered back to the same place.
 // HBM traffic per band is T1 in and out (V stays cached across bands).
-constexpr int kPlaneThreads = 512;
-constexpr int kPlaneMaxE = 24;  // complex values per thread in one FFT pass
+constexpr int kPlaneThreads = 512;  // planes above kPlaneSmallPts points
+constexpr int kPlaneThreadsSmall = 256;
+constexpr int kPlaneSmallPts = 4096;  // nx * ny up to a 64 x 64 plane
+constexpr int kPlaneMaxE = 24;       // complex values per thread in one FFT pass (512 threads)
+constexpr int kPlaneMaxESmall = 16;  // the same with 256 threads (planes up to 64 x 64)
+constexpr int kPlaneColPer = 4;      // active columns staged per thread
+constexpr int kPlaneVPer = 16;       // potential values prefetched per thread
 
-template <typename C2, typename R>
-__global__ void __launch_bounds__(kPlaneThreads)
+// Every global read of a plane is issued before the work that needs it: the
+// twiddles, the active columns (kept in registers across the LDS zeroing)
+// and the potential of the plane (in registers across the inverse y and x
+// transforms), so their latency overlaps LDS work instead of stalling it.
+template <typename C2, typename R, int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
     k_plane_potential(C2* __restrict__ T1, const R* __restrict__ Vt,
                       const int* __restrict__ vslot, const int* __restrict__ col_ix,
                       const int* __restrict__ col_iy, const int* __restrict__ x_list,
@@ -338,26 +347,58 @@ __global__ void __launch_bounds__(kPlaneThreads)
   C2* twy = twx + nx;
   int* xl = reinterpret_cast<int*>(twy + ny);
   const int z = blockIdx.x, ib = blockIdx.y, k = blockIdx.z;
-  for (int i = threadIdx.x; i < nx; i += blockDim.x) twx[i] = twx_g[i];
-  for (int i = threadIdx.x; i < ny; i += blockDim.x) twy[i] = twy_g[i];
-  for (int i = threadIdx.x; i < nxa; i += blockDim.x) xl[i] = x_list[i];
+  const int tid = threadIdx.x;
+  C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
+  C2 cv[kPlaneColPer];
+  int co[kPlaneColPer];
+#pragma unroll
+  for (int q = 0; q < kPlaneColPer; ++q) {
+    const int c = tid + q * NT;
+    if (c < ncol) {
+      cv[q] = row[c];
+      co[q] = col_ix[c] * ldp + col_iy[c];
+    }
+  }
+  for (int i = tid; i < nx; i += NT) twx[i] = twx_g[i];
+  for (int i = tid; i < ny; i += NT) twy[i] = twy_g[i];
+  for (int i = tid; i < nxa; i += NT) xl[i] = x_list[i];
   zero_lds(P, nx * ldp);
   __syncthreads();
-  C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) P[col_ix[c] * ldp + col_iy[c]] = row[c];
+#pragma unroll
+  for (int q = 0; q < kPlaneColPer; ++q)
+    if (tid + q * NT < ncol) P[co[q]] = cv[q];
   __syncthreads();
-  fft_ip<C2, +1, kPlaneThreads, kPlaneMaxE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
-  fft_ip<C2, +1, kPlaneThreads, kPlaneMaxE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
+  fft_ip<C2, +1, NT, MAXE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
   const R* Vz = Vt + ((int64_t)vslot[k] * nz + z) * nx * ny;
-  for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
+  const int nxy = nx * ny;
+  constexpr int kVPer = sizeof(R) == 4 ? kPlaneVPer : kPlaneVPer / 2;
+  R vv[kVPer];
+#pragma unroll
+  for (int q = 0; q < kVPer; ++q) {
+    const int i = tid + q * NT;
+    vv[q] = i < nxy ? Vz[i] : R(0);
+  }
+  fft_ip<C2, +1, NT, MAXE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
+#pragma unroll
+  for (int q = 0; q < kVPer; ++q) {
+    const int i = tid + q * NT;
+    if (i < nxy) {
+      const int x = fdiv(i, fny);
+      const int y = i - x * ny;
+      P[x * ldp + y] = cscale(P[x * ldp + y], vv[q]);
+    }
+  }
+  for (int i = tid + kVPer * NT; i < nxy; i += NT) {  // points beyond the prefetch
     const int x = fdiv(i, fny);
     const int y = i - x * ny;
     P[x * ldp + y] = cscale(P[x * ldp + y], Vz[i]);
   }
   __syncthreads();
-  fft_ip<C2, -1, kPlaneThreads, kPlaneMaxE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
-  fft_ip<C2, -1, kPlaneThreads, kPlaneMaxE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) row[c] = P[col_ix[c] * ldp + col_iy[c]];
+  fft_ip<C2, -1, NT, MAXE, true, false>(P, twx, rx, ny, fny, ldp, 1, nullptr);
+  fft_ip<C2, -1, NT, MAXE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
+#pragma unroll
+  for (int q = 0; q < kPlaneColPer; ++q)
+    if (tid + q * NT < ncol) row[tid + q * NT] = P[co[q]];
 }
 
 // Vt[s][z][xy] = V[s][xy][z]: each potential box to z-major order (32 x 32
@@ -564,8 +605,20 @@ static size_t plane_lds_bytes(const qmx_plan* p, int cs) {
   return (size_t)nx * odd_ld(ny) * cs + (size_t)(nx + ny) * cs + (size_t)p->nxa * sizeof(int);
 }
 
+// Threads of the plane kernel for this box: 256 for planes of up to
+// kPlaneSmallPts points whose passes fit the register budget, else 512.
+static int plane_threads(const qmx_plan* p) {
+  const int nt = kPlaneThreadsSmall;
+  if ((int64_t)p->shape[0] * p->shape[1] <= kPlaneSmallPts && p->ncol <= kPlaneColPer * nt &&
+      fft_ip_fits(p->rad[1], p->nxa, nt, kPlaneMaxESmall) &&
+      fft_ip_fits(p->rad[0], p->shape[1], nt, kPlaneMaxESmall))
+    return nt;
+  return kPlaneThreads;
+}
+
 static bool plane_fits(const qmx_plan* p, int cs) {
   return p->nxa > 0 && plane_lds_bytes(p, cs) <= (size_t)kLdsMax &&
+         p->ncol <= kPlaneColPer * kPlaneThreads &&
          fft_ip_fits(p->rad[1], p->nxa, kPlaneThreads, kPlaneMaxE) &&
          fft_ip_fits(p->rad[0], p->shape[1], kPlaneThreads, kPlaneMaxE);
 }
@@ -583,11 +636,11 @@ static int64_t vt_bytes(const qmx_plan* p, int rs) {
   return ((int64_t)p->n_vslots * p->N * rs + 255) / 256 * 256;
 }
 
-template <typename C2, typename R>
+template <typename C2, typename R, int NT, int MAXE>
 static int set_plane_lds(size_t bytes) {
   static size_t done = 0;
   if (bytes > done) {
-    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_potential<C2, R>,
+    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_potential<C2, R, NT, MAXE>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     done = bytes;
   }
@@ -666,8 +719,11 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     Vt = vt;
   }
   const size_t shP = plane_lds_bytes(p, sizeof(C2));
+  const int ntP = plane ? plane_threads(p) : 0;
   if (plane) {
-    const int rc = set_plane_lds<C2, R>(shP);
+    const int rc = ntP == kPlaneThreadsSmall
+                       ? set_plane_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>(shP)
+                       : set_plane_lds<C2, R, kPlaneThreads, kPlaneMaxE>(shP);
     if (rc) return rc;
   }
   const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
@@ -690,9 +746,16 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     if (plane) {
       dim3 gP(nz, cur, nk);
       prof_begin(st, &ev);
-      hipLaunchKernelGGL((k_plane_potential<C2, R>), gP, dim3(kPlaneThreads), shP, st, T1, Vt,
-                         vslot, p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
-                         tw_of<C2>(p, 1), p->rad[0], p->rad[1], fny, fnxa, ncol, nxa, cur);
+      if (ntP == kPlaneThreadsSmall)
+        hipLaunchKernelGGL((k_plane_potential<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>), gP,
+                           dim3(kPlaneThreadsSmall), shP, st, T1, Vt, vslot, p->d_col_ix,
+                           p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), p->rad[0],
+                           p->rad[1], fny, fnxa, ncol, nxa, cur);
+      else
+        hipLaunchKernelGGL((k_plane_potential<C2, R, kPlaneThreads, kPlaneMaxE>), gP, dim3(kPlaneThreads),
+                           shP, st, T1, Vt, vslot, p->d_col_ix, p->d_col_iy, p->d_x_list,
+                           tw_of<C2>(p, 0), tw_of<C2>(p, 1), p->rad[0], p->rad[1], fny, fnxa,
+                           ncol, nxa, cur);
       QMX_LAUNCHED();
       prof_end(st, ev, idoff + QMX_KERNEL_PLANE, cur * 2.0 * t1_elems * bc + (double)nk * p->N * br);
     } else {
