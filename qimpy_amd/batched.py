@@ -172,22 +172,27 @@ class CellJob:
         self.admitted = None
         self.result: Optional[CellResult] = None
         self._dev = None
+        # Host copies of the device tables, pinned here (by the thread that
+        # builds the job) so that admission copies them without waiting.
+        host = dict(k=np.asarray(spec.k, dtype=np.float64).reshape(-1, 3), iG=self.iG_np,
+                    GbT=np.asarray(spec.Gbasis, dtype=np.float64).T, n=self.n,
+                    V=np.asarray(spec.V, dtype=np.float64).reshape(-1),
+                    pos=np.asarray(spec.positions, dtype=np.float64),
+                    D=np.asarray(spec.D, dtype=np.float64))
+        self._host = {name: torch.from_numpy(np.ascontiguousarray(a)) for name, a in host.items()}
+        if torch.cuda.is_available():
+            self._host = {name: t.pin_memory() for name, t in self._host.items()}
 
     def device_tables(self) -> dict:
         """iG, k, KE and the potential on the device (built once)."""
         if self._dev is None:
             dev = rc.device
-            k = _to_device(np.asarray(self.spec.k, dtype=np.float64).reshape(-1, 3))
-            iG = _to_device(self.iG_np)
-            Gk = (iG.to(torch.float64) + k[:, None, :]) @ _to_device(
-                np.asarray(self.spec.Gbasis, dtype=np.float64).T)
+            h = {name: t.to(dev, non_blocking=True) for name, t in self._host.items()}
+            Gk = (h["iG"].to(torch.float64) + h["k"][:, None, :]) @ h["GbT"]
             ke = 0.5 * Gk.square().sum(-1)
-            n_dev = _to_device(self.n)
-            ke[torch.arange(self.n_tot, device=dev)[None, :] >= n_dev[:, None]] = 0.0
-            V = _to_device(np.asarray(self.spec.V, dtype=np.float64).reshape(-1))
-            self._dev = dict(k=k, iG=iG, ke=ke, V=V, n=n_dev,
-                             pos=_to_device(np.asarray(self.spec.positions, dtype=np.float64)),
-                             D=_to_device(np.asarray(self.spec.D, dtype=np.float64)))
+            ke[torch.arange(self.n_tot, device=dev)[None, :] >= h["n"][:, None]] = 0.0
+            self._dev = dict(k=h["k"], iG=h["iG"], ke=ke, V=h["V"], n=h["n"], pos=h["pos"],
+                             D=h["D"])
         return self._dev
 
     def release(self) -> None:

@@ -21,6 +21,8 @@
 #include <vector>
 
 #include "qmx_common.h"
+#include <algorithm>
+
 #include "qmx_fft_core.h"
 
 namespace qmx {
@@ -939,19 +941,26 @@ int build_tables(const qmx_geom* geom, GeomTables* t) {
     QMX_CHECK_ARG(factorize(geom->shape[a], &rad),
                   "qmx_geom_tables: box sizes must be products of 2, 3 and 5");
   }
-  // Columns and planes touched by any k:
+  // Columns and planes touched by any k. Sphere points come in increasing
+  // box order, so one column's points are consecutive: the column index is
+  // recomputed only where a point leaves the current column.
   std::vector<char> col_seen((size_t)n0 * n1, 0), x_seen(n0, 0);
   for (int k = 0; k < nk; ++k) {
     const int nkk = geom->n[k];
     QMX_CHECK_ARG(nkk >= 0 && nkk <= n_tot, "qmx_geom_tables: n[k] outside 0..n_tot");
     const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
+    int64_t prev = -1, col_end = -1;
     for (int g = 0; g < nkk; ++g) {
       const int64_t idx = fi[g];
-      QMX_CHECK_ARG(idx >= 0 && idx < N && (g == 0 || idx > fi[g - 1]),
+      QMX_CHECK_ARG(idx > prev && idx < N,
                     "qmx_geom_tables: fft_index must be increasing box indices within each k");
-      const int64_t col = idx / n2;
-      col_seen[col] = 1;
-      x_seen[col / n1] = 1;
+      prev = idx;
+      if (idx >= col_end) {
+        const int col = (int)(idx / n2);
+        col_end = (int64_t)(col + 1) * n2;
+        col_seen[col] = 1;
+        x_seen[col / n1] = 1;
+      }
     }
   }
   std::vector<int> col_of((size_t)n0 * n1, -1);
@@ -974,18 +983,27 @@ int build_tables(const qmx_geom* geom, GeomTables* t) {
   const int ncol = (int)t->col_iy.size();
   t->cstart.assign((size_t)nk * (ncol + 1), 0);
   t->t1idx.assign((size_t)nk * n_tot, 0);
+  std::vector<int> counts(ncol);
   for (int k = 0; k < nk; ++k) {
     const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
     int* cs = t->cstart.data() + (size_t)k * (ncol + 1);
-    std::vector<int> counts(ncol, 0);
+    int* t1 = t->t1idx.data() + (size_t)k * n_tot;
+    std::fill(counts.begin(), counts.end(), 0);
+    int64_t col_begin = 0, col_end = -1;
+    int c = 0;
     for (int g = 0; g < geom->n[k]; ++g) {
       const int64_t idx = fi[g];
-      const int c = col_of[idx / n2];
+      if (idx >= col_end) {
+        const int64_t col = idx / n2;
+        col_begin = col * n2;
+        col_end = col_begin + n2;
+        c = col_of[col];
+      }
       counts[c]++;
-      t->t1idx[(size_t)k * n_tot + g] = c * n2 + (int)(idx % n2);
+      t1[g] = c * n2 + (int)(idx - col_begin);
     }
     cs[0] = 0;
-    for (int c = 0; c < ncol; ++c) cs[c + 1] = cs[c] + counts[c];
+    for (int cc = 0; cc < ncol; ++cc) cs[cc + 1] = cs[cc] + counts[cc];
   }
   return QMX_OK;
 }
