@@ -74,6 +74,66 @@ __device__ __forceinline__ void dft(C2 (&v)[R]) {
   if constexpr (R == 5) dft5<C2, S>(v[0], v[1], v[2], v[3], v[4]);
 }
 
+// Composite radix R = R1 R2 (R1, R2 in 2..5) in registers: with n = R2 n1 + n2
+// and k = k1 + R1 k2,
+//   X[k] = sum_n2 W_R^(n2 k1) W_R2^(n2 k2) sum_n1 v[R2 n1 + n2] W_R1^(n1 k1),
+// R2 DFTs of size R1, the internal twiddles W_R^m = tw[m twR] (tw the exact
+// n-point table, twR = n / R; conjugated for the inverse), R1 DFTs of size R2.
+template <typename C2, int R1, int R2, int S>
+__device__ __forceinline__ void dft_comp(C2 (&v)[R1 * R2], const C2* __restrict__ tw, int twR) {
+  constexpr int R = R1 * R2;
+  C2 y[R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    C2 a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+    dft<C2, R1, S>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int m = (n2 * k1) % R;
+      if (m != 0) {
+        C2 w = tw[m * twR];
+        if constexpr (S > 0) w.y = -w.y;
+        a[k1] = cmul(a[k1], w);
+      }
+      y[n2 * R1 + k1] = a[k1];
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    C2 b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2 * R1 + k1];
+    dft<C2, R2, S>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
+// Radices 2..5 directly, composite radices 6, 8, 9, 10, 12, 15, 16 through
+// dft_comp with the twiddle table.
+template <typename C2, int R, int S>
+__device__ __forceinline__ void dft_any(C2 (&v)[R], const C2* __restrict__ tw, int n) {
+  if constexpr (R <= 5) {
+    dft<C2, R, S>(v);
+  } else if constexpr (R == 6) {
+    dft_comp<C2, 2, 3, S>(v, tw, n / R);
+  } else if constexpr (R == 8) {
+    dft_comp<C2, 4, 2, S>(v, tw, n / R);
+  } else if constexpr (R == 9) {
+    dft_comp<C2, 3, 3, S>(v, tw, n / R);
+  } else if constexpr (R == 10) {
+    dft_comp<C2, 2, 5, S>(v, tw, n / R);
+  } else if constexpr (R == 12) {
+    dft_comp<C2, 4, 3, S>(v, tw, n / R);
+  } else if constexpr (R == 15) {
+    dft_comp<C2, 3, 5, S>(v, tw, n / R);
+  } else if constexpr (R == 16) {
+    dft_comp<C2, 4, 4, S>(v, tw, n / R);
+  }
+}
+
 // One Stockham pass over the M lines (division by M via fastdiv `fm`).
 template <typename C2, int R, int S>
 __device__ __forceinline__ void stockham_pass(const C2* __restrict__ a, C2* __restrict__ b,
@@ -221,7 +281,7 @@ __device__ __forceinline__ void ip_pass(C2* buf, const C2* __restrict__ tw, int 
         if constexpr (S > 0) w.y = -w.y;
         v[it][r] = cmul(v[it][r], w);
       }
-      dft<C2, R, S>(v[it]);
+      dft_any<C2, R, S>(v[it], tw, n);
     }
   }
   __syncthreads();
@@ -270,8 +330,94 @@ __device__ __forceinline__ void fft_ip(C2* buf, const C2* tw, const Radices& rad
         ip_pass<C2, 5, S, NT, MAXE / 5, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],
                                                               rad.st[ip], nl, fnl, ps, ls, lines);
         break;
+#define QMX_IP_CASE(RR)                                                                   \
+  case RR:                                                                                \
+    if constexpr (MAXE >= RR)                                                             \
+      ip_pass<C2, RR, S, NT, MAXE / RR, LINE_FAST, INDIRECT>(buf, tw, rad.n, rad.ns[ip],  \
+                                                             rad.st[ip], nl, fnl, ps, ls, \
+                                                             lines);                      \
+    break;
+      QMX_IP_CASE(6)
+      QMX_IP_CASE(8)
+      QMX_IP_CASE(9)
+      QMX_IP_CASE(10)
+      QMX_IP_CASE(12)
+      QMX_IP_CASE(15)
+      QMX_IP_CASE(16)
+#undef QMX_IP_CASE
     }
   }
+}
+
+// Radices of a length for the in-place passes of nl lines with NT threads
+// and MAXE values per thread: the prime-power factors of `in` grouped into
+// composite radices of at most 16 (6, 8, 9, 10, 12, 15, 16), the grouping with
+// the fewest passes whose butterflies fit the register budget. Falls back
+// to `in` when no grouping fits better.
+inline Radices combine_radices(const Radices& in, int nl, int nt, int maxe) {
+  auto fits = [&](int R) {
+    if (R > maxe) return false;
+    const long total = (long)(in.n / R) * nl;
+    return (total + nt - 1) / nt <= maxe / R;
+  };
+  auto allowed = [](int R) {
+    return R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 || R == 9 || R == 10 ||
+           R == 12 || R == 15 || R == 16;
+  };
+  // Multiset of prime factors (a 4 counts as two 2s).
+  int f[32], nf = 0;
+  for (int i = 0; i < in.nr; ++i) {
+    if (in.r[i] == 4) {
+      f[nf++] = 2;
+      f[nf++] = 2;
+    } else {
+      f[nf++] = in.r[i];
+    }
+  }
+  // Greedy: repeatedly take the largest allowed product of remaining factors
+  // that fits (products of subsets of at most four factors).
+  int used[32] = {0};
+  int groups[32], ng = 0;
+  int left = nf;
+  while (left > 0) {
+    int best = 0, bestmask = 0;
+    const int idx_n = nf;
+    for (int mask = 1; mask < (1 << idx_n); ++mask) {
+      int prod = 1, cnt = 0;
+      bool ok = true;
+      for (int i = 0; i < idx_n && ok; ++i) {
+        if (!(mask >> i & 1)) continue;
+        if (used[i]) ok = false;
+        prod *= f[i];
+        ++cnt;
+        if (prod > 16 || cnt > 4) ok = false;
+      }
+      if (!ok || !allowed(prod) || !fits(prod)) continue;
+      if (prod > best) {
+        best = prod;
+        bestmask = mask;
+      }
+    }
+    if (best == 0) return in;  // some factor does not fit alone: keep the plain radices
+    for (int i = 0; i < idx_n; ++i)
+      if (bestmask >> i & 1) {
+        used[i] = 1;
+        --left;
+      }
+    groups[ng++] = best;
+  }
+  if (ng >= in.nr) return in;
+  Radices out;
+  out.n = in.n;
+  out.nr = ng;
+  int ns = 1;
+  for (int i = 0; i < ng; ++i) {
+    out.r[i] = groups[i];
+    out.ns[i] = make_fastdiv(ns);
+    out.st[i] = make_fastdiv(in.n / groups[i]);
+    ns *= groups[i];
+  }
+  return out;
 }
 
 // Largest butterfly count per thread that fft_ip needs for nl lines of the

@@ -748,16 +748,20 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     if (plane) {
       dim3 gP(nz, cur, nk);
       prof_begin(st, &ev);
+      // Composite radices where they fit the plane kernel's register budget.
+      const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
+      const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
+      const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
       if (ntP == kPlaneThreadsSmall)
         hipLaunchKernelGGL((k_plane_potential<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>), gP,
                            dim3(kPlaneThreadsSmall), shP, st, T1, Vt, vslot, p->d_col_ix,
-                           p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), p->rad[0],
-                           p->rad[1], fny, fnxa, ncol, nxa, cur);
+                           p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), rxc,
+                           ryc, fny, fnxa, ncol, nxa, cur);
       else
         hipLaunchKernelGGL((k_plane_potential<C2, R, kPlaneThreads, kPlaneMaxE>), gP, dim3(kPlaneThreads),
                            shP, st, T1, Vt, vslot, p->d_col_ix, p->d_col_iy, p->d_x_list,
-                           tw_of<C2>(p, 0), tw_of<C2>(p, 1), p->rad[0], p->rad[1], fny, fnxa,
-                           ncol, nxa, cur);
+                           tw_of<C2>(p, 0), tw_of<C2>(p, 1), rxc, ryc, fny, fnxa, ncol, nxa,
+                           cur);
       QMX_LAUNCHED();
       prof_end(st, ev, idoff + QMX_KERNEL_PLANE, cur * 2.0 * t1_elems * bc + (double)nk * p->N * br);
     } else {

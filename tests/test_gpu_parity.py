@@ -104,6 +104,31 @@ def test_apply_potential(which, dtype, small_el, cfg0_el):
     assert rel(kin, ref + C * ob.KE[:, None, :]) < (1e-12 if dtype == C128 else 2e-5)
 
 
+def _cell_with_box(g):
+    i = 0
+    while True:
+        spec = systems.make_cfg2_cell(i, cfg=3)
+        if int(spec.shape[0]) == g and len(np.asarray(spec.k).reshape(-1, 3)) <= 4:
+            return spec
+        i += 1
+
+
+@pytest.mark.parametrize("g", [24, 30, 40, 48, 60])
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_apply_potential_boxes(gpu, g, dtype):
+    """The fused plane pass on every box size of the batched workloads (the
+    composite-radix transforms 12x2, 15x2, 10x4, 16x3, 15x4 and their plain
+    fallbacks) against the oracle's convolution by FFT."""
+    spec = _cell_with_box(g)
+    print(f"[apply_potential_boxes] {g}^3 {dtype}", flush=True)
+    ob = OBasis(spec)
+    el = electrons_from_spec(spec)
+    C = rand_coeff(ob, 3, seed=g)
+    ref = o_apply_potential(ob, spec.V, C)
+    got = el.basis.apply_potential(el.V_ks, to_wf(el.basis, C, dtype)).coeff[0, :, :, 0]
+    assert rel(got, ref) < (1e-12 if dtype == C128 else 2e-5)
+
+
 def test_apply_local_blocking_inplace_accumulate(cfg0_el):
     """Tiny workspace (many band blocks), out == C, and accumulate."""
     spec, ob, el = cfg0_el
