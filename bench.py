@@ -238,8 +238,13 @@ def main():
         names[i]: {"launches": c, "avg_us": round(m / c * 1e3, 2),
                    "GBps": round(b / c / (m / c * 1e-3) / 1e9, 1),
                    "share_of_stream": round(m * 1e-3 / wall, 3)}
-        for i, (m, b, c) in prof.items()
+        for i, (m, b, c) in prof.items() if not names[i].startswith("local_apply")
     }
+    # The whole local apply (all its launches, V reorder included) under the
+    # dense-transform byte convention b(2G + 4N) + b_V N / B per band:
+    apply = [prof[i] for i in prof if names[i].startswith("local_apply")]
+    apply_ms = sum(a[0] for a in apply)
+    apply_gbs = sum(a[1] for a in apply) / (apply_ms * 1e-3) / 1e9 if apply_ms else None
     out = {
         "metric": "served band solves/s with p50/p99 latency (mixed c64/FP64 vs plain FP64, "
                   "<=1e-6 Ha vs FP64)",
@@ -274,6 +279,12 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_launch": round(by / cnt),
+            "local_apply_dense_convention": None if apply_gbs is None else {
+                "bytes_per_band": "b(2G + 4N) + b_V N / B",
+                "achieved": round(apply_gbs, 1), "frac": round(apply_gbs / HBM_PEAK_GBS, 4),
+                "calls": sum(a[2] for a in apply),
+                "avg_call_us": round(apply_ms / sum(a[2] for a in apply) * 1e3, 2),
+            },
         },
         "kernels": kernels,
     }

@@ -288,6 +288,7 @@ enum {
   QMX_KERNEL_GEMM = 6,
   QMX_KERNEL_PLANE = 7,   /* fused plane pass: inverse y, x, * V, forward x, y */
   QMX_KERNEL_VZMAJOR = 8, /* potential boxes to z-major order */
+  QMX_KERNEL_APPLY = 9,   /* whole local apply, dense-transform byte convention */
   QMX_KERNEL_C128 = 16,   /* added to the id of a complex128 launch */
   QMX_N_KERNEL_IDS = 32
 };

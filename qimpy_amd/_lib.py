@@ -180,6 +180,7 @@ _BASE_NAMES = {
     0: "k_sphere_to_cols", 1: "k_cols_to_planes", 2: "k_xpass_potential",
     3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
     7: "k_plane_potential", 8: "k_potential_zmajor",
+    9: "local_apply",  # the whole apply (a span of launches, not one kernel)
 }
 #: Profile ids: the kernel id for complex64 launches, + 16 for complex128.
 KERNEL_NAMES = {**{i: f"{n}<c64>" for i, n in _BASE_NAMES.items()},

@@ -707,7 +707,8 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   const double n_sphere = (double)(p->h_nsum[k0 + nk] - p->h_nsum[k0]);
   const double bc = (double)cs, br = (double)sizeof(R);
   const double t1_elems = (double)nk * ncol * nz, t2_elems = (double)nk * nxa * ny * nz;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr, ev_apply = nullptr;
+  prof_begin(st, &ev_apply);
   const R* Vt = nullptr;
   if (plane) {
     // z-major copy of every potential box at the start of the workspace
@@ -811,6 +812,12 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
              cur * (t1_elems * bc + n_sphere * bc * (1 + (ke ? 1 : 0) + (accumulate ? 1 : 0)) +
                     (ke ? n_sphere * br : 0.0)));
   }
+  // The whole apply under the dense-transform convention: per band a sphere
+  // read and write plus one box read and write for each of the two 3D
+  // transforms, and the potential read once per k-point.
+  prof_end(st, ev_apply, idoff + QMX_KERNEL_APPLY,
+           (double)nb * (2.0 * n_sphere * bc + 4.0 * nk * (double)p->N * bc) +
+               (double)nk * p->N * br);
   return QMX_OK;
 }
 
