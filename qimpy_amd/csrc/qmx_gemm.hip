@@ -14,6 +14,8 @@
 // into the complex128 result, so complex64 inputs are accumulated in FP32
 // only within a chunk.
 #include <algorithm>
+#include <string>
+#include <type_traits>
 
 #include "qmx_common.h"
 
@@ -168,6 +170,195 @@ __global__ void __launch_bounds__(kDotThreads)
       }
 }
 
+// Output blocks fitted to the matrix: a workgroup of eight waves computes a
+// (16 bm16) x (16 bn16) block of one batch over one basis chunk, so the
+// shapes of the band solve (144 x 72, 72 x 72, 64 x 32, 32 x 32) run without
+// the padding of a fixed 64 x 64 tile. The block's 16 x 16 MFMA tiles are
+// dealt to the waves round-robin (at most kMaxT each). One LDS slab holds the
+// block's A rows followed by its B rows (real and imaginary planes, kSlab
+// basis points per row); the next slab is loaded into registers while the
+// MFMAs run. The four MFMA steps of a slab give lane (i, k) the basis points
+// 4k .. 4k + 3 of its row, so each operand of a tile is one 16-byte LDS read
+// per slab (two for FP64); the order of the basis sum does not matter.
+constexpr int kBlkThreads = 512;
+constexpr int kBlkRows16 = 14;  // bm16 + bn16 at most (224 staged rows)
+constexpr int kBlkTiles = 48;   // bm16 * bn16 at most
+constexpr int kBlkQuads = (kBlkRows16 * 16 * (kSlab / 4) + kBlkThreads - 1) / kBlkThreads;
+
+// LDS row stride in elements: 16-byte aligned rows whose 16 lanes of one
+// read start on distinct groups of four banks.
+template <typename T>
+constexpr int blk_ld() { return sizeof(T) == 4 ? 20 : 18; }
+
+template <typename T>
+size_t blk_lds_bytes(int rows16) { return (size_t)rows16 * 16 * blk_ld<T>() * 2 * sizeof(T); }
+
+// Four consecutive LDS elements (one ds_read_b128 for FP32, two for FP64).
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+typedef double lds_d2 __attribute__((ext_vector_type(2)));
+struct Quad32 { float v[4]; };
+struct Quad64 { double v[4]; };
+__device__ inline Quad32 lds_quad(const float* p) {
+  const lds_f4 x = *reinterpret_cast<const lds_f4*>(p);
+  return Quad32{{x[0], x[1], x[2], x[3]}};
+}
+__device__ inline Quad64 lds_quad(const double* p) {
+  const lds_d2 x = reinterpret_cast<const lds_d2*>(p)[0], y = reinterpret_cast<const lds_d2*>(p)[1];
+  return Quad64{{x[0], x[1], y[0], y[1]}};
+}
+__device__ inline void lds_store4(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<lds_f4*>(p) = lds_f4{a, b, c, d};
+}
+__device__ inline void lds_store4(double* p, double a, double b, double c, double d) {
+  reinterpret_cast<lds_d2*>(p)[0] = lds_d2{a, b};
+  reinterpret_cast<lds_d2*>(p)[1] = lds_d2{c, d};
+}
+
+template <typename T, int kMaxT>
+__global__ void __launch_bounds__(kBlkThreads)
+    k_dot_block(const typename Cplx<T>::T* __restrict__ A, int64_t lda, int64_t sa,
+                const typename Cplx<T>::T* __restrict__ B, int64_t ldb, int64_t sb, int M, int N,
+                int64_t G, int64_t chunk_len, int bm16, int bn16, int mblocks, int nblocks,
+                int nchunks, typename Cplx<T>::T* __restrict__ part) {
+  using C2 = typename Cplx<T>::T;
+  using MF = Mfma<T>;
+  constexpr int LD = blk_ld<T>();
+  constexpr int kWaves = kBlkThreads / 64;
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  const int rowsA = bm16 * 16, rows = rowsA + bn16 * 16, nquads = rows * (kSlab / 4);
+  T* sR = reinterpret_cast<T*>(blk_smem);
+  T* sI = sR + rows * LD;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int bt = blockIdx.x;
+  const int bj = bt % nblocks;
+  bt /= nblocks;
+  const int bi = bt % mblocks;
+  const int b = bt / mblocks;
+  const int chunk = blockIdx.y;
+  const int m0 = bi * bm16 * 16, n0 = bj * bn16 * 16;
+  const int64_t g_begin = (int64_t)chunk * chunk_len;
+  const int64_t g_end = std::min<int64_t>(G, g_begin + chunk_len);
+  const C2* Ab = A + (int64_t)b * sa;
+  const C2* Bb = B + (int64_t)b * sb;
+  // Staging: quad q -> (staged row q / 4, 4 consecutive g at (q % 4) * 4).
+  // Loads are unconditional and in bounds (rows past M or N read row 0 of A,
+  // basis points past the chunk its last point) and are masked to zero only
+  // when stored, so a slab's loads are issued back to back and waited for
+  // after the MFMAs of the previous slab.
+  const C2* src[kBlkQuads];
+  int sofs[kBlkQuads], gq[kBlkQuads];
+  bool live[kBlkQuads];
+#pragma unroll
+  for (int u = 0; u < kBlkQuads; ++u) {
+    const int q = tid + u * kBlkThreads;
+    const int row = q >> 2;
+    src[u] = Ab;
+    live[u] = false;
+    gq[u] = (q & 3) * 4;
+    sofs[u] = row * LD + gq[u];
+    if (q < nquads) {
+      if (row < rowsA) {
+        if (m0 + row < M) {
+          src[u] = Ab + (int64_t)(m0 + row) * lda;
+          live[u] = true;
+        }
+      } else if (n0 + row - rowsA < N) {
+        src[u] = Bb + (int64_t)(n0 + row - rowsA) * ldb;
+        live[u] = true;
+      }
+    }
+  }
+  C2 reg[kBlkQuads][4];
+  auto load = [&](int64_t g0) {
+#pragma unroll
+    for (int u = 0; u < kBlkQuads; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        reg[u][e] = src[u][std::min<int64_t>(g0 + gq[u] + e, g_end - 1)];
+  };
+  auto store = [&](int64_t g0) {
+#pragma unroll
+    for (int u = 0; u < kBlkQuads; ++u) {
+      if (tid + u * kBlkThreads < nquads) {
+        T r[4], i[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool ok = live[u] && g0 + gq[u] + e < g_end;
+          r[e] = ok ? reg[u][e].x : T(0);
+          i[e] = ok ? reg[u][e].y : T(0);
+        }
+        lds_store4(sR + sofs[u], r[0], r[1], r[2], r[3]);
+        lds_store4(sI + sofs[u], i[0], i[1], i[2], i[3]);
+      }
+    }
+  };
+  // This wave's tiles: t = wave + kWaves s. A slot past the block's last
+  // tile (the host picks kMaxT = ceil(tiles / kWaves), so only a wave's last
+  // slot can be one) points at that tile, skips its MFMAs and is not stored.
+  const int ntiles = bm16 * bn16;
+  const int li = lane & 15, lk = lane >> 4;
+  int offA[kMaxT], offB[kMaxT];
+  typename MF::Acc re[kMaxT], im[kMaxT];
+#pragma unroll
+  for (int s = 0; s < kMaxT; ++s) {
+    const int t = std::min(wave + kWaves * s, ntiles - 1);
+    const int ti = t / bn16, tj = t - ti * bn16;
+    offA[s] = (16 * ti + li) * LD + 4 * lk;
+    offB[s] = (rowsA + 16 * tj + li) * LD + 4 * lk;
+    re[s] = MF::zero();
+    im[s] = MF::zero();
+  }
+  const bool last_live = wave + kWaves * (kMaxT - 1) < ntiles;  // uniform per wave
+  if (g_begin < g_end) load(g_begin);
+  for (int64_t g0 = g_begin; g0 < g_end; g0 += kSlab) {
+    store(g0);
+    // The next slab's loads go out before the barrier, which keeps the
+    // compiler from sinking them below the MFMAs they should overlap
+    // (clamped past the end of the chunk).
+    load(g0 + kSlab);
+    __syncthreads();
+    // Operands of tile s + 1 are read from LDS before the MFMAs of tile s.
+    auto ar = lds_quad(sR + offA[0]), ai = lds_quad(sI + offA[0]);
+    auto br = lds_quad(sR + offB[0]), bi = lds_quad(sI + offB[0]);
+#pragma unroll
+    for (int s = 0; s < kMaxT; ++s) {
+      const int sn = s + 1 < kMaxT ? s + 1 : s;
+      const auto ar1 = lds_quad(sR + offA[sn]), ai1 = lds_quad(sI + offA[sn]);
+      const auto br1 = lds_quad(sR + offB[sn]), bi1 = lds_quad(sI + offB[sn]);
+      if (s + 1 < kMaxT || last_live) {  // a repeated last slot skips its MFMAs
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          re[s] = MF::mac(ar.v[k], br.v[k], re[s]);
+          im[s] = MF::mac(ar.v[k], bi.v[k], im[s]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          re[s] = MF::mac(ai.v[k], bi.v[k], re[s]);
+          im[s] = MF::mac(-ai.v[k], br.v[k], im[s]);
+        }
+      }
+      ar = ar1;
+      ai = ai1;
+      br = br1;
+      bi = bi1;
+    }
+    __syncthreads();
+  }
+  C2* pb = part + ((int64_t)b * nchunks + chunk) * M * N;
+#pragma unroll
+  for (int s = 0; s < kMaxT; ++s) {
+    const int t = wave + kWaves * s;
+    if (t < ntiles) {
+      const int ti = t / bn16, tj = t - ti * bn16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * ti + MF::row(lk, r), n = n0 + 16 * tj + li;
+        if (m < M && n < N) pb[(int64_t)m * N + n] = C2{re[s][r], im[s][r]};
+      }
+    }
+  }
+}
+
 // out[b][m][n] (complex128, row stride ldc, batch stride sc) = sum over chunks.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -194,16 +385,47 @@ __global__ void __launch_bounds__(256)
 }
 
 struct DotPlan {
-  int mtiles, ntiles, nchunks;
+  bool block;          // k_dot_block (fitted blocks) or k_dot_partial (64 x 64 tiles)
+  int mtiles, ntiles;  // output blocks or tiles along M and N
+  int bm16, bn16;      // block size in 16-row MFMA tiles (block kernel)
+  int nchunks;
   int64_t chunk_len;
 };
 
+constexpr int kBlkTarget = 1024;   // workgroups the block kernel aims for
+constexpr int kTileTarget = 2048;  // workgroups the 64 x 64 kernel aims for
+
+// Modelled cost of an output tiling, in units of one 16-row slab load per
+// basis point: each 16 x 16 complex MFMA tile costs about 13 of those on the
+// matrix cores (4 real MFMAs against the HBM share of one CU).
+inline int64_t tiling_cost(int64_t tiles16, int64_t rows16) { return 13 * tiles16 + rows16; }
+
 DotPlan dot_plan(int64_t batch, int64_t M, int64_t N, int64_t G) {
-  DotPlan p;
-  p.mtiles = (int)((M + kTile - 1) / kTile);
-  p.ntiles = (int)((N + kTile - 1) / kTile);
+  DotPlan p{};
+  const int64_t M16 = (M + 15) / 16, N16 = (N + 15) / 16;
+  int64_t best = -1;
+  for (int bm = 1; bm <= std::min<int64_t>(M16, kBlkRows16 - 1); ++bm)
+    for (int bn = 1; bn <= std::min<int64_t>(N16, kBlkRows16 - bm); ++bn) {
+      if (bm * bn > kBlkTiles) continue;
+      const int64_t mb = (M16 + bm - 1) / bm, nb = (N16 + bn - 1) / bn;
+      const int64_t c = tiling_cost(mb * nb * bm * bn, mb * nb * (bm + bn));
+      if (best < 0 || c < best) {
+        best = c;
+        p.bm16 = bm;
+        p.bn16 = bn;
+        p.mtiles = (int)mb;
+        p.ntiles = (int)nb;
+      }
+    }
+  const int64_t mt = (M + kTile - 1) / kTile, nt = (N + kTile - 1) / kTile;
+  p.block = best <= tiling_cost(mt * nt * 16, mt * nt * 8);
+  if (!p.block) {
+    p.mtiles = (int)mt;
+    p.ntiles = (int)nt;
+  }
   const int64_t tiles = std::max<int64_t>(1, batch * p.mtiles * p.ntiles);
-  const int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+  const int64_t target = p.block ? kBlkTarget : kTileTarget;
+  const int64_t want = std::max<int64_t>(1, (target + tiles - 1) / tiles);
   const int64_t most = std::max<int64_t>(1, (G + kMinChunk - 1) / kMinChunk);
   int64_t nch = std::min<int64_t>(std::min<int64_t>(want, most), 65535);
   int64_t len = (G + nch - 1) / nch;
@@ -212,6 +434,29 @@ DotPlan dot_plan(int64_t batch, int64_t M, int64_t N, int64_t G) {
   p.chunk_len = len;
   p.nchunks = (int)std::max<int64_t>(1, (G + len - 1) / len);
   return p;
+}
+
+template <typename T, int kMaxT>
+int launch_block(const DotPlan& p, dim3 grid, hipStream_t st, const void* A, int64_t lda,
+                 int64_t sa, const void* B, int64_t ldb, int64_t sb, int M, int N, int64_t G,
+                 void* ws) {
+  using C2 = typename Cplx<T>::T;
+  static bool lds_set = false;  // the same value from any thread
+  if (!lds_set) {
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_dot_block<T, kMaxT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)blk_lds_bytes<T>(kBlkRows16));
+    if (e != hipSuccess) {
+      set_error(std::string("qmx_dot: ") + hipGetErrorString(e));
+      return QMX_ERR_HIP;
+    }
+    lds_set = true;
+  }
+  hipLaunchKernelGGL((k_dot_block<T, kMaxT>), grid, dim3(kBlkThreads),
+                     blk_lds_bytes<T>(p.bm16 + p.bn16), st, static_cast<const C2*>(A), lda, sa,
+                     static_cast<const C2*>(B), ldb, sb, M, N, G, p.chunk_len, p.bm16, p.bn16,
+                     p.mtiles, p.ntiles, p.nchunks, static_cast<C2*>(ws));
+  return QMX_OK;
 }
 
 template <typename T>
@@ -226,10 +471,27 @@ int dot_impl(const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
   hipEvent_t ev = nullptr;
   prof_begin(st, &ev);
   dim3 grid((unsigned)(batch * p.mtiles * p.ntiles), (unsigned)p.nchunks);
-  hipLaunchKernelGGL((k_dot_partial<T>), grid, dim3(kDotThreads), 0, st,
-                     static_cast<const C2*>(A), lda, sa, static_cast<const C2*>(B), ldb, sb,
-                     (int)M, (int)N, G, p.chunk_len, p.mtiles, p.ntiles, p.nchunks,
-                     static_cast<C2*>(ws));
+  if (p.block) {
+    auto go = [&](auto kmax) {
+      return launch_block<T, decltype(kmax)::value>(p, grid, st, A, lda, sa, B, ldb, sb, (int)M,
+                                                    (int)N, G, ws);
+    };
+    int rc;
+    switch ((p.bm16 * p.bn16 + kBlkThreads / 64 - 1) / (kBlkThreads / 64)) {
+      case 1: rc = go(std::integral_constant<int, 1>{}); break;
+      case 2: rc = go(std::integral_constant<int, 2>{}); break;
+      case 3: rc = go(std::integral_constant<int, 3>{}); break;
+      case 4: rc = go(std::integral_constant<int, 4>{}); break;
+      case 5: rc = go(std::integral_constant<int, 5>{}); break;
+      default: rc = go(std::integral_constant<int, 6>{}); break;
+    }
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL((k_dot_partial<T>), grid, dim3(kDotThreads), 0, st,
+                       static_cast<const C2*>(A), lda, sa, static_cast<const C2*>(B), ldb, sb,
+                       (int)M, (int)N, G, p.chunk_len, p.mtiles, p.ntiles, p.nchunks,
+                       static_cast<C2*>(ws));
+  }
   QMX_LAUNCHED();
   const int64_t total = batch * M * N;
   hipLaunchKernelGGL((k_dot_reduce<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
