@@ -58,8 +58,8 @@ constexpr int kBtRows = kXMaxN / 64;      // rows per lane in the back-transform
 constexpr int kBtR = 36;                  // rows per wave, row-block back-transformation
 constexpr int kStage = 8;                 // steps per register-staged block in phase 3
 constexpr double kEps = 2.220446049250313e-16;
-constexpr double kClusterRtol = 1e-6;  // final MGS inside clusters (T scaled to norm ~1)
-constexpr double kTightTol = 1e-12;    // numerically degenerate eigenvalues (scaled)
+constexpr double kClusterRtol = 1e-6;  // final MGS inside clusters (relative to the block norm)
+constexpr double kTightTol = 1e-12;    // numerically degenerate eigenvalues (same)
 
 // Work area of one matrix: W (n x n complex: the working copy, then the
 // reflectors in its rows), X, U1, U2, U3 (n x n_want real each, inverse
@@ -214,6 +214,7 @@ struct XLds {
   double* e;
   double* e2;
   double* lam;     // scaled eigenvalues
+  double* lsc;     // scaled norm of the diagonal block(s) each eigenvalue lies in
   double* red;     // [8]
   int* ctl;        // [4]
   int* pos;        // [n] position of each eigenvector in its tight group
@@ -229,25 +230,27 @@ __device__ __forceinline__ XLds x_lds(unsigned char* base, int n) {
   L.e = L.d + n;
   L.e2 = L.e + n;
   L.lam = L.e2 + n;
-  L.red = L.lam + n;
+  L.lsc = L.lam + n;
+  L.red = L.lsc + n;
   L.ctl = reinterpret_cast<int*>(L.red + 8);
   L.pos = L.ctl + 4;
   return L;
 }
 
 __host__ __device__ inline size_t x_lds_bytes(int64_t n) {
-  return (size_t)(3 * n + kXThreads) * 16 + (size_t)(4 * n + 8) * 8 + (size_t)(4 + n) * sizeof(int);
+  return (size_t)(3 * n + kXThreads) * 16 + (size_t)(5 * n + 8) * 8 + (size_t)(4 + n) * sizeof(int);
 }
 
 // One inverse-iteration solve for eigenvalue lj of the scaled T (d, e in
 // LDS): b = scale * Xc (the previous iterate), factorisation of T - lj I
 // with partial pivoting carrying b along (y overwrites Xc, the factors go to
 // u1 = 1 / pivot, u2, u3), then back substitution x_i = (y_i - u2_i x_{i+1} -
-// u3_i x_{i+2}) u1_i into Xc. Columns are strided by nw (one lane per
-// column); loads and stores are staged kStage rows at a time. Returns
-// 1 / |x|.
-__device__ double inverse_solve(const XLds& L, int n, int nw, double lj, double scale, gd* Xc,
-                                gd* u1, gd* u2, gd* u3) {
+// u3_i x_{i+2}) u1_i into Xc. A pivot below pivmin (eps times the norm of
+// the eigenvalue's block of T) is replaced by +-pivmin. Columns are strided
+// by nw (one lane per column); loads and stores are staged kStage rows at a
+// time. Returns 1 / |x|.
+__device__ double inverse_solve(const XLds& L, int n, int nw, double lj, double pivmin,
+                                double scale, gd* Xc, gd* u1, gd* u2, gd* u3) {
   double a = L.d[0] - lj, c = n > 1 ? L.e[0] : 0.0;
   double yk = Xc[0] * scale;
   for (int k0 = 0; k0 < n - 1; k0 += kStage) {
@@ -264,7 +267,7 @@ __device__ double inverse_solve(const XLds& L, int n, int nw, double lj, double 
         const double ek = L.e[k], dk1 = L.d[k + 1] - lj;
         const double ek1 = k + 2 < n ? L.e[k + 1] : 0.0;
         if (fabs(a) >= fabs(ek)) {
-          if (fabs(a) < kEps) a = a < 0.0 ? -kEps : kEps;
+          if (fabs(a) < pivmin) a = a < 0.0 ? -pivmin : pivmin;
           const double ai = 1.0 / a, l = ek * ai;
           o1[q] = ai;
           o2[q] = c;
@@ -296,7 +299,7 @@ __device__ double inverse_solve(const XLds& L, int n, int nw, double lj, double 
       }
     }
   }
-  if (fabs(a) < kEps) a = a < 0.0 ? -kEps : kEps;
+  if (fabs(a) < pivmin) a = a < 0.0 ? -pivmin : pivmin;
   u1[(int64_t)(n - 1) * nw] = 1.0 / a;
   u2[(int64_t)(n - 1) * nw] = 0.0;
   u3[(int64_t)(n - 1) * nw] = 0.0;
@@ -760,6 +763,30 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
   }
 }
 
+// Norm scale of eigenvalue x of the scaled T (d, e, e2 in LDS): T splits
+// into diagonal blocks where a squared coupling is zero (the Sturm counts
+// treat those rows as decoupled); the result is the largest Gershgorin norm
+// among the blocks whose Gershgorin interval, widened by pad, holds x (1.0
+// if none does).
+__device__ double block_scale(const XLds& L, int n, double x, double pad) {
+  double s = 0.0, blo = 0.0, bhi = 0.0, bn = 0.0;
+  bool open = false;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? fabs(L.e[i - 1]) : 0.0) + (i + 1 < n ? fabs(L.e[i]) : 0.0);
+    const double lo = L.d[i] - r, hi = L.d[i] + r;
+    const double m = fmax(fabs(lo), fabs(hi));
+    blo = open ? fmin(blo, lo) : lo;
+    bhi = open ? fmax(bhi, hi) : hi;
+    bn = open ? fmax(bn, m) : m;
+    open = true;
+    if (i + 1 == n || L.e2[i] == 0.0) {  // the block ends at row i
+      if (x >= blo - pad && x <= bhi + pad) s = fmax(s, bn);
+      open = false;
+    }
+  }
+  return s > 0.0 ? s : 1.0;
+}
+
 // Phases 2-4 of one matrix from the tridiagonal form in its work area.
 __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsigned char* work,
                                int nw, int* info) {
@@ -817,13 +844,19 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
     L.e2[i] = q < 0x1p-900 ? 0.0 : q;
   }
   __syncthreads();
+  // Each eigenvalue to 2 eps of its magnitude or of |T| (scaled to ~1),
+  // then the norm of the diagonal block(s) of T it lies in: the scale of the
+  // pivot floor of its inverse iteration and of the tight-group and cluster
+  // tolerances below. Rows decoupled far above the wanted spectrum (the
+  // dropped directions of a Davidson step, shifted by 1e6) then neither
+  // perturb the wanted eigenvectors by eps |T| nor merge every wanted
+  // eigenvalue into one cluster.
   {
     const double pad = 4.0 * kEps + 0x1p-1000;
     const double lo0 = glo * sc - pad, hi0 = ghi * sc + pad;
-    for (int j = tid; j < n; j += kXThreads) {
-      double lo = lo0, hi = hi0;
+    auto bisect = [&](double& lo, double& hi, int j, double floor_tol) {
       for (int it = 0; it < 100; ++it) {
-        const double tol = fmax(2.0 * kEps, 2.0 * kEps * fmax(fabs(lo), fabs(hi)));
+        const double tol = fmax(floor_tol, 2.0 * kEps * fmax(fabs(lo), fabs(hi)));
         const double h = 0.25 * (hi - lo);
         const double x[3] = {lo + h, lo + 2.0 * h, hi - h};
         if (hi - lo <= tol || !(x[0] > lo && x[2] < hi)) break;
@@ -841,8 +874,14 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
           lo = x[2];
         }
       }
+    };
+    for (int j = tid; j < n; j += kXThreads) {
+      double lo = lo0, hi = hi0;
+      bisect(lo, hi, j, 2.0 * kEps);
+      const double s = block_scale(L, n, 0.5 * (lo + hi), 8.0 * kEps);
       const double lj = tnorm > 0.0 ? 0.5 * (lo + hi) : 0.0;
       L.lam[j] = lj;
+      L.lsc[j] = s;
       E[j] = lj * isc;
     }
   }
@@ -864,7 +903,10 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
   __syncthreads();
   int mypos = 0;
   if (tid < nw) {
-    while (tid - mypos > 0 && L.lam[tid - mypos] - L.lam[tid - mypos - 1] <= kTightTol) ++mypos;
+    while (tid - mypos > 0 &&
+           L.lam[tid - mypos] - L.lam[tid - mypos - 1] <=
+               fmax(kTightTol * fmax(L.lsc[tid - mypos], L.lsc[tid - mypos - 1]), 8.0 * kEps))
+      ++mypos;
     L.pos[tid] = mypos;
     atomicMax(&L.ctl[1], mypos);
   }
@@ -874,7 +916,9 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
     const bool active = tid < nw && mypos == r;
     double scale = 1.0;
     for (int it = 0; it < kXIters; ++it) {
-      if (active) scale = inverse_solve(L, n, nw, L.lam[tid], scale, X + tid, U1 + tid, U2 + tid, U3 + tid);
+      if (active)
+        scale = inverse_solve(L, n, nw, L.lam[tid], fmax(kEps * L.lsc[tid], 0x1p-900), scale,
+                              X + tid, U1 + tid, U2 + tid, U3 + tid);
       if (r > 0) {
         __threadfence_block();
         __syncthreads();
@@ -897,11 +941,10 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
   // Clusters of close eigenvalues: modified Gram-Schmidt, twice, one wave per
   // cluster, lanes along the rows.
   {
-    const double ctol = kClusterRtol;  // T is scaled to norm ~1
     int cl = 0;  // cluster counter (same on every lane)
     for (int s = 0; s < nw;) {
       int e = s + 1;
-      while (e < nw && L.lam[e] - L.lam[e - 1] < ctol) ++e;
+      while (e < nw && L.lam[e] - L.lam[e - 1] < kClusterRtol * fmax(L.lsc[e], L.lsc[e - 1])) ++e;
       if (e - s > 1 && (cl++ % kXWaves) == wave) {
         for (int pass = 0; pass < 2; ++pass) {
           for (int c = s; c < e; ++c) {
