@@ -45,7 +45,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
@@ -238,16 +238,23 @@ def _to_device(a, dtype=None) -> torch.Tensor:
 
 #: Plans of restacked buckets, destroyed at the next host synchronisation
 #: (freeing device memory waits for the whole device).
-_retired_plans: list = []
+_retired_plans: list = []  # (stamp, plan handle)
+_stamp = [0]  # advanced by every solver tick
 
 
-def flush_retired_plans() -> None:
-    while _retired_plans:
-        plan = _retired_plans.pop()
+def flush_retired_plans(upto: Optional[int] = None) -> None:
+    """Destroy the plans retired at stamps <= `upto` (all if None); the
+    caller guarantees that no queued device work still uses them."""
+    keep = []
+    for stamp, plan in _retired_plans:
+        if upto is not None and stamp > upto:
+            keep.append((stamp, plan))
+            continue
         try:
             _lib.lib().qmx_plan_destroy(plan)
         except Exception:  # pragma: no cover - interpreter shutdown
             pass
+    _retired_plans[:] = keep
 
 
 def _band_ke(X: torch.Tensor, ke: torch.Tensor) -> torch.Tensor:
@@ -332,7 +339,7 @@ class Bucket:
     def _destroy_plan(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:
-            _retired_plans.append(plan)  # destroyed at the next synchronisation
+            _retired_plans.append((_stamp[0], plan))  # destroyed once no work uses it
         self._plan = None
 
     def _restack(self, keep: list[CellJob], new: list[CellJob]) -> None:
@@ -591,26 +598,43 @@ class _Pending:
     event: "torch.cuda.Event"
 
 
+@dataclass
+class _GroupTail:
+    """What the end of a group's step left for the host: its buckets with
+    their convergence data, its finished jobs with their results, the
+    number of failed eigensolves (host copies, valid once `event` has
+    completed)."""
+    event: Optional["torch.cuda.Event"] = None
+    packs: list = field(default_factory=list)  # (bucket, host tensor)
+    pending: list = field(default_factory=list)  # (_Pending, host tensor)
+    fails: Optional[torch.Tensor] = None
+
+
 # ---------------------------------------------------------------- solver
 class BatchSolver:
     """Runs band-solve jobs in buckets, admitting new jobs as others finish.
 
-    ``submit`` queues jobs (arrival times relative to the solver's start);
-    ``tick`` performs one iteration of every bucket: residuals and one
-    device-to-host read for all buckets (convergence data and the previous
-    tick's results), retirement of converged jobs, admission of arrived jobs
-    into the freed capacity (HBM budget and per-bucket k-point limit) and one
-    Davidson step. Every launch goes to the caller's current stream: the
-    library routines behind torch.linalg and the batched GEMMs share one
-    handle (and its device workspace) per thread across streams, so buckets
-    are not spread over streams. ``run`` ticks until every submitted job is
-    done.
+    ``submit`` queues jobs (arrival times relative to the solver's start).
+    Buckets are dealt to ``groups`` groups (balanced by basis size) that
+    ``tick`` serves in turn: a tick of group g waits for g's convergence data
+    and finished results (copied to the host at the end of g's previous
+    step), retires g's converged jobs, admits arrived jobs into g's buckets
+    (HBM budget and per-bucket k-point limit) and runs one Davidson step of
+    each of g's buckets, ending with their residuals and the device-to-host
+    copy of the next convergence data. With more than one group the device
+    runs one group's step while the host works on another, which pays when
+    the host, not the device, limits the rate; each job still takes one
+    Davidson step per visit of its group and its results are those of a
+    single-group solver. Every launch goes to the caller's current stream:
+    the library routines behind torch.linalg and the batched GEMMs share one
+    handle (and its device workspace) per thread across streams. ``run``
+    ticks until every submitted job is done.
     """
 
     def __init__(self, dtype: torch.dtype = C64, *, residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
                  band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
-                 max_k_per_bucket: int = 2048):
+                 max_k_per_bucket: int = 2048, groups: int = 1):
         rc.require_gpu()
         self.dtype = dtype
         self.residual_threshold = residual_threshold
@@ -626,6 +650,10 @@ class BatchSolver:
         self.hbm_in_use = 0
         self._held: dict[int, int] = {}
         self._info: list[torch.Tensor] = []
+        self.n_groups = max(1, int(groups))
+        self._group_of: dict = {}  # bucket key -> group
+        self._tail = [_GroupTail() for _ in range(self.n_groups)]
+        self._turn = 0
         self.n_ticks = 0
         torch.cuda.synchronize()
         self._t0_event = torch.cuda.Event(enable_timing=True)
@@ -661,15 +689,34 @@ class BatchSolver:
             self.buckets[key] = b
         return b
 
-    def _admit(self, now: Optional[float]) -> dict:
-        """Take queued jobs (arrived by `now`) that fit; returns key -> jobs."""
+    def _group_loads(self) -> list[float]:
+        load = [0.0] * self.n_groups
+        for key, b in self.buckets.items():
+            if b.jobs:
+                load[self._group_of[key]] += float(b.Kb) * b.Gb * b.nb
+        return load
+
+    def _admit(self, now: Optional[float], group: int = 0) -> dict:
+        """Take queued jobs (arrived by `now`) that fit into buckets of
+        `group` (a bucket seen for the first time joins the group with the
+        least work); returns key -> jobs."""
         taken: dict = {}
         rest = []
+        load = None
         for job in self.queue:
             if now is not None and job.arrival > now:
                 rest.append(job)
                 continue
             key = (job.shape, self.band_class(job))
+            if key not in self._group_of:
+                if load is None:
+                    load = self._group_loads()
+                g = int(np.argmin(load))
+                self._group_of[key] = g
+                load[g] += float(job.K) * job.n.max() * key[1]
+            if self._group_of[key] != group:
+                rest.append(job)
+                continue
             b = self.buckets.get(key)
             k_in = (b.Kb if b else 0) + sum(j.K for j in taken.get(key, []))
             need = job.hbm_bytes(key[1], self.dtype)
@@ -701,40 +748,44 @@ class BatchSolver:
             self.results[job.job_id] = res
 
     def tick(self, now: Optional[float] = None) -> list[CellResult]:
-        """One iteration of every bucket; returns the results that became
-        available (jobs finished in the previous tick)."""
-        active = [b for b in self.buckets.values() if b.jobs]
-        # 1. convergence data of every bucket and the previous results, one
-        #    read (each copy queued on the stream that produced its data).
-        host_packs = []
-        for b in active:
-            b.residuals()
-            host_packs.append(b.pack().to("cpu", non_blocking=True))
-        prev = self.pending
-        self.pending = []
-        host_prev = [p.data.to("cpu", non_blocking=True) for p in prev]
-        fails = None
-        if self._info:
-            fails = torch.stack([(i != 0).sum() for i in self._info]).sum().to("cpu", non_blocking=True)
-            self._info = []
-        torch.cuda.current_stream().synchronize()
-        flush_retired_plans()
-        if fails is not None and int(fails) != 0:
-            raise RuntimeError(f"grouped QL eigensolver did not converge ({int(fails)} matrices)")
-        self._collect(prev, [h.numpy() for h in host_prev])
-        new_results = [self.results[p.job.job_id] for p in prev]
-        host = [h.numpy() for h in host_packs]
-        # 2. retire converged jobs (their FP64 results are read next tick);
-        #    each finish runs up to its eigenproblem before the restack.
-        retire, gens, mats = {}, [], []
-        for b, h in zip(active, host):
-            done = b.converged_jobs(h)
-            retire[id(b)] = done
+        """One iteration of the buckets of the next group; returns the
+        results that became available (that group's jobs finished in its
+        previous tick)."""
+        g = self._turn
+        self._turn = (g + 1) % self.n_groups
+        _stamp[0] += 1
+        tail, self._tail[g] = self._tail[g], _GroupTail()
+        # 1. the group's convergence data and finished results (copied at
+        #    the end of its previous step; the other groups' work stays
+        #    queued). Work queued n_groups ticks ago is complete.
+        if tail.event is not None:
+            tail.event.synchronize()
+        flush_retired_plans(_stamp[0] - self.n_groups)
+        if tail.fails is not None and int(tail.fails) != 0:
+            raise RuntimeError(f"grouped QL eigensolver did not converge ({int(tail.fails)} matrices)")
+        self._collect([p for p, _ in tail.pending], [h.numpy() for _, h in tail.pending])
+        new_results = [self.results[p.job.job_id] for p, _ in tail.pending]
+        active = [b for b, _ in tail.packs if b.jobs]
+        host = {id(b): h.numpy() for b, h in tail.packs}
+        # 2. converged jobs (their HBM is free for admission) and admissions
+        #    (host decisions only).
+        retire = {id(b): b.converged_jobs(host[id(b)]) for b in active}
+        for done in retire.values():
             for job in done:
-                self._start(b.finish_steps(job), gens, mats)
                 self.hbm_in_use -= self._held.pop(job.job_id, 0)
-        # 3. admission, restack, start of new jobs.
-        newcomers = self._admit(now)
+        newcomers = self._admit(now, g)
+        # 3. the Davidson step of every bucket that keeps its jobs starts
+        #    first (up to its first eigenproblem).
+        steps, smats = [], []
+        for b in active:
+            if not retire[id(b)] and (b.shape, b.nb) not in newcomers:
+                self._start(b.expand_steps(), steps, smats)
+        # 4. retirement (each finish runs up to its eigenproblem before the
+        #    restack), admission, restack and start of new jobs.
+        gens, mats = [], []
+        for b in active:
+            for job in retire[id(b)]:
+                self._start(b.finish_steps(job), gens, mats)
         keys = sorted(set(newcomers) | {(b.shape, b.nb) for b in active})
         changed = []
         for key in keys:
@@ -753,12 +804,27 @@ class BatchSolver:
         self.pending.extend(self._drive(gens, mats))
         for b in changed:
             b.residuals()
-        # 4. one Davidson step of every bucket.
-        steps, mats = [], []
-        for b in self.buckets.values():
-            if b.jobs:
-                self._start(b.expand_steps(), steps, mats)
-        self._drive(steps, mats)
+        # 5. the Davidson step of the changed buckets; all steps' later
+        #    eigenproblems run grouped.
+        for b in changed:
+            self._start(b.expand_steps(), steps, smats)
+        self._drive(steps, smats)
+        # 6. the group's tail: residuals of the new state and the copies
+        #    the group's next tick reads.
+        nxt = self._tail[g]
+        for key, b in self.buckets.items():
+            if b.jobs and self._group_of.get(key) == g:
+                b.residuals()
+                nxt.packs.append((b, b.pack().to("cpu", non_blocking=True)))
+        nxt.pending = [(p, p.data.to("cpu", non_blocking=True)) for p in self.pending]
+        self.pending = []
+        if self._info:
+            nxt.fails = torch.stack([(i != 0).sum() for i in self._info]).sum().to(
+                "cpu", non_blocking=True)
+            self._info = []
+        if nxt.packs or nxt.pending or nxt.fails is not None:
+            nxt.event = torch.cuda.Event()
+            nxt.event.record()
         self.n_ticks += 1
         return new_results
 
@@ -798,7 +864,7 @@ class BatchSolver:
 
     @property
     def idle(self) -> bool:
-        return (not self.queue and not self.pending
+        return (not self.queue and not self.pending and not any(t.pending for t in self._tail)
                 and not any(b.jobs for b in self.buckets.values()))
 
     def run(self) -> dict[int, CellResult]:

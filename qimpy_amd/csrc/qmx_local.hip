@@ -126,6 +126,8 @@ struct qmx_plan {
   double2* d_tw64[3];
   void* d_block;  // device block holding the integer tables
   void* h_stage;  // pinned staging copy of the block
+  size_t block_bytes;    // capacity of d_block and h_stage
+  hipEvent_t uploaded;   // recorded after the copy out of h_stage
 };
 
 namespace {
@@ -885,11 +887,81 @@ int fft3d_impl(const int32_t* shape, void* datav, int64_t batch, int direction, 
   return QMX_OK;
 }
 
+// Table blocks of destroyed plans (a device block, its pinned staging
+// copy and the event recorded after the copy between them) are kept for
+// reuse by later plans: hipFree and hipHostMalloc wait for or serialise
+// with the device, and a server creates and destroys a plan whenever a
+// bucket's membership changes. A block is reused only for a plan of at
+// most its size and at least half of it; at most kPoolMax bytes of free
+// blocks are kept (beyond that the largest ones are released).
+struct PoolBlock {
+  void* d;
+  void* h;
+  size_t bytes;
+  hipEvent_t uploaded;
+};
+static std::mutex g_pool_mu;
+static std::vector<PoolBlock> g_pool;
+static size_t g_pool_bytes = 0;
+constexpr size_t kPoolMax = size_t(1) << 30;
+constexpr size_t kPoolGrain = size_t(1) << 16;
+
+static int pool_acquire(size_t need, PoolBlock* out) {
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    int best = -1;
+    for (int i = 0; i < (int)g_pool.size(); ++i) {
+      const size_t b = g_pool[i].bytes;
+      if (b >= need && b <= 2 * need + kPoolGrain && (best < 0 || b < g_pool[best].bytes)) best = i;
+    }
+    if (best >= 0) {
+      *out = g_pool[best];
+      g_pool_bytes -= out->bytes;
+      g_pool.erase(g_pool.begin() + best);
+    } else {
+      out->d = nullptr;
+    }
+  }
+  if (out->d) {
+    // The staging copy is rewritten: its previous copy to the device must
+    // have finished (long ago in practice).
+    QMX_HIP(hipEventSynchronize(out->uploaded));
+    return QMX_OK;
+  }
+  const size_t bytes = (need + kPoolGrain - 1) / kPoolGrain * kPoolGrain;
+  *out = PoolBlock{nullptr, nullptr, bytes, nullptr};
+  QMX_HIP(hipHostMalloc(&out->h, bytes, hipHostMallocDefault));
+  QMX_HIP(hipMalloc(&out->d, bytes));
+  QMX_HIP(hipEventCreateWithFlags(&out->uploaded, hipEventDisableTiming));
+  return QMX_OK;
+}
+
+static void pool_release(const PoolBlock& blk) {
+  std::vector<PoolBlock> drop;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    g_pool.push_back(blk);
+    g_pool_bytes += blk.bytes;
+    while (g_pool_bytes > kPoolMax && !g_pool.empty()) {
+      auto it = std::max_element(g_pool.begin(), g_pool.end(),
+                                 [](const PoolBlock& a, const PoolBlock& b) { return a.bytes < b.bytes; });
+      drop.push_back(*it);
+      g_pool_bytes -= it->bytes;
+      g_pool.erase(it);
+    }
+  }
+  for (auto& b : drop) {
+    (void)hipFree(b.d);
+    (void)hipHostFree(b.h);
+    (void)hipEventDestroy(b.uploaded);
+  }
+}
+
 // The integer tables of a plan go to the device as one block: staged in
 // pinned host memory and copied asynchronously on the null stream (which
 // orders with every blocking stream), so creating a plan does not wait for
-// the work already queued on the device. The staging buffer lives as long as
-// the plan.
+// the work already queued on the device. Device work that still reads a
+// reused block was queued before the copy, so the copy follows it.
 static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>& src,
                          const std::vector<int**>& dst) {
   std::vector<size_t> off(src.size());
@@ -900,8 +972,13 @@ static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>
   }
   for (int** d : dst) *d = nullptr;
   if (total == 0) return QMX_OK;
-  QMX_HIP(hipHostMalloc(&p->h_stage, total, hipHostMallocDefault));
-  QMX_HIP(hipMalloc(&p->d_block, total));
+  PoolBlock blk;
+  const int rc = pool_acquire(total, &blk);
+  if (rc) return rc;
+  p->d_block = blk.d;
+  p->h_stage = blk.h;
+  p->block_bytes = blk.bytes;
+  p->uploaded = blk.uploaded;
   auto* h = static_cast<char*>(p->h_stage);
   auto* d = static_cast<char*>(p->d_block);
   for (size_t i = 0; i < src.size(); ++i) {
@@ -910,13 +987,13 @@ static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>
     *dst[i] = reinterpret_cast<int*>(d + off[i]);
   }
   QMX_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, nullptr));
+  QMX_HIP(hipEventRecord(p->uploaded, nullptr));
   return QMX_OK;
 }
 
 static void free_plan(qmx_plan* p) {
   if (!p) return;
-  if (p->d_block) (void)hipFree(p->d_block);
-  if (p->h_stage) (void)hipHostFree(p->h_stage);
+  if (p->d_block) pool_release(PoolBlock{p->d_block, p->h_stage, p->block_bytes, p->uploaded});
   delete[] p->h_nsum;
   delete p;
 }
