@@ -32,6 +32,7 @@ def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
     N = B.shape[-2]
     if B.shape[-1] != G:
         raise ValueError("dot128: basis lengths differ")
+    A, B = _resolved(A), _resolved(B)  # the kernel reads raw memory
     A3 = _as_batched(A.expand(batch + A.shape[-2:]))
     B3 = _as_batched(B.expand(batch + B.shape[-2:]))
     nb = A3.shape[0]
@@ -48,6 +49,16 @@ def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
               B3.stride(1), B3.stride(0), nb, M, N, G, o3.data_ptr(), o3.stride(1), o3.stride(0),
               int(accumulate), ws.data_ptr(), need, rc.stream_ptr())
     return out
+
+
+def _resolved(X: torch.Tensor) -> torch.Tensor:
+    """X with torch's lazy conjugate and negative bits applied (a copy only
+    when one is set), so its memory holds the values it represents."""
+    if X.is_conj():
+        X = X.resolve_conj()
+    if X.is_neg():
+        X = X.resolve_neg()
+    return X
 
 
 def _as_batched(X: torch.Tensor) -> torch.Tensor:

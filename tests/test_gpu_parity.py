@@ -478,3 +478,6 @@ def test_dot128_matches_fp64_reference(gpu, dtype, shape):
     assert (got.cpu() - ref).abs().max().item() <= tol * scale
     again = dot128(Ad[:, :M], Ad[:, M:], out=got.clone(), accumulate=True)
     assert (again.cpu() - 2 * ref).abs().max().item() <= 2 * tol * scale
+    lazy = dot128(Ad[:, :M].conj(), Ad[:, M:])  # torch's lazy conjugate view
+    ref_c = torch.matmul(A[:, :M], A[:, M:].transpose(-2, -1))
+    assert (lazy.cpu() - ref_c).abs().max().item() <= tol * scale
