@@ -197,10 +197,11 @@ def main():
     print(f"[bench] stream of {len(specs)} requests over {arrivals[-1]:.2f} s", flush=True)
 
     # Warm-up streams (kernels, allocator, library handles):
-    warm = [systems.make_cfg2_cell(10_000 + i, cfg=3) for i in range(12)]
+    # Warm-up steps: the same stream, untimed, in both precisions (library
+    # modules, allocator and plan-block pools reach their working sizes).
     for _ in range(args.warmup):
         for dt in (C64, C128):
-            served(warm, [0.0] * len(warm), dt)
+            served(specs, arrivals, dt)
     print("[bench] warm-up done", flush=True)
 
     reps, prof = [], None

@@ -52,7 +52,7 @@ __device__ __forceinline__ void st2(gd2* p, double2 a) {
 constexpr int kXThreads = 256;
 constexpr int kXWaves = kXThreads / 64;
 constexpr int kXMaxN = 256;
-constexpr int kXIters = 3;                // inverse-iteration solves per eigenvector
+constexpr int kXIters = 2;      // inverse-iteration solves per eigenvector
 constexpr int kBtCols = 4;                // back-transformation columns per wave pass
 constexpr int kBtRows = kXMaxN / 64;      // rows per lane in the back-transformation
 constexpr int kBtR = 36;                  // rows per wave, row-block back-transformation
@@ -152,27 +152,40 @@ __device__ __forceinline__ double start_value(int i, int j) {
 // Numbers of eigenvalues of the scaled T (diagonal d, squared off-diagonal
 // e2, e2[i] couples i and i + 1) below x[0], x[1] and x[2]: three
 // independent recurrences interleaved, d and e2 read four rows at a time.
+// Sturm counts of the scaled T at three points at once: the number of
+// sign changes of p_0 = 1, p_i = (d_i - x) p_{i-1} - e2_{i-1} p_{i-2}
+// (division free) is the number of eigenvalues below x. A zero p_{i-1}
+// matters only where the next coupling e2_{i-1} is zero (the recurrence
+// would stay at zero); there it is replaced by -piv p_{i-2}, a backward
+// perturbation of d_{i-1}. Sign changes are read from the sign bits (a zero
+// counts either way consistently); pairs (p_i, p_{i-1}) are renormalised
+// through their exponent every four rows.
 struct Sturm3 {
   double pm[3], pmm[3];
   int cnt[3];
-  __device__ __forceinline__ void step(double di, double c2, const double* x) {
+  __device__ __forceinline__ void step(double di, double c2, bool split_next, const double* x) {
     constexpr double piv = kEps * 0x1p-10;
+    double p[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) p[t] = (di - x[t]) * pm[t] - c2 * pmm[t];
+    if (__builtin_expect(split_next, false)) {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) p[t] = p[t] == 0.0 ? -piv * pm[t] : p[t];
+    }
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-      double p = (di - x[t]) * pm[t] - c2 * pmm[t];
-      p = fabs(p) < piv * fabs(pm[t]) ? -piv * pm[t] : p;
-      cnt[t] += (p < 0.0) != (pm[t] < 0.0);
+      const unsigned s = (unsigned)(__double2hiint(p[t]) ^ __double2hiint(pm[t]));
+      cnt[t] += (int)(s >> 31);
       pmm[t] = pm[t];
-      pm[t] = p;
+      pm[t] = p[t];
     }
   }
   __device__ __forceinline__ void rescale() {
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-      const double a = fmax(fabs(pm[t]), fabs(pmm[t]));
-      const double f = a > 0x1p600 ? 0x1p-600 : (a < 0x1p-600 ? 0x1p600 : 1.0);
-      pm[t] *= f;
-      pmm[t] *= f;
+      const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(pm[t]), fabs(pmm[t])));  // 0 for 0
+      pm[t] = __builtin_amdgcn_ldexp(pm[t], -e);
+      pmm[t] = __builtin_amdgcn_ldexp(pmm[t], -e);
     }
   }
 };
@@ -187,20 +200,21 @@ __device__ __forceinline__ void sturm_count3(const double* __restrict__ d,
     st.pmm[t] = 0.0;
     st.cnt[t] = 0;
   }
-  // e2[-1] = 0: row 0 has no coupling below.
+  // e2[-1] = 0: row 0 has no coupling below; row n - 1 has no next row.
   int i = 0;
   for (; i + 4 <= n; i += 4) {
-    double dd[4], cc[4];
+    double dd[4], cc[5];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      dd[q] = d[i + q];
-      cc[q] = i + q > 0 ? e2[i + q - 1] : 0.0;
-    }
+    for (int q = 0; q < 4; ++q) dd[q] = d[i + q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) st.step(dd[q], cc[q], x);
+    for (int q = 0; q < 5; ++q) cc[q] = (i + q > 0 && i + q < n) ? e2[i + q - 1] : 1.0;
+    cc[0] = i > 0 ? cc[0] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) st.step(dd[q], cc[q], cc[q + 1] == 0.0, x);
     st.rescale();
   }
-  for (; i < n; ++i) st.step(d[i], i > 0 ? e2[i - 1] : 0.0, x);
+  for (; i < n; ++i)
+    st.step(d[i], i > 0 ? e2[i - 1] : 0.0, i + 1 < n && e2[i] == 0.0, x);
 #pragma unroll
   for (int t = 0; t < 3; ++t) cnt[t] = st.cnt[t];
 }
@@ -209,7 +223,7 @@ struct XLds {
   double2* v;
   double2* w;
   double2* tau;
-  double2* ypart;  // [kXThreads]
+  double2* ypart;  // [kXThreads]; the counts of the first bisection pass
   double* d;       // scaled diagonal after phase 2 starts
   double* e;
   double* e2;
@@ -841,7 +855,10 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
     L.d[i] = di;
     L.e[i] = ei;
     const double q = ei * ei;
-    L.e2[i] = q < 0x1p-900 ? 0.0 : q;
+    // Couplings below 2^-250 |T| split T (a relative perturbation below 1e-75);
+    // then two rows of the Sturm recurrence shrink it by at most 2^-500 and
+    // four rows (between renormalisations) stay above the underflow limit.
+    L.e2[i] = q < 0x1p-500 ? 0.0 : q;
   }
   __syncthreads();
   // Each eigenvalue to 2 eps of its magnitude or of |T| (scaled to ~1),
@@ -875,8 +892,32 @@ __device__ void heevx_tail_one(int n, double* E_, double2* V_, int64_t ldv, unsi
         }
       }
     };
+    // First pass, shared: the counts at 3 kXThreads evenly spaced points
+    // (one Sturm pass of the workgroup; the reduction's ypart area is free
+    // here). Eigenvalue j then starts from the points around it, an interval
+    // 3 kXThreads + 1 times narrower; the search keeps count(lo) <= j <
+    // count(hi), so the interval holds eigenvalue j.
+    constexpr int kPts = 3 * kXThreads;
+    int* c0 = reinterpret_cast<int*>(L.ypart);
+    const double w0 = (hi0 - lo0) / (kPts + 1);
+    {
+      double x[3];
+      int c[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) x[t] = lo0 + (double)(tid + t * kXThreads + 1) * w0;
+      sturm_count3(L.d, L.e2, n, x, c);
+#pragma unroll
+      for (int t = 0; t < 3; ++t) c0[tid + t * kXThreads] = c[t];
+    }
+    __syncthreads();
     for (int j = tid; j < n; j += kXThreads) {
-      double lo = lo0, hi = hi0;
+      int a = -1, b = kPts;  // points a, b with count(a) <= j < count(b)
+      while (b - a > 1) {
+        const int m = (a + b) >> 1;
+        if (c0[m] <= j) a = m; else b = m;
+      }
+      double lo = a < 0 ? lo0 : lo0 + (double)(a + 1) * w0;
+      double hi = b >= kPts ? hi0 : lo0 + (double)(b + 1) * w0;
       bisect(lo, hi, j, 2.0 * kEps);
       const double s = block_scale(L, n, 0.5 * (lo + hi), 8.0 * kEps);
       const double lj = tnorm > 0.0 ? 0.5 * (lo + hi) : 0.0;
