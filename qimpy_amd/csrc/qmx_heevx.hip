@@ -609,12 +609,14 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
     const int m = n - k - 1;
     gd2* rowk = W + (int64_t)k * n;
     const double2 alpha = P[pk(k + 1, k)];  // x_i = A[i][k], i > k
+    // Every wave sums the column itself (the same lanes, the same order, so
+    // every thread gets the same sigma and reflector without a block sum).
     double s = 0;
-    for (int i = k + 2 + tid; i < n; i += kRThreads) {
+    for (int i = k + 2 + lane; i < n; i += 64) {
       const double2 x = P[pk(i, k)];
       s += x.x * x.x + x.y * x.y;
     }
-    const double sigma = rblock_sum<kRWaves>(s, L.red);
+    const double sigma = xwave_sum(s);
     double beta, sr = 0, si = 0;
     double2 tk;
     if (sigma == 0.0 && alpha.y == 0.0) {
@@ -706,24 +708,28 @@ __device__ void heevx_reduce_one(const double2* A_, int n, int64_t lda, unsigned
       L.ypart[tid] = double2{yr + yr1, yi + yi1};
     }
     __syncthreads();
-    double part = 0;
-    for (int ii = tid; ii < m; ii += kRThreads) {
+    // y_i = sum of the partial products; v^H y by every wave (same order on
+    // every wave), then w = tau y - (|tau|^2 v^H y / 2) v.
+    auto ysum = [&](int ii) {
       double2 y = L.ypart[ii];
       for (int p = 1; p < Pp; ++p) {
         const double2 t = L.ypart[p * m + ii];
         y.x += t.x;
         y.y += t.y;
       }
-      L.w[k + 1 + ii] = y;
-      const double2 vi = L.v[k + 1 + ii];
+      return y;
+    };
+    double part = 0;
+    for (int ii = lane; ii < m; ii += 64) {
+      const double2 y = ysum(ii), vi = L.v[k + 1 + ii];
       part += vi.x * y.x + vi.y * y.y;
     }
-    const double vy = rblock_sum<kRWaves>(part, L.red);
+    const double vy = xwave_sum(part);
     const double t2 = tk.x * tk.x + tk.y * tk.y;
-    for (int i = k + 1 + tid; i < n; i += kRThreads) {
-      const double2 y = L.w[i], vi = L.v[i];
-      L.w[i] = double2{tk.x * y.x - tk.y * y.y - 0.5 * t2 * vy * vi.x,
-                       tk.x * y.y + tk.y * y.x - 0.5 * t2 * vy * vi.y};
+    for (int ii = tid; ii < m; ii += kRThreads) {
+      const double2 y = ysum(ii), vi = L.v[k + 1 + ii];
+      L.w[k + 1 + ii] = double2{tk.x * y.x - tk.y * y.y - 0.5 * t2 * vy * vi.x,
+                                tk.x * y.y + tk.y * y.x - 0.5 * t2 * vy * vi.y};
     }
     __syncthreads();
     // Lower triangle of A' -= v w^H + w v^H: rows to waves, columns to lanes.
