@@ -89,6 +89,23 @@ def test_jobs_join_running_bucket(gpu, dense_refs):
         assert np.abs(solver.results[i].eig - ref).max() < TOL[C64]
 
 
+def test_bucket_groups_served_in_turn(gpu, dense_refs):
+    """Buckets dealt to two groups that tick in turn (each group's step
+    ends with the copies its next tick reads): every result still matches
+    the dense answer, and jobs keep one Davidson step per visit."""
+    specs = _specs()
+    solver = _solver(C64, groups=2)
+    solver.submit([CellJob(s, job_id=i, arrival=0.01 * i) for i, s in enumerate(specs)])
+    now = 0.0
+    while not solver.idle:
+        solver.tick(now=now)
+        now += 0.01
+    assert len(set(solver._group_of.values())) == 2  # both groups hold buckets
+    for i, ref in enumerate(dense_refs):
+        r = solver.results[i]
+        assert r.converged and np.abs(r.eig - ref).max() < TOL[C64]
+
+
 def test_served_stream_matches_dense_and_alone(gpu, dense_refs):
     """Requests arriving over time (continuous batching): every result equals
     the dense answer and its own result served alone, within tolerance."""
