@@ -177,6 +177,46 @@ def single(spec, dtype, **kw):
     return 1.0 / (time.perf_counter() - t0), rep
 
 
+def density_measurement(spec) -> dict:
+    """Basis.collect_density (qmx_collect_density) on the 64-atom cell: 282
+    normalised random bands, complex64 and complex128, HIP events around
+    ten calls. Algorithmic bytes per band follow the dense-transform
+    convention of the local apply: a sphere read b G and one box transform
+    read and write 2 b N, plus the FP64 density once per call (16 N)."""
+    import torch
+
+    from qimpy_amd.electrons import electrons_from_spec
+
+    el = electrons_from_spec(spec)
+    b = el.basis
+    K, G, N = b.nk, b.n_tot, b.grid.N
+    B = int(spec.n_bands + spec.n_bands_extra)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    res = {"workload": f"64-atom cell, {b.grid.shape[0]}^3 box, {K} k, {B} bands, fillings 1",
+           "unit": "bands/s"}
+    for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
+        C = torch.randn((1, K, B, 1, G), dtype=dt, device="cuda", generator=g)
+        C /= torch.linalg.vector_norm(C, dim=-1, keepdim=True)
+        f = torch.ones((1, K, B), dtype=torch.float64)
+        rho = b.collect_density(C, f)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            rho = b.collect_density(C, f)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 10 * 1e-3
+        cs = C.element_size()
+        by = K * B * (cs * G + 2 * cs * N) + 16 * N
+        charge = float(rho[0].sum()) * b.volume / N
+        res[name] = {"ms_per_call": round(t * 1e3, 3), "bands_per_s": round(K * B / t, 1),
+                     "GBps_dense_convention": round(by / t / 1e9, 1),
+                     "frac": round(by / t / 1e9 / HBM_PEAK_GBS, 4),
+                     "charge_minus_sum_fw": charge - float((f[0] * b.w_sk[0].cpu()).sum())}
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -313,6 +353,8 @@ def main():
             "iterations": [r1m.n_iterations, r1p.n_iterations],
             "eig_err_vs_fp64_Ha": float(np.abs(r1m.eig.cpu().numpy() - r1p.eig.cpu().numpy()).max())}
         print(f"[bench] single 64-atom cell: mixed {v1m:.3f}, plain {v1p:.3f} solves/s", flush=True)
+        out["collect_density"] = density_measurement(spec1)
+        print(f"[bench] collect_density: {out['collect_density']}", flush=True)
     if not args.no_cpu:
         n_cpu = min(8, len(specs))
         out["cpu_baseline"] = cpu_baseline(

@@ -116,6 +116,25 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
                       int accumulate, void* workspace, int64_t workspace_bytes,
                       void* stream);
 
+/* Electron density of the bands C (k-points [k0, k0 + nk) of the plan):
+ *   rho[x][y][z] (= or +=) sum_{k, b} prefac[k][b] |psi_kb(x, y, z)|^2,
+ * psi = the unscaled inverse FFT of the sphere coefficients scattered into
+ * the box (Basis.collect_density / _CollectDensityKernel with accum_norm_,
+ * dft/electrons/_basis_ops.py:207-343 and math/_linalg.py:31-76, for one
+ * spin channel and scalar wavefunctions; prefac = f w_sk / volume as in
+ * _basis_ops.py:230). C, ldb, ldk as for qmx_apply_local_k; prefac: device
+ * double, k stride ldp (bands with prefac 0 are skipped, like the
+ * argwhere(f) walk of accum_norm_); rho: device double, the box in C order
+ * (z fastest), accumulated in FP64. Boxes whose x-y planes fit the plane
+ * pass only (every box up to 64 x 64 planes). Workspace:
+ * qmx_density_workspace_bytes(plan, dtype, n_bands) bytes for one pass over
+ * the bands (less runs them in blocks). */
+int64_t qmx_density_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands);
+int qmx_collect_density(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,
+                        int64_t ldb, int64_t ldk, int64_t n_bands, const double* prefac,
+                        int64_t ldp, double* rho, int accumulate, void* workspace,
+                        int64_t workspace_bytes, void* stream);
+
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
  * The result is multiplied by `scale` (1/N for QimPy's forward transform,

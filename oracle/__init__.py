@@ -23,6 +23,9 @@ How each function is pinned (beyond retyping the formula):
 - ``linalg.eighg``: H V = O V E, V^H O V = I, against scipy.linalg.eigh(H, O).
 - ``init.randomize``: integer xorshift states bit-exact against an
   independent pure-Python big-integer implementation of the same recurrence.
+- ``density.collect_density``: against the closed forms of one and two plane
+  waves, the cell integral sum f w (Parseval) and a direct O(N^2) inverse
+  DFT on a tiny box.
 - ``davidson.davidson`` / ``chefsi.chefsi``: converged eigenvalues against
   the dense brute force to 1e-9 Ha; the Chebyshev recurrence against the
   closed-form Chebyshev polynomial on a diagonal H.

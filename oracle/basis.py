@@ -34,6 +34,7 @@ class Basis:
         self.pad_mask = np.arange(self.n_tot)[None] >= self.n[:, None]  # :195-197
         self.KE = self._ke(self.iG, self.k)  # (K, n_tot)
         self.w_sk = 2.0 * spec.wk  # integer spin weight 2 // (S X)  :137-138
+        self.volume = float(abs(np.linalg.det(np.asarray(spec.Rbasis))))  # lattice volume
 
     def _ke(self, iG: np.ndarray, k: np.ndarray) -> np.ndarray:
         G = (iG + k[:, None, :]) @ self.Gbasis.T

@@ -72,6 +72,9 @@ SIGNATURES = {
     "qmx_dot_workspace_bytes": (i64, [i32, i64, i64, i64, i64]),
     "qmx_dot": (i32, [i32, vp, i64, i64, vp, i64, i64, i64, i64, i64, i64, vp, i64, i64, i32,
                       vp, i64, vp]),
+    "qmx_density_workspace_bytes": (i64, [vp, i32, i64]),
+    "qmx_collect_density": (i32, [vp, i64, i64, i32, vp, i64, i64, i64, vp, i64, vp, i32, vp, i64,
+                                  vp]),
     "qmx_heev_small_work_bytes": (i64, [i64]),
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_heevx_work_bytes": (i64, [i64, i64]),

@@ -198,6 +198,43 @@ class Basis:
         out = self.apply_local(C.coeff, V, with_ke=True)
         return Wavefunction(self, coeff=out, band_division=C.band_division)
 
+    def collect_density(self, C, f: torch.Tensor, need_Mvec: bool = False) -> torch.Tensor:
+        """Electron density of `C` with fillings `f` (S, K, B) on the box, as
+        (n_spins, S0, S1, S2) float64: [n_tot] for one spin channel, [n_tot,
+        M_z] for two (Basis.collect_density, dft/electrons/_basis_ops.py:
+        207-287; prefactor f w_sk / volume at :230, accum_norm_ over k and
+        bands at :332). One qmx_collect_density launch sequence per spin;
+        scalar wavefunctions only (need_Mvec needs spinors)."""
+        if need_Mvec:
+            raise NotImplementedError("magnetization vectors need spinorial wavefunctions")
+        coeff = C.coeff if hasattr(C, "coeff") else C
+        if coeff.dtype not in DTYPE_CODE:
+            raise TypeError(f"unsupported wavefunction dtype {coeff.dtype}")
+        S, K, B, X, G = coeff.shape
+        if K != self.nk or G != self.n_tot or X != 1:
+            raise ValueError(f"coefficients {tuple(coeff.shape)} do not match the basis")
+        f = torch.as_tensor(f, dtype=torch.float64, device=coeff.device)
+        if tuple(f.shape) != (S, K, B):
+            raise ValueError(f"fillings {tuple(f.shape)} != {(S, K, B)}")
+        if coeff.stride(-1) != 1 or coeff.stride(2) < G or (K > 1 and coeff.stride(1) < G):
+            coeff = coeff.contiguous()
+        prefac = (f * (self.w_sk.to(torch.float64) / self.volume)).contiguous()  # (S, K, B)
+        code = DTYPE_CODE[coeff.dtype]
+        need = int(_lib.lib().qmx_density_workspace_bytes(self._plan, code, B))
+        one = int(_lib.lib().qmx_density_workspace_bytes(self._plan, code, 1))
+        ws = torch.empty(max(one, min(need, WORKSPACE_CAP)), dtype=torch.uint8, device=coeff.device)
+        rho_diag = torch.empty((S,) + self.grid.shape, dtype=torch.float64, device=coeff.device)
+        for s in range(S):
+            cs = coeff[s, :, :, 0]
+            _lib.call("qmx_collect_density", self._plan, 0, K, code, cs.data_ptr(), cs.stride(1),
+                      cs.stride(0), B, prefac[s].data_ptr(), prefac.stride(1),
+                      rho_diag[s].data_ptr(), 0, ws.data_ptr(), ws.numel(), rc.stream_ptr())
+        density = torch.empty_like(rho_diag)
+        density[0] = rho_diag.sum(dim=0)  # n_tot
+        if S == 2:
+            density[1] = rho_diag[0] - rho_diag[1]  # M_z
+        return density
+
     @classmethod
     def from_spec(cls, spec, ions=None) -> "Basis":
         return cls(Rbasis=spec.Rbasis, k=spec.k, wk=spec.wk, ke_cutoff=spec.ke_cutoff,

@@ -405,6 +405,108 @@ __global__ void __launch_bounds__(NT)
     if (tid + q * NT < ncol) row[tid + q * NT] = P[co[q]];
 }
 
+// ------------------------------------------------------ density (plane pass)
+// Density of a block of bands on the plane path: one workgroup per (z, band
+// group, k). For each band of its group with a nonzero prefactor w (the
+// filling times the k weight over the cell volume) it scatters the band's
+// active columns at this z (T1 after the inverse z-transform, z-major) into
+// the zeroed plane, transforms them back along y (active x rows) and x
+// (every y), and adds w |psi|^2 to its own points, kept in registers in
+// FP64 across the bands; the sum over its bands is added atomically to the
+// z-major density rhoT[z][x][y].
+template <typename C2, typename R, int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+    k_plane_density(const C2* __restrict__ T1, const double* __restrict__ prefac, int64_t ldp,
+                    const int* __restrict__ col_ix, const int* __restrict__ col_iy,
+                    const int* __restrict__ x_list, const C2* __restrict__ twx_g,
+                    const C2* __restrict__ twy_g, Radices rx, Radices ry, FastDiv fny,
+                    FastDiv fnxa, int ncol, int nxa, int nb, int b0, int bpg,
+                    double* __restrict__ rhoT) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int kAcc = MAXE;  // points per thread: nx * ny <= NT * MAXE on the plane path
+  const int nx = rx.n, ny = ry.n, ldp_ = odd_ld(ny), nz = gridDim.x;
+  C2* P = reinterpret_cast<C2*>(smem_raw);
+  C2* twx = P + nx * ldp_;
+  C2* twy = twx + nx;
+  int* xl = reinterpret_cast<int*>(twy + ny);
+  const int z = blockIdx.x, g = blockIdx.y, k = blockIdx.z;
+  const int tid = threadIdx.x, nxy = nx * ny;
+  for (int i = tid; i < nx; i += NT) twx[i] = twx_g[i];
+  for (int i = tid; i < ny; i += NT) twy[i] = twy_g[i];
+  for (int i = tid; i < nxa; i += NT) xl[i] = x_list[i];
+  double acc[kAcc];
+#pragma unroll
+  for (int q = 0; q < kAcc; ++q) acc[q] = 0.0;
+  const int bend = min(nb, (g + 1) * bpg);
+  for (int ib = g * bpg; ib < bend; ++ib) {
+    const double w = prefac[(int64_t)k * ldp + b0 + ib];
+    if (w == 0.0) continue;  // empty band (uniform over the workgroup)
+    const C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
+    C2 cv[kPlaneColPer];
+    int co[kPlaneColPer];
+#pragma unroll
+    for (int q = 0; q < kPlaneColPer; ++q) {
+      const int c = tid + q * NT;
+      if (c < ncol) {
+        cv[q] = row[c];
+        co[q] = col_ix[c] * ldp_ + col_iy[c];
+      }
+    }
+    zero_lds(P, nx * ldp_);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPlaneColPer; ++q)
+      if (tid + q * NT < ncol) P[co[q]] = cv[q];
+    __syncthreads();
+    fft_ip<C2, +1, NT, MAXE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp_, xl);
+    fft_ip<C2, +1, NT, MAXE, true, false>(P, twx, rx, ny, fny, ldp_, 1, nullptr);
+#pragma unroll
+    for (int q = 0; q < kAcc; ++q) {
+      const int i = tid + q * NT;
+      if (i < nxy) {
+        const int x = fdiv(i, fny);
+        const C2 v = P[x * ldp_ + (i - x * ny)];
+        acc[q] += w * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+      }
+    }
+    __syncthreads();  // the plane is zeroed for the next band
+  }
+  double* rz = rhoT + (int64_t)z * nxy;
+#pragma unroll
+  for (int q = 0; q < kAcc; ++q) {
+    const int i = tid + q * NT;
+    if (i < nxy && acc[q] != 0.0) unsafeAtomicAdd(rz + i, acc[q]);
+  }
+}
+
+// rho[x][y][z] (= or +=) rhoT[z][x][y] (32 x 32 tiles through LDS) and a
+// fill kernel for the z-major accumulator.
+template <bool kAccumulate>
+__global__ void __launch_bounds__(256)
+    k_density_unmajor(const double* __restrict__ rhoT, double* __restrict__ rho, int nxy, int nz) {
+  __shared__ double tile[32][33];
+  const int xy0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int zz = z0 + r, xy = xy0 + tx;
+    if (xy < nxy && zz < nz) tile[r][tx] = rhoT[(int64_t)zz * nxy + xy];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int xy = xy0 + r, zz = z0 + tx;
+    if (xy < nxy && zz < nz) {
+      double* o = rho + (int64_t)xy * nz + zz;
+      *o = kAccumulate ? *o + tile[tx][r] : tile[tx][r];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero(double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
 // Vt[s][z][xy] = V[s][xy][z]: each potential box to z-major order (32 x 32
 // tiles through LDS, both sides coalesced).
 template <typename R>
@@ -823,6 +925,96 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   return QMX_OK;
 }
 
+template <typename C2, typename R, int NT, int MAXE>
+static int set_density_lds(size_t bytes) {
+  static size_t done = 0;
+  if (bytes > done) {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_density<C2, R, NT, MAXE>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done = bytes;
+  }
+  return QMX_OK;
+}
+
+// Workspace of the density: the z-major accumulator (N doubles), then T1
+// of a block of bands.
+static int64_t density_fixed_bytes(const qmx_plan* p) { return ((int64_t)p->N * 8 + 255) / 256 * 256; }
+
+template <typename R>
+int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t ldb,
+                         int64_t ldk, int64_t nb, const double* prefac, int64_t ldp, double* rho,
+                         int accumulate, void* ws, int64_t ws_bytes, hipStream_t st) {
+  using C2 = typename Cplx<R>::T;
+  const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
+  const int ncol = p->ncol, nxa = p->nxa, cs = sizeof(C2);
+  QMX_CHECK_ARG(use_plane_path(p, cs),
+                "qmx_collect_density: the box's x-y planes exceed the plane pass (LDS)");
+  const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
+  const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
+  const int64_t fixed = density_fixed_bytes(p);
+  const int64_t per_band = (int64_t)ncol * nz * cs;
+  QMX_CHECK_ARG(ws_bytes > fixed, "qmx_collect_density: workspace smaller than the density");
+  int64_t blk = (ws_bytes - fixed) / (per_band * nk);
+  QMX_CHECK_ARG(blk >= 1, "qmx_collect_density: workspace smaller than one band of every k");
+  blk = std::min<int64_t>(blk, nb);
+  blk = std::min<int64_t>(blk, 65535 / nk);
+  const C2* C = static_cast<const C2*>(Cv);
+  double* rhoT = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((p->N + 255) / 256, 4096)), dim3(256), 0,
+                     st, rhoT, (int64_t)p->N);
+  QMX_LAUNCHED();
+  const int Mz = max_lines(nz, cs, 0, 64);
+  const FastDiv fz = make_fastdiv(Mz);
+  const size_t shA = (size_t)(2 * odd_ld(Mz) * nz + nz) * cs;
+  const size_t shP = plane_lds_bytes(p, cs);
+  const int ntP = plane_threads(p);
+  const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
+  int rc = ntP == kPlaneThreadsSmall
+               ? set_density_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>(shP)
+               : set_density_lds<C2, R, kPlaneThreads, kPlaneMaxE>(shP);
+  if (rc) return rc;
+  QMX_CHECK_ARG((int64_t)nx * ny <= (int64_t)ntP * maxeP,
+                "qmx_collect_density: plane larger than the accumulator registers");
+  const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
+  const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
+  const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
+  for (int64_t b0 = 0; b0 < nb; b0 += blk) {
+    const int cur = (int)std::min<int64_t>(blk, nb - b0);
+    C2* T1 = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
+    dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
+    hipLaunchKernelGGL((k_sphere_to_cols<C2, R, true>), gA, dim3(kThreads), shA, st, C, ldb, ldk,
+                       T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot,
+                       (int)b0);
+    QMX_LAUNCHED();
+    // Band groups: enough workgroups to fill the chip, each accumulating
+    // several bands in registers.
+    const int64_t wg_base = (int64_t)nz * nk;
+    int groups = (int)std::min<int64_t>(cur, std::max<int64_t>(1, (2048 + wg_base - 1) / wg_base));
+    const int bpg = (cur + groups - 1) / groups;
+    groups = (cur + bpg - 1) / bpg;
+    dim3 gP(nz, groups, nk);
+    if (ntP == kPlaneThreadsSmall)
+      hipLaunchKernelGGL((k_plane_density<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>), gP,
+                         dim3(kPlaneThreadsSmall), shP, st, T1, prefac + k0 * ldp, ldp,
+                         p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1),
+                         rxc, ryc, fny, fnxa, ncol, nxa, cur, (int)b0, bpg, rhoT);
+    else
+      hipLaunchKernelGGL((k_plane_density<C2, R, kPlaneThreads, kPlaneMaxE>), gP,
+                         dim3(kPlaneThreads), shP, st, T1, prefac + k0 * ldp, ldp, p->d_col_ix,
+                         p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), rxc, ryc, fny,
+                         fnxa, ncol, nxa, cur, (int)b0, bpg, rhoT);
+    QMX_LAUNCHED();
+  }
+  const int nxy = nx * ny;
+  dim3 gT((nxy + 31) / 32, (nz + 31) / 32);
+  if (accumulate)
+    hipLaunchKernelGGL(k_density_unmajor<true>, gT, dim3(256), 0, st, rhoT, rho, nxy, nz);
+  else
+    hipLaunchKernelGGL(k_density_unmajor<false>, gT, dim3(256), 0, st, rhoT, rho, nxy, nz);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
 template <typename R>
 int fft3d_impl(const int32_t* shape, void* datav, int64_t batch, int direction, double scale,
                hipStream_t st) {
@@ -1235,6 +1427,40 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
                                     V, ke, ke_ldk, n_bands, accumulate, workspace,
                                     workspace_bytes, st);
   set_error("qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
+  return QMX_ERR_ARG;
+}
+
+int64_t qmx_density_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
+  if (!plan || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
+  const int cs = dtype == QMX_C64 ? 8 : 16;
+  return density_fixed_bytes(plan) + (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+}
+
+int qmx_collect_density(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,
+                        int64_t ldb, int64_t ldk, int64_t n_bands, const double* prefac,
+                        int64_t ldp, double* rho, int accumulate, void* workspace,
+                        int64_t workspace_bytes, void* stream) {
+  QMX_CHECK_ARG(plan && C && prefac && rho, "qmx_collect_density: null argument");
+  QMX_CHECK_ARG(n_bands >= 0 && ldp >= n_bands, "qmx_collect_density: bad n_bands or ldp");
+  QMX_CHECK_ARG(k0 >= 0 && nk >= 0 && k0 + nk <= plan->nk,
+                "qmx_collect_density: k range outside plan");
+  QMX_CHECK_ARG(workspace && workspace_bytes > 0, "qmx_collect_density: workspace required");
+  QMX_CHECK_ARG(ldb >= plan->n_tot && (nk <= 1 || ldk > 0), "qmx_collect_density: bad strides");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n_bands == 0 || nk == 0) {
+    if (accumulate) return QMX_OK;
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((plan->N + 255) / 256, 4096)),
+                       dim3(256), 0, st, rho, (int64_t)plan->N);
+    QMX_LAUNCHED();
+    return QMX_OK;
+  }
+  if (dtype == QMX_C64)
+    return collect_density_impl<float>(plan, (int)k0, (int)nk, C, ldb, ldk, n_bands, prefac, ldp,
+                                       rho, accumulate, workspace, workspace_bytes, st);
+  if (dtype == QMX_C128)
+    return collect_density_impl<double>(plan, (int)k0, (int)nk, C, ldb, ldk, n_bands, prefac,
+                                        ldp, rho, accumulate, workspace, workspace_bytes, st);
+  set_error("qmx_collect_density: dtype must be QMX_C64 or QMX_C128");
   return QMX_ERR_ARG;
 }
 

@@ -129,6 +129,63 @@ def test_apply_potential_boxes(gpu, g, dtype):
     assert rel(got, ref) < (1e-12 if dtype == C128 else 2e-5)
 
 
+@pytest.mark.parametrize("g", [24, 36, 48, 60])
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_collect_density_boxes(gpu, g, dtype):
+    """Basis.collect_density (qmx_collect_density) against the oracle on the
+    box sizes of the batched workloads: fractional and zero fillings,
+    complex128 to 1e-12 and complex64 to 1e-5 of the peak density; the cell
+    integral equals sum f w (normalised bands)."""
+    from oracle.density import collect_density as o_density
+    spec = _cell_with_box(g)
+    print(f"[collect_density] {g}^3 {dtype}", flush=True)
+    ob = OBasis(spec)
+    el = electrons_from_spec(spec)
+    C = rand_coeff(ob, 7, seed=g + 1)
+    C /= np.linalg.norm(C, axis=-1, keepdims=True)
+    rng = np.random.default_rng(g)
+    f = rng.uniform(0.0, 1.0, (len(ob.k), 7))
+    f[:, 3] = 0.0  # an empty band
+    ref = o_density(ob, C, f)
+    got = el.basis.collect_density(to_wf(el.basis, C, dtype), torch.tensor(f)[None])
+    assert got.shape == (1,) + ob.shape and got.dtype == torch.float64
+    assert rel(got[0], ref) < (1e-12 if dtype == C128 else 1e-5)
+    charge = float(got[0].sum()) * ob.volume / ob.N
+    assert abs(charge - (f * ob.w_sk[:, None]).sum()) < (1e-10 if dtype == C128 else 1e-5)
+
+
+def test_collect_density_spins_blocks_empty(cfg0_el):
+    """Two spin channels give [n_tot, M_z]; a tiny workspace (many band
+    blocks) gives the same density; all-zero fillings give zero."""
+    from oracle.density import collect_density as o_density
+    import qimpy_amd.basis as qb
+    spec, ob, el = cfg0_el
+    b = el.basis
+    C0, C1 = rand_coeff(ob, 6, seed=21), rand_coeff(ob, 6, seed=22)
+    rng = np.random.default_rng(7)
+    f = rng.uniform(0.0, 1.0, (2, len(ob.k), 6))
+    coeff = torch.stack([to_wf(b, C0, C128).coeff[0], to_wf(b, C1, C128).coeff[0]])
+    w_sk = ob.w_sk.copy()
+    try:
+        ob.w_sk = w_sk / 2  # spin weight 1 with two channels (_basis.py:137)
+        b.w_sk = b.w_sk / 2
+        r0, r1 = o_density(ob, C0, f[0]), o_density(ob, C1, f[1])
+        got = b.collect_density(coeff, torch.tensor(f))
+        assert rel(got[0], r0 + r1) < 1e-12 and rel(got[1], r0 - r1) < 1e-12
+        cap = qb.WORKSPACE_CAP
+        try:
+            qb.WORKSPACE_CAP = 1
+            blocked = b.collect_density(coeff, torch.tensor(f))
+        finally:
+            qb.WORKSPACE_CAP = cap
+        assert rel(blocked, got) < 1e-13
+        zero = b.collect_density(coeff, torch.zeros(2, len(ob.k), 6, dtype=torch.float64))
+        assert float(zero.abs().max()) == 0.0
+    finally:
+        ob.w_sk = w_sk
+        b.w_sk = b.w_sk * 2
+
+
 def test_apply_local_blocking_inplace_accumulate(cfg0_el):
     """Tiny workspace (many band blocks), out == C, and accumulate."""
     spec, ob, el = cfg0_el

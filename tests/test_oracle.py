@@ -248,3 +248,49 @@ def test_chebyshev_recurrence_diagonal():
     # inside [lo, up] the filter is bounded by 1/|T_m((c - a)/e)|
     inside = (x >= lo) & (x <= up)
     assert np.abs(Y[inside]).max() <= 1.0 / abs(cheb((c - a) / e)) * (1 + 1e-12)
+
+
+def test_density_closed_forms(small):
+    """Oracle density: one plane wave gives f w |c|^2 / V everywhere; two give
+    |a|^2 + |b|^2 + 2 Re(a b* exp(2 pi i (m1 - m2) . r / S)) point by point
+    (m the integer box frequencies); normalised random bands integrate to
+    sum f w over the cell; all against a direct O(N^2) inverse DFT."""
+    from oracle.density import collect_density
+
+    spec, b = small
+    K, G = b.fft_index.shape
+    rng = np.random.default_rng(3)
+    shape = np.array(b.shape)
+    r = np.stack(np.meshgrid(*[np.arange(s) for s in b.shape], indexing="ij"), -1)
+    # one plane wave in band 0 of k 0, filling 1 (other bands and k empty)
+    C = np.zeros((K, 2, G), complex)
+    C[0, 0, 3] = 0.6 - 0.8j
+    f = np.zeros((K, 2))
+    f[0, 0] = 1.0
+    rho = collect_density(b, C, f)
+    assert np.abs(rho - b.w_sk[0] / b.volume).max() < 1e-13 * b.w_sk[0] / b.volume
+    # two plane waves (a at sphere point 1, c at 5) in band 1 of k 1, filling 0.5
+    a, c = 0.3 + 0.4j, -0.7 + 0.2j
+    C[:] = 0.0
+    C[1, 1, 1], C[1, 1, 5] = a, c
+    f[:] = 0.0
+    f[1, 1] = 0.5
+    m1, m2 = b.iG[1, 1], b.iG[1, 5]
+    phase = np.exp(2j * np.pi * ((m1 - m2) * r / shape).sum(-1))
+    want = 0.5 * b.w_sk[1] / b.volume * (abs(a) ** 2 + abs(c) ** 2 + 2 * (a * np.conj(c) * phase).real)
+    assert np.abs(collect_density(b, C, f) - want).max() < 1e-13
+    # normalised random bands: cell integral (rho dV, dV = V / N) = sum f w
+    C = rng.standard_normal((K, 3, G)) + 1j * rng.standard_normal((K, 3, G))
+    b.constrain(C)
+    C /= np.linalg.norm(C, axis=-1, keepdims=True)
+    f = rng.uniform(0.0, 1.0, (K, 3))
+    f[0, 2] = 0.0
+    rho = collect_density(b, C, f)
+    assert abs(rho.sum() * b.volume / b.N - (f * b.w_sk[:, None]).sum()) < 1e-12
+    # direct inverse DFT of k 0, band 1
+    rr = r.reshape(-1, 3)
+    psi = np.exp(2j * np.pi * ((rr[:, None, :] * b.iG[0][None]) / shape).sum(-1)) @ C[0, 1]
+    f1 = np.zeros((K, 3))
+    f1[0, 1] = 1.0
+    want = b.w_sk[0] / b.volume * np.abs(psi) ** 2
+    assert np.abs(collect_density(b, C, f1).reshape(-1) - want).max() < 1e-12 * want.max()
