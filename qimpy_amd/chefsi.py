@@ -19,8 +19,10 @@ from .linalg import eigh, eighg, eighg_robust
 
 class CheFSI(Davidson):
     def __init__(self, *, electrons, n_iterations: int = 100, eig_threshold: float = 1e-8,
-                 filter_order: int = 10, init_threshold: float = 1e-1):
-        super().__init__(electrons=electrons, n_iterations=n_iterations, eig_threshold=eig_threshold)
+                 filter_order: int = 10, init_threshold: float = 1e-1,
+                 residual_threshold: Optional[float] = None):
+        super().__init__(electrons=electrons, n_iterations=n_iterations, eig_threshold=eig_threshold,
+                         residual_threshold=residual_threshold)
         self.filter_order = filter_order
         self.init_threshold = init_threshold
         self._line_prefix = "CheFSI"
@@ -38,7 +40,13 @@ class CheFSI(Davidson):
         eig_threshold = eig_threshold if eig_threshold else self.eig_threshold
         if el.deig_max > self.init_threshold:
             self._line_prefix = "ChefSI_init"
-            super().__call__(n_iterations, self.init_threshold)
+            # The warm start converges on |deig| < init_threshold only (a
+            # residual threshold belongs to the filter iterations below).
+            res_thr, self.residual_threshold = self.residual_threshold, None
+            try:
+                super().__call__(n_iterations, self.init_threshold)
+            finally:
+                self.residual_threshold = res_thr
             self._line_prefix = "CheFSI"
             HC = self._HC
             del self._HC
@@ -97,6 +105,19 @@ class CheFSI(Davidson):
             deig = torch.abs(el.eig - eig_prev)
             el.deig_max, n_eigs_done = self._check_deig(deig, eig_threshold)
             converged = n_eigs_done == n_bands
+            if not converged and self.residual_threshold is not None:
+                # Mixed form: |deig| stalls at the complex64 rounding of the
+                # subspace, so bands lock (leave the filter) and the solve
+                # stops by residual |HC - eig C| < threshold, counted from the
+                # lowest band like the deig locking (the FP64 Rayleigh-Ritz
+                # that follows sets the final eigenvalues).
+                R = HC.coeff[:, :, :n_bands] - el.C.coeff[:, :, :n_bands] * el.eig[
+                    :, :, :n_bands, None, None].to(rdtype)
+                r = torch.linalg.vector_norm(R, dim=(-2, -1))  # (S, K, n_bands)
+                pending = torch.where((r > self.residual_threshold).flatten(0, 1).any(dim=0))[0]
+                n_res_done = int(pending[0].item()) if len(pending) else n_bands
+                n_eigs_done = max(n_eigs_done, n_res_done)
+                converged = n_res_done >= n_bands
             self._report(n_eigs_done, converged)
             if converged:
                 break

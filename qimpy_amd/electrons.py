@@ -74,7 +74,8 @@ class Electrons:
         elif diagonalizer == "chefsi":
             self.diagonalize = CheFSI(electrons=self, n_iterations=n_iterations,
                                       eig_threshold=eig_threshold, filter_order=filter_order,
-                                      init_threshold=init_threshold)
+                                      init_threshold=init_threshold,
+                                      residual_threshold=residual_threshold)
         else:
             raise ValueError(f"unknown diagonalizer {diagonalizer!r}")
 

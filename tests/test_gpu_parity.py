@@ -333,6 +333,22 @@ def test_cfg0_mixed_within_1e6(cfg0_el):
     assert d < 1e-6
 
 
+def test_cfg0_chefsi_mixed_residual(cfg0_el):
+    """Mixed CheFSI: bands lock and the solve stops on ||HC - eig C|| < 3e-5
+    (complex64 filter), then the FP64 Rayleigh-Ritz; within 1e-7 Ha of the
+    dense FP64 eigenvalues."""
+    spec, ob, _ = cfg0_el
+    exact = dense_eigenvalues(ob, spec.n_bands)
+    el, rep = _solve(spec, C64, 100, 1e-8, diagonalizer="chefsi", filter_order=8,
+                     residual_threshold=3e-5)
+    d = np.abs(rep.eig[0].cpu().numpy() - exact).max()
+    print(f"[cfg0 chefsi mixed] iterations {rep.n_iterations}, max |eig - FP64| = {d:.2e},"
+          f" max residual {rep.residual_norm.max().item():.2e}", flush=True)
+    assert rep.n_iterations < 100
+    assert d < 1e-7
+    assert rep.residual_norm.max().item() < 1e-4
+
+
 def _hermitian_batch(n, seed):
     """Three random Hermitian matrices plus one with a 4-fold degenerate,
     one zero and one tiny eigenvalue (U diag U^H), all in complex128."""
