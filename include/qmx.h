@@ -121,8 +121,8 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
  * psi = the unscaled inverse FFT of the sphere coefficients scattered into
  * the box (Basis.collect_density / _CollectDensityKernel with accum_norm_,
  * dft/electrons/_basis_ops.py:207-343 and math/_linalg.py:31-76, for one
- * spin channel and scalar wavefunctions; prefac = f w_sk / volume as in
- * _basis_ops.py:230). C, ldb, ldk as for qmx_apply_local_k; prefac: device
+ * spin channel and one spinor component; prefac = f w_sk / volume as in
+ * _basis_ops.py:229). C, ldb, ldk as for qmx_apply_local_k; prefac: device
  * double, k stride ldp (bands with prefac 0 are skipped, like the
  * argwhere(f) walk of accum_norm_); rho: device double, the box in C order
  * (z fastest), accumulated in FP64. Boxes whose x-y planes fit the plane
@@ -134,6 +134,22 @@ int qmx_collect_density(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype,
                         int64_t ldb, int64_t ldk, int64_t n_bands, const double* prefac,
                         int64_t ldp, double* rho, int accumulate, void* workspace,
                         int64_t workspace_bytes, void* stream);
+
+/* Spinor cross term of the density (Basis.collect_density with need_Mvec,
+ * dft/electrons/_basis_ops.py:330-336: accum_prod_(f, psi_1, conj(psi_0)),
+ * math/_linalg.py:77-107), summed over k and bands:
+ *   rho[0][r] + i rho[N][r] (= or +=) sum_{k,b} prefac[k, b] psi1 conj(psi0)(r)
+ * C0 and C1 hold the spinor components 0 and 1 of the same bands, with the
+ * strides of qmx_collect_density; rho is 2 N doubles (real part box, then
+ * imaginary part box), accumulated in FP64. The two x-y planes of a box must
+ * fit LDS together (every box up to 64 x 64 planes in both precisions).
+ * Workspace: qmx_density_cross_workspace_bytes(plan, dtype, n_bands). */
+int64_t qmx_density_cross_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands);
+int qmx_collect_density_cross(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype,
+                              const void* C0, const void* C1, int64_t ldb, int64_t ldk,
+                              int64_t n_bands, const double* prefac, int64_t ldp, double* rho,
+                              int accumulate, void* workspace, int64_t workspace_bytes,
+                              void* stream);
 
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).

@@ -75,6 +75,9 @@ SIGNATURES = {
     "qmx_density_workspace_bytes": (i64, [vp, i32, i64]),
     "qmx_collect_density": (i32, [vp, i64, i64, i32, vp, i64, i64, i64, vp, i64, vp, i32, vp, i64,
                                   vp]),
+    "qmx_density_cross_workspace_bytes": (i64, [vp, i32, i64]),
+    "qmx_collect_density_cross": (i32, [vp, i64, i64, i32, vp, vp, i64, i64, i64, vp, i64, vp, i32,
+                                        vp, i64, vp]),
     "qmx_heev_small_work_bytes": (i64, [i64]),
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_heevx_work_bytes": (i64, [i64, i64]),

@@ -43,8 +43,8 @@ class Basis:
         n_spinor: int = 1,
     ):
         rc.require_gpu()
-        if n_spinor != 1:
-            raise NotImplementedError("spinorial wavefunctions are not on this path")
+        if n_spinor not in (1, 2):
+            raise ValueError(f"n_spinor must be 1 or 2, not {n_spinor}")
         self.Rbasis = np.asarray(Rbasis, dtype=np.float64)
         self.Gbasis = 2 * np.pi * np.linalg.inv(self.Rbasis.T)  # lattice/_lattice.py:217
         self.volume = float(abs(np.linalg.det(self.Rbasis)))
@@ -200,39 +200,73 @@ class Basis:
 
     def collect_density(self, C, f: torch.Tensor, need_Mvec: bool = False) -> torch.Tensor:
         """Electron density of `C` with fillings `f` (S, K, B) on the box, as
-        (n_spins, S0, S1, S2) float64: [n_tot] for one spin channel, [n_tot,
-        M_z] for two (Basis.collect_density, dft/electrons/_basis_ops.py:
-        207-287; prefactor f w_sk / volume at :230, accum_norm_ over k and
-        bands at :332). One qmx_collect_density launch sequence per spin;
-        scalar wavefunctions only (need_Mvec needs spinors)."""
-        if need_Mvec:
-            raise NotImplementedError("magnetization vectors need spinorial wavefunctions")
+        (n, S0, S1, S2) float64 (Basis.collect_density, dft/electrons/
+        _basis_ops.py:207-287; prefactor f w_sk / volume at :229):
+        - without need_Mvec, [n_tot] for one spin channel, [n_tot, M_z] for
+          two; spinor components are summed (the prefactor tiled over them,
+          :231-234, accum_norm_ over k, bands and components at :338);
+        - with need_Mvec (spinorial C), [n_tot, M_x, M_y, M_z] from the
+          component densities rho_0, rho_1 (accum_norm_ at :329) and the
+          cross term X = sum f psi_1 conj(psi_0) (accum_prod_ at :330-336):
+          n_tot = rho_0 + rho_1, M_x = 2 Re X, M_y = 2 Im X,
+          M_z = rho_0 - rho_1 (:277-284).
+        One qmx_collect_density launch sequence per spin and component, one
+        qmx_collect_density_cross per spin."""
         coeff = C.coeff if hasattr(C, "coeff") else C
         if coeff.dtype not in DTYPE_CODE:
             raise TypeError(f"unsupported wavefunction dtype {coeff.dtype}")
         S, K, B, X, G = coeff.shape
-        if K != self.nk or G != self.n_tot or X != 1:
+        if K != self.nk or G != self.n_tot or X not in (1, 2):
             raise ValueError(f"coefficients {tuple(coeff.shape)} do not match the basis")
+        if need_Mvec and X != 2:
+            raise ValueError("need_Mvec needs spinorial wavefunctions (n_spinor = 2)")
         f = torch.as_tensor(f, dtype=torch.float64, device=coeff.device)
         if tuple(f.shape) != (S, K, B):
             raise ValueError(f"fillings {tuple(f.shape)} != {(S, K, B)}")
-        if coeff.stride(-1) != 1 or coeff.stride(2) < G or (K > 1 and coeff.stride(1) < G):
+        if (coeff.stride(-1) != 1 or coeff.stride(3) < G or coeff.stride(2) < X * G
+                or (K > 1 and coeff.stride(1) < G)):
             coeff = coeff.contiguous()
         prefac = (f * (self.w_sk.to(torch.float64) / self.volume)).contiguous()  # (S, K, B)
         code = DTYPE_CODE[coeff.dtype]
-        need = int(_lib.lib().qmx_density_workspace_bytes(self._plan, code, B))
-        one = int(_lib.lib().qmx_density_workspace_bytes(self._plan, code, 1))
-        ws = torch.empty(max(one, min(need, WORKSPACE_CAP)), dtype=torch.uint8, device=coeff.device)
-        rho_diag = torch.empty((S,) + self.grid.shape, dtype=torch.float64, device=coeff.device)
+        lib = _lib.lib()
+
+        def workspace(fn):
+            need, one = int(fn(self._plan, code, B)), int(fn(self._plan, code, 1))
+            return torch.empty(max(one, min(need, WORKSPACE_CAP)), dtype=torch.uint8,
+                               device=coeff.device)
+
+        ws = workspace(lib.qmx_density_workspace_bytes)
+        box = self.grid.shape
+        n_diag = 2 if need_Mvec else S
+        rho_diag = torch.empty((n_diag,) + box, dtype=torch.float64, device=coeff.device)
         for s in range(S):
-            cs = coeff[s, :, :, 0]
-            _lib.call("qmx_collect_density", self._plan, 0, K, code, cs.data_ptr(), cs.stride(1),
-                      cs.stride(0), B, prefac[s].data_ptr(), prefac.stride(1),
-                      rho_diag[s].data_ptr(), 0, ws.data_ptr(), ws.numel(), rc.stream_ptr())
-        density = torch.empty_like(rho_diag)
+            for x in range(X):
+                cs = coeff[s, :, :, x]
+                # need_Mvec: component x summed over spins; else spin s over components
+                out, acc = (rho_diag[x], s > 0) if need_Mvec else (rho_diag[s], x > 0)
+                _lib.call("qmx_collect_density", self._plan, 0, K, code, cs.data_ptr(),
+                          cs.stride(1), cs.stride(0), B, prefac[s].data_ptr(), prefac.stride(1),
+                          out.data_ptr(), int(acc), ws.data_ptr(), ws.numel(), rc.stream_ptr())
+        if not need_Mvec:
+            density = torch.empty_like(rho_diag)
+            density[0] = rho_diag.sum(dim=0)  # n_tot
+            if S == 2:
+                density[1] = rho_diag[0] - rho_diag[1]  # M_z
+            return density
+        del ws
+        ws = workspace(lib.qmx_density_cross_workspace_bytes)
+        cross = torch.empty((2,) + box, dtype=torch.float64, device=coeff.device)  # Re, Im
+        for s in range(S):
+            c0, c1 = coeff[s, :, :, 0], coeff[s, :, :, 1]
+            _lib.call("qmx_collect_density_cross", self._plan, 0, K, code, c0.data_ptr(),
+                      c1.data_ptr(), c0.stride(1), c0.stride(0), B, prefac[s].data_ptr(),
+                      prefac.stride(1), cross.data_ptr(), int(s > 0), ws.data_ptr(), ws.numel(),
+                      rc.stream_ptr())
+        density = torch.empty((4,) + box, dtype=torch.float64, device=coeff.device)
         density[0] = rho_diag.sum(dim=0)  # n_tot
-        if S == 2:
-            density[1] = rho_diag[0] - rho_diag[1]  # M_z
+        density[1] = 2.0 * cross[0]  # M_x
+        density[2] = 2.0 * cross[1]  # M_y
+        density[3] = rho_diag[0] - rho_diag[1]  # M_z
         return density
 
     @classmethod

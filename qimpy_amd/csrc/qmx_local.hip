@@ -414,9 +414,14 @@ __global__ void __launch_bounds__(NT)
 // (every y), and adds w |psi|^2 to its own points, kept in registers in
 // FP64 across the bands; the sum over its bands is added atomically to the
 // z-major density rhoT[z][x][y].
-template <typename C2, typename R, int NT, int MAXE>
+// CROSS: the spinor cross term instead. T1 and T1b hold the two spinor
+// components of the same bands; both planes are transformed (two LDS
+// planes) and w psi_1 conj(psi_0) is accumulated, its real part into
+// rhoT[z][x][y] and its imaginary part into rhoT[nz][...] above it.
+template <typename C2, typename R, int NT, int MAXE, bool CROSS>
 __global__ void __launch_bounds__(NT)
-    k_plane_density(const C2* __restrict__ T1, const double* __restrict__ prefac, int64_t ldp,
+    k_plane_density(const C2* __restrict__ T1, const C2* __restrict__ T1b,
+                    const double* __restrict__ prefac, int64_t ldp,
                     const int* __restrict__ col_ix, const int* __restrict__ col_iy,
                     const int* __restrict__ x_list, const C2* __restrict__ twx_g,
                     const C2* __restrict__ twy_g, Radices rx, Radices ry, FastDiv fny,
@@ -424,9 +429,11 @@ __global__ void __launch_bounds__(NT)
                     double* __restrict__ rhoT) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int kAcc = MAXE;  // points per thread: nx * ny <= NT * MAXE on the plane path
+  constexpr int kNp = CROSS ? 2 : 1;  // planes in LDS
   const int nx = rx.n, ny = ry.n, ldp_ = odd_ld(ny), nz = gridDim.x;
+  const int plane = nx * ldp_;
   C2* P = reinterpret_cast<C2*>(smem_raw);
-  C2* twx = P + nx * ldp_;
+  C2* twx = P + kNp * plane;
   C2* twy = twx + nx;
   int* xl = reinterpret_cast<int*>(twy + ny);
   const int z = blockIdx.x, g = blockIdx.y, k = blockIdx.z;
@@ -434,48 +441,67 @@ __global__ void __launch_bounds__(NT)
   for (int i = tid; i < nx; i += NT) twx[i] = twx_g[i];
   for (int i = tid; i < ny; i += NT) twy[i] = twy_g[i];
   for (int i = tid; i < nxa; i += NT) xl[i] = x_list[i];
-  double acc[kAcc];
+  double acc[kAcc], acc_im[CROSS ? kAcc : 1];
 #pragma unroll
   for (int q = 0; q < kAcc; ++q) acc[q] = 0.0;
+  if constexpr (CROSS) {
+#pragma unroll
+    for (int q = 0; q < kAcc; ++q) acc_im[q] = 0.0;
+  }
   const int bend = min(nb, (g + 1) * bpg);
   for (int ib = g * bpg; ib < bend; ++ib) {
     const double w = prefac[(int64_t)k * ldp + b0 + ib];
     if (w == 0.0) continue;  // empty band (uniform over the workgroup)
-    const C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
-    C2 cv[kPlaneColPer];
+    const int64_t roff = (((int64_t)k * nb + ib) * nz + z) * ncol;
+    C2 cv[kNp][kPlaneColPer];
     int co[kPlaneColPer];
 #pragma unroll
     for (int q = 0; q < kPlaneColPer; ++q) {
       const int c = tid + q * NT;
       if (c < ncol) {
-        cv[q] = row[c];
+        cv[0][q] = T1[roff + c];
+        if constexpr (CROSS) cv[1][q] = T1b[roff + c];
         co[q] = col_ix[c] * ldp_ + col_iy[c];
       }
     }
-    zero_lds(P, nx * ldp_);
+    zero_lds(P, kNp * plane);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPlaneColPer; ++q)
-      if (tid + q * NT < ncol) P[co[q]] = cv[q];
+      if (tid + q * NT < ncol) {
+        P[co[q]] = cv[0][q];
+        if constexpr (CROSS) P[plane + co[q]] = cv[1][q];
+      }
     __syncthreads();
-    fft_ip<C2, +1, NT, MAXE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp_, xl);
-    fft_ip<C2, +1, NT, MAXE, true, false>(P, twx, rx, ny, fny, ldp_, 1, nullptr);
+#pragma unroll
+    for (int j = 0; j < kNp; ++j) {
+      fft_ip<C2, +1, NT, MAXE, false, true>(P + j * plane, twy, ry, nxa, fnxa, 1, ldp_, xl);
+      fft_ip<C2, +1, NT, MAXE, true, false>(P + j * plane, twx, rx, ny, fny, ldp_, 1, nullptr);
+    }
 #pragma unroll
     for (int q = 0; q < kAcc; ++q) {
       const int i = tid + q * NT;
       if (i < nxy) {
         const int x = fdiv(i, fny);
         const C2 v = P[x * ldp_ + (i - x * ny)];
-        acc[q] += w * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        if constexpr (CROSS) {  // psi_1 conj(psi_0), psi_0 = v, psi_1 = u
+          const C2 u = P[plane + x * ldp_ + (i - x * ny)];
+          acc[q] += w * ((double)u.x * (double)v.x + (double)u.y * (double)v.y);
+          acc_im[q] += w * ((double)u.y * (double)v.x - (double)u.x * (double)v.y);
+        } else {
+          acc[q] += w * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        }
       }
     }
-    __syncthreads();  // the plane is zeroed for the next band
+    __syncthreads();  // the planes are zeroed for the next band
   }
   double* rz = rhoT + (int64_t)z * nxy;
 #pragma unroll
   for (int q = 0; q < kAcc; ++q) {
     const int i = tid + q * NT;
     if (i < nxy && acc[q] != 0.0) unsafeAtomicAdd(rz + i, acc[q]);
+    if constexpr (CROSS)
+      if (i < nxy && acc_im[q] != 0.0) unsafeAtomicAdd(rz + (int64_t)nz * nxy + i, acc_im[q]);
   }
 }
 
@@ -925,67 +951,103 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   return QMX_OK;
 }
 
-template <typename C2, typename R, int NT, int MAXE>
+template <typename C2, typename R, int NT, int MAXE, bool CROSS>
 static int set_density_lds(size_t bytes) {
   static size_t done = 0;
   if (bytes > done) {
-    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_density<C2, R, NT, MAXE>,
+    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_density<C2, R, NT, MAXE, CROSS>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     done = bytes;
   }
   return QMX_OK;
 }
 
-// Workspace of the density: the z-major accumulator (N doubles), then T1
-// of a block of bands.
-static int64_t density_fixed_bytes(const qmx_plan* p) { return ((int64_t)p->N * 8 + 255) / 256 * 256; }
+// Workspace of the density: the z-major accumulator (N doubles, 2 N for the
+// spinor cross term), then T1 of a block of bands (of both components for
+// the cross term).
+static int64_t density_fixed_bytes(const qmx_plan* p, bool cross) {
+  return ((int64_t)p->N * 8 * (cross ? 2 : 1) + 255) / 256 * 256;
+}
 
+template <typename C2, typename R, int NT, int MAXE, bool CROSS>
+static void launch_density(dim3 grid, size_t sh, hipStream_t st, const C2* T1, const C2* T1b,
+                           const double* prefac, int64_t ldp, const qmx_plan* p, Radices rxc,
+                           Radices ryc, FastDiv fny, FastDiv fnxa, int cur, int b0, int bpg,
+                           double* rhoT) {
+  hipLaunchKernelGGL((k_plane_density<C2, R, NT, MAXE, CROSS>), grid, dim3(NT), sh, st, T1, T1b,
+                     prefac, ldp, p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
+                     tw_of<C2>(p, 1), rxc, ryc, fny, fnxa, p->ncol, p->nxa, cur, b0, bpg, rhoT);
+}
+
+// Cv1 == nullptr: rho (N doubles) = or += sum_b w_b |psi_b|^2. Otherwise the
+// spinor cross term: rho (2 N doubles, real then imaginary part) = or +=
+// sum_b w_b psi1_b conj(psi0_b), psi0 in Cv, psi1 in Cv1 (same strides).
 template <typename R>
-int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t ldb,
-                         int64_t ldk, int64_t nb, const double* prefac, int64_t ldp, double* rho,
-                         int accumulate, void* ws, int64_t ws_bytes, hipStream_t st) {
+int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, const void* Cv1,
+                         int64_t ldb, int64_t ldk, int64_t nb, const double* prefac, int64_t ldp,
+                         double* rho, int accumulate, void* ws, int64_t ws_bytes, hipStream_t st) {
   using C2 = typename Cplx<R>::T;
+  const bool cross = Cv1 != nullptr;
+  const int n_comp = cross ? 2 : 1;
   const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
   const int ncol = p->ncol, nxa = p->nxa, cs = sizeof(C2);
-  QMX_CHECK_ARG(use_plane_path(p, cs),
-                "qmx_collect_density: the box's x-y planes exceed the plane pass (LDS)");
+  const char* name = cross ? "qmx_collect_density_cross" : "qmx_collect_density";
+  if (!use_plane_path(p, cs)) {
+    set_error(std::string(name) + ": the box's x-y planes exceed the plane pass (LDS)");
+    return QMX_ERR_ARG;
+  }
+  const size_t shP = plane_lds_bytes(p, cs) + (cross ? (size_t)nx * odd_ld(ny) * cs : 0);
+  if (shP > (size_t)kLdsMax) {
+    set_error(std::string(name) + ": two x-y planes of this box exceed LDS");
+    return QMX_ERR_ARG;
+  }
   const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
   const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
-  const int64_t fixed = density_fixed_bytes(p);
-  const int64_t per_band = (int64_t)ncol * nz * cs;
-  QMX_CHECK_ARG(ws_bytes > fixed, "qmx_collect_density: workspace smaller than the density");
+  const int64_t fixed = density_fixed_bytes(p, cross);
+  const int64_t per_band = (int64_t)ncol * nz * cs * n_comp;
+  if (ws_bytes <= fixed || (ws_bytes - fixed) / (per_band * nk) < 1) {
+    set_error(std::string(name) + ": workspace smaller than the density and one band of every k");
+    return QMX_ERR_ARG;
+  }
   int64_t blk = (ws_bytes - fixed) / (per_band * nk);
-  QMX_CHECK_ARG(blk >= 1, "qmx_collect_density: workspace smaller than one band of every k");
   blk = std::min<int64_t>(blk, nb);
   blk = std::min<int64_t>(blk, 65535 / nk);
-  const C2* C = static_cast<const C2*>(Cv);
+  const C2* Cs[2] = {static_cast<const C2*>(Cv), static_cast<const C2*>(Cv1)};
   double* rhoT = static_cast<double*>(ws);
-  hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((p->N + 255) / 256, 4096)), dim3(256), 0,
-                     st, rhoT, (int64_t)p->N);
+  const int64_t n_acc = (int64_t)p->N * n_comp;
+  hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((n_acc + 255) / 256, 4096)), dim3(256), 0,
+                     st, rhoT, n_acc);
   QMX_LAUNCHED();
   const int Mz = max_lines(nz, cs, 0, 64);
   const FastDiv fz = make_fastdiv(Mz);
   const size_t shA = (size_t)(2 * odd_ld(Mz) * nz + nz) * cs;
-  const size_t shP = plane_lds_bytes(p, cs);
   const int ntP = plane_threads(p);
-  const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
-  int rc = ntP == kPlaneThreadsSmall
-               ? set_density_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>(shP)
-               : set_density_lds<C2, R, kPlaneThreads, kPlaneMaxE>(shP);
+  const bool small = ntP == kPlaneThreadsSmall;
+  const int maxeP = small ? kPlaneMaxESmall : kPlaneMaxE;
+  int rc = cross ? (small ? set_density_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall, true>(shP)
+                          : set_density_lds<C2, R, kPlaneThreads, kPlaneMaxE, true>(shP))
+                 : (small ? set_density_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall, false>(shP)
+                          : set_density_lds<C2, R, kPlaneThreads, kPlaneMaxE, false>(shP));
   if (rc) return rc;
-  QMX_CHECK_ARG((int64_t)nx * ny <= (int64_t)ntP * maxeP,
-                "qmx_collect_density: plane larger than the accumulator registers");
+  if ((int64_t)nx * ny > (int64_t)ntP * maxeP) {
+    set_error(std::string(name) + ": plane larger than the accumulator registers");
+    return QMX_ERR_ARG;
+  }
   const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
   const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
   const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
   for (int64_t b0 = 0; b0 < nb; b0 += blk) {
     const int cur = (int)std::min<int64_t>(blk, nb - b0);
-    C2* T1 = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
+    C2* T1[2];
+    T1[0] = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
+    T1[1] = T1[0] + (int64_t)cur * nk * ncol * nz;
     dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
-    hipLaunchKernelGGL((k_sphere_to_cols<C2, R, true>), gA, dim3(kThreads), shA, st, C, ldb, ldk,
-                       T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot,
-                       (int)b0);
-    QMX_LAUNCHED();
+    for (int c = 0; c < n_comp; ++c) {
+      hipLaunchKernelGGL((k_sphere_to_cols<C2, R, true>), gA, dim3(kThreads), shA, st, Cs[c], ldb,
+                         ldk, T1[c], t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                         p->n_tot, (int)b0);
+      QMX_LAUNCHED();
+    }
     // Band groups: enough workgroups to fill the chip, each accumulating
     // several bands in registers.
     const int64_t wg_base = (int64_t)nz * nk;
@@ -993,25 +1055,35 @@ int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int6
     const int bpg = (cur + groups - 1) / groups;
     groups = (cur + bpg - 1) / bpg;
     dim3 gP(nz, groups, nk);
-    if (ntP == kPlaneThreadsSmall)
-      hipLaunchKernelGGL((k_plane_density<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>), gP,
-                         dim3(kPlaneThreadsSmall), shP, st, T1, prefac + k0 * ldp, ldp,
-                         p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1),
-                         rxc, ryc, fny, fnxa, ncol, nxa, cur, (int)b0, bpg, rhoT);
-    else
-      hipLaunchKernelGGL((k_plane_density<C2, R, kPlaneThreads, kPlaneMaxE>), gP,
-                         dim3(kPlaneThreads), shP, st, T1, prefac + k0 * ldp, ldp, p->d_col_ix,
-                         p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), rxc, ryc, fny,
-                         fnxa, ncol, nxa, cur, (int)b0, bpg, rhoT);
+    const double* pf = prefac + k0 * ldp;
+    if (cross) {
+      if (small)
+        launch_density<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall, true>(
+            gP, shP, st, T1[0], T1[1], pf, ldp, p, rxc, ryc, fny, fnxa, cur, (int)b0, bpg, rhoT);
+      else
+        launch_density<C2, R, kPlaneThreads, kPlaneMaxE, true>(
+            gP, shP, st, T1[0], T1[1], pf, ldp, p, rxc, ryc, fny, fnxa, cur, (int)b0, bpg, rhoT);
+    } else {
+      if (small)
+        launch_density<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall, false>(
+            gP, shP, st, T1[0], nullptr, pf, ldp, p, rxc, ryc, fny, fnxa, cur, (int)b0, bpg, rhoT);
+      else
+        launch_density<C2, R, kPlaneThreads, kPlaneMaxE, false>(
+            gP, shP, st, T1[0], nullptr, pf, ldp, p, rxc, ryc, fny, fnxa, cur, (int)b0, bpg, rhoT);
+    }
     QMX_LAUNCHED();
   }
   const int nxy = nx * ny;
   dim3 gT((nxy + 31) / 32, (nz + 31) / 32);
-  if (accumulate)
-    hipLaunchKernelGGL(k_density_unmajor<true>, gT, dim3(256), 0, st, rhoT, rho, nxy, nz);
-  else
-    hipLaunchKernelGGL(k_density_unmajor<false>, gT, dim3(256), 0, st, rhoT, rho, nxy, nz);
-  QMX_LAUNCHED();
+  for (int c = 0; c < n_comp; ++c) {
+    const double* src = rhoT + (int64_t)c * p->N;
+    double* dst = rho + (int64_t)c * p->N;
+    if (accumulate)
+      hipLaunchKernelGGL(k_density_unmajor<true>, gT, dim3(256), 0, st, src, dst, nxy, nz);
+    else
+      hipLaunchKernelGGL(k_density_unmajor<false>, gT, dim3(256), 0, st, src, dst, nxy, nz);
+    QMX_LAUNCHED();
+  }
   return QMX_OK;
 }
 
@@ -1433,35 +1505,84 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
 int64_t qmx_density_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
   if (!plan || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
   const int cs = dtype == QMX_C64 ? 8 : 16;
-  return density_fixed_bytes(plan) + (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+  return density_fixed_bytes(plan, false) +
+         (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+}
+
+int64_t qmx_density_cross_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
+  if (!plan || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
+  const int cs = dtype == QMX_C64 ? 8 : 16;
+  return density_fixed_bytes(plan, true) +
+         2 * (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+}
+
+static int collect_density_entry(const char* name, const qmx_plan* plan, int64_t k0, int64_t nk,
+                                 int dtype, const void* C0, const void* C1, int64_t ldb,
+                                 int64_t ldk, int64_t n_bands, const double* prefac, int64_t ldp,
+                                 double* rho, int accumulate, void* workspace,
+                                 int64_t workspace_bytes, void* stream) {
+  const std::string n(name);
+  if (!plan || !C0 || !prefac || !rho) {
+    set_error(n + ": null argument");
+    return QMX_ERR_ARG;
+  }
+  if (n_bands < 0 || ldp < n_bands) {
+    set_error(n + ": bad n_bands or ldp");
+    return QMX_ERR_ARG;
+  }
+  if (k0 < 0 || nk < 0 || k0 + nk > plan->nk) {
+    set_error(n + ": k range outside plan");
+    return QMX_ERR_ARG;
+  }
+  if (!workspace || workspace_bytes <= 0) {
+    set_error(n + ": workspace required");
+    return QMX_ERR_ARG;
+  }
+  if (ldb < plan->n_tot || (nk > 1 && ldk <= 0)) {
+    set_error(n + ": bad strides");
+    return QMX_ERR_ARG;
+  }
+  if (dtype != QMX_C64 && dtype != QMX_C128) {
+    set_error(n + ": dtype must be QMX_C64 or QMX_C128");
+    return QMX_ERR_ARG;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n_bands == 0 || nk == 0) {
+    if (accumulate) return QMX_OK;
+    const int64_t len = (int64_t)plan->N * (C1 ? 2 : 1);
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((len + 255) / 256, 4096)), dim3(256),
+                       0, st, rho, len);
+    QMX_LAUNCHED();
+    return QMX_OK;
+  }
+  if (dtype == QMX_C64)
+    return collect_density_impl<float>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
+                                       ldp, rho, accumulate, workspace, workspace_bytes, st);
+  return collect_density_impl<double>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
+                                      ldp, rho, accumulate, workspace, workspace_bytes, st);
 }
 
 int qmx_collect_density(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,
                         int64_t ldb, int64_t ldk, int64_t n_bands, const double* prefac,
                         int64_t ldp, double* rho, int accumulate, void* workspace,
                         int64_t workspace_bytes, void* stream) {
-  QMX_CHECK_ARG(plan && C && prefac && rho, "qmx_collect_density: null argument");
-  QMX_CHECK_ARG(n_bands >= 0 && ldp >= n_bands, "qmx_collect_density: bad n_bands or ldp");
-  QMX_CHECK_ARG(k0 >= 0 && nk >= 0 && k0 + nk <= plan->nk,
-                "qmx_collect_density: k range outside plan");
-  QMX_CHECK_ARG(workspace && workspace_bytes > 0, "qmx_collect_density: workspace required");
-  QMX_CHECK_ARG(ldb >= plan->n_tot && (nk <= 1 || ldk > 0), "qmx_collect_density: bad strides");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (n_bands == 0 || nk == 0) {
-    if (accumulate) return QMX_OK;
-    hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((plan->N + 255) / 256, 4096)),
-                       dim3(256), 0, st, rho, (int64_t)plan->N);
-    QMX_LAUNCHED();
-    return QMX_OK;
+  return collect_density_entry("qmx_collect_density", plan, k0, nk, dtype, C, nullptr, ldb, ldk,
+                               n_bands, prefac, ldp, rho, accumulate, workspace, workspace_bytes,
+                               stream);
+}
+
+int qmx_collect_density_cross(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype,
+                              const void* C0, const void* C1, int64_t ldb, int64_t ldk,
+                              int64_t n_bands, const double* prefac, int64_t ldp, double* rho,
+                              int accumulate, void* workspace, int64_t workspace_bytes,
+                              void* stream) {
+  if (!C1) {
+    set_error("qmx_collect_density_cross: null argument");
+    return QMX_ERR_ARG;
   }
-  if (dtype == QMX_C64)
-    return collect_density_impl<float>(plan, (int)k0, (int)nk, C, ldb, ldk, n_bands, prefac, ldp,
-                                       rho, accumulate, workspace, workspace_bytes, st);
-  if (dtype == QMX_C128)
-    return collect_density_impl<double>(plan, (int)k0, (int)nk, C, ldb, ldk, n_bands, prefac,
-                                        ldp, rho, accumulate, workspace, workspace_bytes, st);
-  set_error("qmx_collect_density: dtype must be QMX_C64 or QMX_C128");
-  return QMX_ERR_ARG;
+  return collect_density_entry("qmx_collect_density_cross", plan, k0, nk, dtype, C0, C1, ldb,
+                               ldk, n_bands, prefac, ldp, rho, accumulate, workspace,
+                               workspace_bytes, stream);
 }
 
 int qmx_apply_local(const qmx_plan* plan, int dtype, const void* C, int64_t ldb, int64_t ldk,

@@ -186,6 +186,71 @@ def test_collect_density_spins_blocks_empty(cfg0_el):
         b.w_sk = b.w_sk * 2
 
 
+def _spinor_density_check(b, ob, dtype, n_bands, seed):
+    """need_Mvec = True and False on random spinor bands against the oracle:
+    n = rho_0 + rho_1, M_x = 2 Re X, M_y = 2 Im X, M_z = rho_0 - rho_1."""
+    from oracle.density import collect_density as o_density
+    from oracle.density import collect_density_cross as o_cross
+    C0, C1 = rand_coeff(ob, n_bands, seed=seed), rand_coeff(ob, n_bands, seed=seed + 1)
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0.0, 1.0, (len(ob.k), n_bands))
+    f[:, 1] = 0.0  # an empty band
+    coeff = torch.cat([to_wf(b, C0, dtype).coeff, to_wf(b, C1, dtype).coeff], dim=3)
+    r0, r1 = o_density(ob, C0, f), o_density(ob, C1, f)
+    X = o_cross(ob, C0, C1, f)
+    tol = 1e-12 if dtype == C128 else 1e-5
+    got = b.collect_density(coeff, torch.tensor(f)[None], need_Mvec=True)
+    assert got.shape == (4,) + ob.shape and got.dtype == torch.float64
+    peak = np.abs(r0 + r1).max()
+    for i, want in enumerate((r0 + r1, 2 * X.real, 2 * X.imag, r0 - r1)):
+        err = np.abs(got[i].cpu().numpy() - want).max() / peak
+        print(f"[spinor density] component {i}: {err:.2e}", flush=True)
+        assert err < tol
+    plain = b.collect_density(coeff, torch.tensor(f)[None])
+    assert plain.shape == (1,) + ob.shape
+    assert np.abs(plain[0].cpu().numpy() - (r0 + r1)).max() / peak < tol
+    # one band with spin along +x, (phi, phi) / sqrt 2, then +y, (phi, i phi) / sqrt 2
+    phi = C0[:, :1] / np.sqrt(2)
+    fx = np.zeros((len(ob.k), 1))
+    fx[0, 0] = 1.0
+    for spin, axis in ((1.0, 1), (1j, 2)):
+        cs = torch.cat([to_wf(b, phi, dtype).coeff, to_wf(b, spin * phi, dtype).coeff], dim=3)
+        m = b.collect_density(cs, torch.tensor(fx)[None], need_Mvec=True).cpu().numpy()
+        for i in (1, 2, 3):
+            want = m[0] if i == axis else 0.0
+            assert np.abs(m[i] - want).max() < tol * np.abs(m[0]).max()
+
+
+@pytest.mark.parametrize("g", [24, 48, 60])
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_collect_density_spinor(gpu, g, dtype):
+    """Spinor density with and without need_Mvec (qmx_collect_density per
+    component, qmx_collect_density_cross for M_x, M_y) against the oracle,
+    complex128 to 1e-12 and complex64 to 1e-5 of the peak density; a band
+    polarised along +x or +y gives |M| = n along that axis only."""
+    spec = _cell_with_box(g)
+    print(f"[collect_density spinor] {g}^3 {dtype}", flush=True)
+    ob = OBasis(spec)
+    el = electrons_from_spec(spec)
+    _spinor_density_check(el.basis, ob, dtype, 5, seed=g + 3)
+
+
+def test_collect_density_spinor_large_plane(gpu):
+    """96^3 box (512-thread plane kernels): the complex64 cross term against
+    the oracle; complex128 needs two 96 x 97 planes of LDS (298 KB) and the
+    cross term is refused with an error, before any launch."""
+    spec = systems.make_cell(1)
+    print("[collect_density spinor] 96^3", flush=True)
+    ob = OBasis(spec)
+    el = electrons_from_spec(spec)
+    b = el.basis
+    _spinor_density_check(b, ob, C64, 3, seed=11)
+    C = rand_coeff(ob, 2, seed=5)
+    coeff = torch.cat([to_wf(b, C, C128).coeff] * 2, dim=3)
+    with pytest.raises(RuntimeError, match="two x-y planes"):
+        b.collect_density(coeff, torch.ones(1, len(ob.k), 2, dtype=torch.float64), need_Mvec=True)
+
+
 def test_apply_local_blocking_inplace_accumulate(cfg0_el):
     """Tiny workspace (many band blocks), out == C, and accumulate."""
     spec, ob, el = cfg0_el

@@ -294,3 +294,40 @@ def test_density_closed_forms(small):
     f1[0, 1] = 1.0
     want = b.w_sk[0] / b.volume * np.abs(psi) ** 2
     assert np.abs(collect_density(b, C, f1).reshape(-1) - want).max() < 1e-12 * want.max()
+
+
+def test_density_cross_identities(small):
+    """Oracle spinor cross term X = sum f w / V psi_1 conj(psi_0): the
+    polarization identities rho(C0 + C1) - rho(C0) - rho(C1) = 2 Re X and
+    rho(C0 + i C1) - rho(C0) - rho(C1) = -2 Im X against the pinned density;
+    plane waves a (component 0) and c (component 1) give
+    c conj(a) exp(2 pi i (m1 - m0) . r / S)."""
+    from oracle.density import collect_density, collect_density_cross
+
+    spec, b = small
+    K, G = b.fft_index.shape
+    rng = np.random.default_rng(4)
+    C0 = rng.standard_normal((K, 3, G)) + 1j * rng.standard_normal((K, 3, G))
+    C1 = rng.standard_normal((K, 3, G)) + 1j * rng.standard_normal((K, 3, G))
+    b.constrain(C0)
+    b.constrain(C1)
+    f = rng.uniform(0.0, 1.0, (K, 3))
+    f[1, 0] = 0.0
+    X = collect_density_cross(b, C0, C1, f)
+    r0, r1 = collect_density(b, C0, f), collect_density(b, C1, f)
+    scale = np.abs(r0).max()
+    assert np.abs(collect_density(b, C0 + C1, f) - r0 - r1 - 2 * X.real).max() < 1e-12 * scale
+    assert np.abs(collect_density(b, C0 + 1j * C1, f) - r0 - r1 + 2 * X.imag).max() < 1e-12 * scale
+    # swapping the components conjugates X
+    assert np.abs(collect_density_cross(b, C1, C0, f) - X.conj()).max() < 1e-14 * scale
+    # plane waves: component 0 at sphere point 2, component 1 at point 6, k 0
+    a, c = 0.3 - 0.5j, 0.8 + 0.1j
+    P0, P1 = np.zeros((K, 1, G), complex), np.zeros((K, 1, G), complex)
+    P0[0, 0, 2], P1[0, 0, 6] = a, c
+    fp = np.zeros((K, 1))
+    fp[0, 0] = 0.75
+    shape = np.array(b.shape)
+    r = np.stack(np.meshgrid(*[np.arange(s) for s in b.shape], indexing="ij"), -1)
+    phase = np.exp(2j * np.pi * ((b.iG[0, 6] - b.iG[0, 2]) * r / shape).sum(-1))
+    want = 0.75 * b.w_sk[0] / b.volume * c * np.conj(a) * phase
+    assert np.abs(collect_density_cross(b, P0, P1, fp) - want).max() < 1e-14
