@@ -219,6 +219,43 @@ def density_measurement(spec) -> dict:
     return res
 
 
+def projectors_measurement(spec) -> dict:
+    """qmx_projectors on the 64-atom cell: the cell's Gaussian radials
+    sampled at spacing 0.02 as spline radial functions, HIP events around
+    ten calls. Algorithmic bytes: the complex128 output, 16 bytes per
+    (k, atom, projector, plane wave) entry (the inputs are a few MB)."""
+    import torch
+
+    from qimpy_amd.electrons import electrons_from_spec
+    from qimpy_amd.ions import RadialSet, spline_projectors
+
+    el = electrons_from_spec(spec)
+    b = el.basis
+    dG = 0.02
+    G = np.arange(int(np.ceil(np.sqrt(2 * spec.ke_cutoff) / dG)) + 10) * dG
+    l_rf = [l for rc in spec.proj_rc for l in spec.proj_l]
+    radial = RadialSet(l_rf, np.stack([np.exp(-0.5 * rc**2 * G**2)
+                                       for rc in spec.proj_rc for _ in spec.proj_l]), dG)
+    pos = torch.tensor(spec.positions, device="cuda")
+    beta = spline_projectors(b, pos, radial)
+    err = float((beta - el.ions.beta.coeff[0, :, :, 0]).abs().max()
+                / el.ions.beta.coeff.abs().max())
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(10):
+        beta = spline_projectors(b, pos, radial)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 10 * 1e-3
+    by = beta.numel() * 16
+    return {"workload": f"64-atom cell, {b.grid.shape[0]}^3 box, {b.nk} k, n_tot {b.n_tot}, "
+                        f"{beta.shape[1]} projectors (l <= {max(l_rf)})",
+            "ms_per_call": round(t * 1e3, 3), "GBps": round(by / t / 1e9, 1),
+            "frac": round(by / t / 1e9 / HBM_PEAK_GBS, 4), "err_vs_analytic": err}
+
+
 def chefsi_measurement(spec, residual: float = 3e-5) -> dict:
     """One CheFSI solve of the 256-atom cell (filter order 20, ten outer
     iterations at most) in each precision, plain FP64 first. The mixed form
@@ -387,6 +424,8 @@ def main():
         print(f"[bench] single 64-atom cell: mixed {v1m:.3f}, plain {v1p:.3f} solves/s", flush=True)
         out["collect_density"] = density_measurement(spec1)
         print(f"[bench] collect_density: {out['collect_density']}", flush=True)
+        out["projectors"] = projectors_measurement(spec1)
+        print(f"[bench] projectors: {out['projectors']}", flush=True)
     if args.chefsi_256:
         out["chefsi_256_atom_cell"] = chefsi_measurement(systems.make_cell(4))
         print(f"[bench] CheFSI 256-atom cell: {out['chefsi_256_atom_cell']}", flush=True)

@@ -151,6 +151,29 @@ int qmx_collect_density_cross(const qmx_plan* plan, int64_t k0, int64_t nk, int 
                               int accumulate, void* workspace, int64_t workspace_bytes,
                               void* stream);
 
+/* Projectors of one species on the plane waves of a basis
+ * (Ions.get_projectors, dft/ions/_ions_projectors.py:14-72):
+ *   out[k * ldk + (atom * n_proj + p) * ldp + g] =
+ *     R_{i_rf[p]}(|G+k|) (i^l r^l Y_lm)(G+k) exp(-2 pi i (iG+k) . tau_atom) scale
+ * for g < n[k], 0 for the padding g in [n[k], n_tot). G + k = Gbasis (iG + k)
+ * (Gbasis host double [3][3] row-major, Cartesian rows); iG device int32
+ * [nk][n_tot][3]; k device double [nk][3] fractional; n device int32 [nk];
+ * positions device double [n_atoms][3] fractional. Projector p has
+ * angular momentum l[p] <= 6, -l <= m[p] <= l, radial function i_rf[p]
+ * (host int32 arrays, n_proj <= 32), with real solid harmonics ordered as in
+ * math/spherical_harmonics.py:53-137. R is the quintic blip spline of
+ * quintic_spline.Interpolator (math/quintic_spline.py:56-128): coeff device
+ * double [n_coeff][n_rf] (the (N + 4, n_rf) output of get_coeff) on node
+ * spacing dG; every |G+k| <= G_max must lie on the table
+ * (floor(G_max / dG) + 5 < n_coeff). scale: 1/sqrt(volume) (:48-49).
+ * out: device complex128. */
+int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
+                   const int32_t* n, const double* Gbasis, int64_t n_atoms,
+                   const double* positions, int64_t n_proj, const int32_t* l, const int32_t* m,
+                   const int32_t* i_rf, int64_t n_rf, int64_t n_coeff, const double* coeff,
+                   double dG, double G_max, double scale, void* out, int64_t ldp, int64_t ldk,
+                   void* stream);
+
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
  * The result is multiplied by `scale` (1/N for QimPy's forward transform,

@@ -1,0 +1,206 @@
+// Projector generation on the device (Ions.get_projectors,
+// dft/ions/_ions_projectors.py:14-72):
+//   beta[k][atom * n_proj + p][g] = R_{i_rf[p]}(|G + k|) * (i^l Y_lm)(G + k)
+//                                   * exp(-2 pi i (iG + k) . tau_atom) * scale
+// with R a quintic blip spline (math/quintic_spline.py:56-128) and Y_lm the
+// real solid harmonics (math/spherical_harmonics.py:53-137). One thread per
+// plane wave g of k-point blockIdx.y: it evaluates the radial functions and
+// harmonics of its G + k once into LDS slots of its own, then writes every
+// (atom, projector) entry, consecutive threads on consecutive g.
+//
+// Harmonics: the associated Legendre recurrence in Cartesian form. With
+// Q_l^m = r^l P_l^m(cos t) / (r sin t)^m (no Condon-Shortley phase),
+//   Q_m^m = (2m - 1)!!,  Q_l^m = ((2l - 1) z Q_{l-1}^m - (l + m - 1) r^2 Q_{l-2}^m) / (l - m),
+// and r^l Y_lm = N_lm Q_l^|m| * {Re, 1, Im}((x + i y)^|m|) for m {>, =, <} 0,
+// N_lm = sqrt((2l + 1) / (4 pi) (l - |m|)! / (l + |m|)!) (times sqrt 2 for
+// m != 0); this is the convention of the reference's tables (real part at
+// index l + m, imaginary part at l - m, m > 0, of (-1)^m r^l Y_l^m).
+//
+// Spline: the blip centred on node c - 2 has coefficient c; on the interval
+// [i, i + 1) of t = |G + k| / dG, with f = t - i, coefficient i + j carries
+// the weight w_j(f) = (1/66) sum_{q <= 5 - j} (-1)^q C(6, q) (f + 5 - j - q)^5
+// (the cardinal quintic B-spline in truncated-power form, scaled by 120/66 so
+// that a blip is 1 at its centre), expanded on the host into a polynomial in f.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "qmx_common.h"
+
+namespace {
+
+constexpr int kProjThreads = 128;
+constexpr int kProjMax = 32;  // projectors per atom of one species (2 per l up to l = 3)
+constexpr int kLMax = 6;
+
+struct ProjDesc {
+  int n_proj;
+  signed char l[kProjMax];
+  signed char m[kProjMax];
+  signed char i_rf[kProjMax];
+  double norm[kProjMax];  // N_lm, sqrt 2 included for m != 0
+  double w[6][6];         // w_j(f) = sum_p w[j][p] f^p
+  double G[9];            // Cartesian (G + k)_a = sum_b G[3 a + b] (iG + k)_b
+};
+
+__device__ __forceinline__ double solid_harmonic(int l, int m, double x, double y, double z,
+                                                 double r2) {
+  const int am = m < 0 ? -m : m;
+  double cr = 1.0, ci = 0.0;  // (x + i y)^|m|
+  for (int q = 0; q < am; ++q) {
+    const double t = cr * x - ci * y;
+    ci = cr * y + ci * x;
+    cr = t;
+  }
+  double q1 = 1.0;  // Q_|m|^|m| = (2|m| - 1)!!
+  for (int q = 1; q <= am; ++q) q1 *= (double)(2 * q - 1);
+  double q0 = 0.0;
+  for (int ll = am + 1; ll <= l; ++ll) {
+    const double qn = ((2 * ll - 1) * z * q1 - (ll + am - 1) * r2 * q0) / (double)(ll - am);
+    q0 = q1;
+    q1 = qn;
+  }
+  return q1 * (m > 0 ? cr : (m < 0 ? ci : 1.0));
+}
+
+__global__ void __launch_bounds__(kProjThreads)
+    k_projectors(const int* __restrict__ iG, const double* __restrict__ kvec,
+                 const int* __restrict__ n, const double* __restrict__ pos, int n_atoms,
+                 const double* __restrict__ coeff, int n_coeff, int n_rf, double inv_dG,
+                 double scale, ProjDesc d, double2* __restrict__ out, int64_t ldp, int64_t ldk,
+                 int n_tot) {
+  __shared__ double val[kProjMax][kProjThreads];
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x * kProjThreads + tid;
+  const int k = blockIdx.y;
+  if (g >= n_tot) return;
+  double2* o = out + (int64_t)k * ldk + g;
+  const int np = d.n_proj;
+  if (g >= n[k]) {  // padding
+    for (int a = 0; a < n_atoms; ++a)
+      for (int p = 0; p < np; ++p) o[(int64_t)(a * np + p) * ldp] = double2{0.0, 0.0};
+    return;
+  }
+  double f[3];
+  for (int c = 0; c < 3; ++c) f[c] = (double)iG[((int64_t)k * n_tot + g) * 3 + c] + kvec[k * 3 + c];
+  double Gc[3];
+  for (int a = 0; a < 3; ++a) Gc[a] = d.G[3 * a] * f[0] + d.G[3 * a + 1] * f[1] + d.G[3 * a + 2] * f[2];
+  const double r2 = Gc[0] * Gc[0] + Gc[1] * Gc[1] + Gc[2] * Gc[2];
+  const double t = sqrt(r2) * inv_dG;
+  int i = (int)floor(t);
+  double fr = t - i;
+  if (i > n_coeff - 6) {  // beyond the table (the host checks G_max): clamp, stay in bounds
+    i = n_coeff - 6;
+    fr = 1.0;
+  }
+  double w[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s = d.w[j][5];
+#pragma unroll
+    for (int q = 4; q >= 0; --q) s = s * fr + d.w[j][q];
+    w[j] = s;
+  }
+  for (int p = 0; p < np; ++p) {
+    const double* cf = coeff + (int64_t)i * n_rf + d.i_rf[p];
+    double R = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) R += w[j] * cf[(int64_t)j * n_rf];
+    val[p][tid] = scale * d.norm[p] * R * solid_harmonic(d.l[p], d.m[p], Gc[0], Gc[1], Gc[2], r2);
+  }
+  for (int a = 0; a < n_atoms; ++a) {
+    const double dot = f[0] * pos[3 * a] + f[1] * pos[3 * a + 1] + f[2] * pos[3 * a + 2];
+    double s, c;
+    sincospi(-2.0 * dot, &s, &c);  // exp(-2 pi i (iG + k) . tau)
+    for (int p = 0; p < np; ++p) {
+      const double v = val[p][tid];
+      double2 e;
+      switch (d.l[p] & 3) {  // i^l (c + i s)
+        case 0: e = double2{c, s}; break;
+        case 1: e = double2{-s, c}; break;
+        case 2: e = double2{-c, -s}; break;
+        default: e = double2{s, -c}; break;
+      }
+      o[(int64_t)(a * np + p) * ldp] = double2{v * e.x, v * e.y};
+    }
+  }
+}
+
+double factorial(int q) {
+  double r = 1.0;
+  for (int i = 2; i <= q; ++i) r *= i;
+  return r;
+}
+
+// w[j][p]: coefficient of f^p in (1/66) sum_{q <= 5 - j} (-1)^q C(6, q) (f + 5 - j - q)^5.
+void blip_polynomials(double w[6][6]) {
+  static const double C6[7] = {1, 6, 15, 20, 15, 6, 1};
+  static const double C5[6] = {1, 5, 10, 10, 5, 1};
+  for (int j = 0; j < 6; ++j) {
+    for (int p = 0; p < 6; ++p) w[j][p] = 0.0;
+    for (int q = 0; q <= 5 - j; ++q) {
+      const double sgn = (q & 1) ? -1.0 : 1.0;
+      const double s = 5 - j - q;  // (f + s)^5 = sum_p C(5, p) f^p s^(5 - p)
+      for (int p = 0; p < 6; ++p) w[j][p] += sgn * C6[q] * C5[p] * std::pow(s, 5 - p) / 66.0;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
+                              const int32_t* n, const double* Gbasis, int64_t n_atoms,
+                              const double* positions, int64_t n_proj, const int32_t* l,
+                              const int32_t* m, const int32_t* i_rf, int64_t n_rf,
+                              int64_t n_coeff, const double* coeff, double dG, double G_max,
+                              double scale, void* out, int64_t ldp, int64_t ldk, void* stream) {
+  using qmx::set_error;
+  if (!iG || !k || !n || !Gbasis || !l || !m || !i_rf || !coeff || !out ||
+      (n_atoms > 0 && !positions)) {
+    set_error("qmx_projectors: null argument");
+    return QMX_ERR_ARG;
+  }
+  if (nk < 0 || nk > 65535 || n_tot < 0 || n_tot > (1 << 30) || n_atoms < 0 || n_proj < 0 ||
+      n_rf < 1 || n_rf > 127) {
+    set_error("qmx_projectors: bad sizes");
+    return QMX_ERR_ARG;
+  }
+  if (n_proj > kProjMax) {
+    set_error("qmx_projectors: more than 32 projectors per atom");
+    return QMX_ERR_ARG;
+  }
+  if (!(dG > 0.0) || !(G_max >= 0.0) || n_coeff < 6 || std::floor(G_max / dG) + 5 >= n_coeff) {
+    set_error("qmx_projectors: the radial table does not reach G_max");
+    return QMX_ERR_ARG;
+  }
+  if (ldp < n_tot || (nk > 1 && ldk < n_atoms * n_proj * ldp)) {
+    set_error("qmx_projectors: bad strides");
+    return QMX_ERR_ARG;
+  }
+  ProjDesc d{};
+  d.n_proj = (int)n_proj;
+  for (int p = 0; p < n_proj; ++p) {
+    const int lp = l[p], mp = m[p];
+    if (lp < 0 || lp > kLMax || mp < -lp || mp > lp || i_rf[p] < 0 || i_rf[p] >= n_rf) {
+      set_error("qmx_projectors: bad (l, m, i_rf) of a projector");
+      return QMX_ERR_ARG;
+    }
+    const int am = mp < 0 ? -mp : mp;
+    d.l[p] = (signed char)lp;
+    d.m[p] = (signed char)mp;
+    d.i_rf[p] = (signed char)i_rf[p];
+    d.norm[p] = std::sqrt((2 * lp + 1) / (4.0 * M_PI) * factorial(lp - am) / factorial(lp + am)) *
+                (mp != 0 ? std::sqrt(2.0) : 1.0);
+  }
+  blip_polynomials(d.w);
+  for (int i = 0; i < 9; ++i) d.G[i] = Gbasis[i];
+  if (nk == 0 || n_tot == 0 || n_atoms == 0 || n_proj == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)((n_tot + kProjThreads - 1) / kProjThreads), (unsigned)nk);
+  hipLaunchKernelGGL(k_projectors, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
+                     (int)n_atoms, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
+                     static_cast<double2*>(out), ldp, ldk, (int)n_tot);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}

@@ -1,5 +1,9 @@
 """Nonlocal projectors and couplings consumed by H (reference: dft/ions).
 
+Two producers: ``spline_projectors`` (qmx_projectors) for sampled radial
+functions of any l <= 6, as get_projectors with pseudopotential radials, and
+``build_projectors`` for the Gaussian radials of the synthetic cells.
+
 The band solve consumes ``ions.beta`` (a Wavefunction (1, K, P, 1, G)),
 ``ions.D_all`` (P x P) and the ``beta_version`` cache key that invalidates
 ``Wavefunction.proj`` (_wavefunction.py:106-132). Projectors are built on the
@@ -15,7 +19,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import rc
+from . import _lib, rc
 
 Y00 = 0.5 / np.sqrt(np.pi)
 Y1 = np.sqrt(3.0 / (4.0 * np.pi))
@@ -56,15 +60,108 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
     return proj.masked_fill_(pad, 0.0)
 
 
+DG_DEFAULT = 0.02  #: node spacing of the radial splines (math/_radial_function.py:22)
+
+
+def quintic_coeff(samples: np.ndarray) -> np.ndarray:
+    """Blip coefficients (N + 4, ...) of samples (..., N) on a uniform grid
+    (quintic_spline.get_coeff, math/quintic_spline.py:12-53): the
+    pentadiagonal system of blip values 1, 13/33, 1/66 at 0, 1 and 2 nodes,
+    mirror-symmetric at the left end (c[-j] = c[j]) and natural at the right
+    end (c[N], c[N+1] extrapolated so that the third and fourth derivatives
+    vanish), then the two padding coefficients on each side. A host set-up
+    step, once per radial function set, as in the reference."""
+    from scipy.linalg import solve_banded
+
+    samples = np.asarray(samples, dtype=np.float64)
+    N = samples.shape[-1]
+    if N < 4:
+        raise ValueError("a radial spline needs at least 4 samples")
+    near, far = 13.0 / 33.0, 1.0 / 66.0
+    A = np.zeros((5, N))  # A[2 + i - j, j] = matrix element (i, j)
+    A[0, :], A[1, :], A[2, :], A[3, :], A[4, :] = far, near, 1.0, near, far
+    A[2, 1] += far  # row 1: c[-1] = c[1]
+    A[1, 1] += near  # row 0: c[-1] = c[1]
+    A[0, 2] += far  # row 0: c[-2] = c[2]
+    ext = np.array([[1.0, -3.0, 3.0], [3.0, -8.0, 6.0]])  # c[N], c[N+1] from c[N-3:N]
+    for j in range(3):
+        col = N - 3 + j
+        A[2 + (N - 2) - col, col] += far * ext[0, j]
+        A[2 + (N - 1) - col, col] += near * ext[0, j] + far * ext[1, j]
+    c = solve_banded((2, 2), A, samples.reshape(-1, N).T)
+    c = np.concatenate([c[2:0:-1], c, ext @ c[N - 3:]], axis=0)
+    return c.reshape((N + 4,) + samples.shape[:-1])
+
+
+class RadialSet:
+    """Radial functions of one species on a uniform reciprocal grid (the
+    f_tilde of RadialFunction, math/_radial_function.py:11-123, with G^l
+    removed) as blip coefficients on the device, and the angular momentum
+    of each; projectors follow the pseudo quantum numbers
+    (dft/ions/_pseudo_quantum_numbers.py:28-56): per radial function,
+    m = -l..l."""
+
+    def __init__(self, l_rf, f_tilde: np.ndarray, dG: float = DG_DEFAULT):
+        self.l_rf = tuple(int(l) for l in l_rf)
+        f_tilde = np.atleast_2d(np.asarray(f_tilde, dtype=np.float64))
+        if f_tilde.shape[0] != len(self.l_rf):
+            raise ValueError("one sampled radial function per l")
+        self.dG = float(dG)
+        self.G_max = (f_tilde.shape[1] - 1) * self.dG  # last node
+        coeff = np.ascontiguousarray(quintic_coeff(f_tilde))  # (N + 4, n_rf)
+        self.coeff = torch.tensor(coeff, device=rc.device)
+        self.l = np.array([l for l in self.l_rf for _ in range(2 * l + 1)], dtype=np.int32)
+        self.m = np.array([m for l in self.l_rf for m in range(-l, l + 1)], dtype=np.int32)
+        self.i_rf = np.array([i for i, l in enumerate(self.l_rf) for _ in range(2 * l + 1)],
+                             dtype=np.int32)
+
+    @property
+    def n_proj(self) -> int:
+        return len(self.l)
+
+
+def spline_projectors(basis, positions: torch.Tensor, radial: RadialSet) -> torch.Tensor:
+    """beta (K, n_atoms * n_proj, n_tot) complex128 of one species on
+    `basis` (qmx_projectors; _ions_projectors.py:28-72), padding zeroed."""
+    K, G = basis.nk, basis.n_tot
+    A = int(positions.shape[0])
+    out = torch.empty((K, A * radial.n_proj, G), dtype=torch.complex128, device=rc.device)
+    if A == 0 or radial.n_proj == 0 or K == 0:
+        return out.zero_()
+    iG = getattr(basis, "_iG_i32", None)
+    if iG is None:
+        iG = basis._iG_i32 = basis.iG.to(torch.int32).contiguous()
+    pos = positions.to(device=rc.device, dtype=torch.float64).contiguous()
+    Gb = np.ascontiguousarray(np.asarray(basis.Gbasis, dtype=np.float64))
+    G_max = float(np.sqrt(2.0 * basis.ke_cutoff))  # the sphere: |G + k|^2 / 2 < ke_cutoff
+    if G_max > radial.G_max - 4 * radial.dG:
+        raise ValueError(f"radial functions sampled to G = {radial.G_max:g}, the basis needs "
+                         f"{G_max:g} (plus four nodes)")
+    c_i32 = lambda a: a.ctypes.data_as(_lib.c_i32p)
+    _lib.call("qmx_projectors", K, G, iG.data_ptr(), basis.k.data_ptr(), basis.n_i32.data_ptr(),
+              Gb.ctypes.data_as(_lib.c_f64p), A, pos.data_ptr(), radial.n_proj, c_i32(radial.l),
+              c_i32(radial.m), c_i32(radial.i_rf), len(radial.l_rf), radial.coeff.shape[0],
+              radial.coeff.data_ptr(), radial.dG, G_max, 1.0 / np.sqrt(basis.volume),
+              out.data_ptr(), out.stride(1), out.stride(0), rc.stream_ptr())
+    return out
+
+
 class Ions:
     beta_version: int
 
-    def __init__(self, positions: np.ndarray, proj_rc, proj_l, D: np.ndarray):
+    def __init__(self, positions: np.ndarray, proj_rc, proj_l, D: np.ndarray,
+                 radial: "RadialSet | None" = None):
+        """Gaussian radials exp(-rc^2 G^2 / 2) for every (rc, l) of proj_rc x
+        proj_l (l <= 1), or, with `radial`, the sampled radial functions of a
+        species (l <= 6) through qmx_projectors; D: the diagonal couplings."""
         self.positions = torch.tensor(np.asarray(positions, dtype=np.float64), device=rc.device)
+        self.radial = radial
         self.proj_rc = tuple(float(r) for r in proj_rc)
         self.proj_l = tuple(int(l) for l in proj_l)
+        if radial is not None and (self.proj_rc or self.proj_l):
+            raise ValueError("give either Gaussian radials or a RadialSet, not both")
         if any(l > 1 for l in self.proj_l):
-            raise NotImplementedError("projectors with l > 1 are not on this path")
+            raise NotImplementedError("Gaussian projectors with l > 1 are not on this path")
         self.D_diag = torch.tensor(np.asarray(D, dtype=np.float64), device=rc.device)
         self.D_all = torch.diag(self.D_diag).to(torch.complex128)
         self.beta = None
@@ -78,6 +175,8 @@ class Ions:
 
     @property
     def n_projectors(self) -> int:
+        if self.radial is not None:
+            return self.n_atoms * self.radial.n_proj
         return self.n_atoms * len(self.proj_rc) * sum(2 * l + 1 for l in self.proj_l)
 
     def translation_phase(self, iG: torch.Tensor) -> torch.Tensor:
@@ -89,8 +188,11 @@ class Ions:
         """(Re)build beta on `basis` and bump beta_version."""
         from .wavefunction import Wavefunction
 
-        proj = build_projectors(self.positions, self.proj_rc, self.proj_l, basis.iG, basis.k,
-                                basis.Gbasis, basis.volume, basis.n)
+        if self.radial is not None:
+            proj = spline_projectors(basis, self.positions, self.radial)
+        else:
+            proj = build_projectors(self.positions, self.proj_rc, self.proj_l, basis.iG, basis.k,
+                                    basis.Gbasis, basis.volume, basis.n)
         self.beta = Wavefunction(basis, coeff=proj[None, :, :, None, :].contiguous())
         self.beta_version += 1
         self._beta_cast = {}

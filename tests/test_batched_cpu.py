@@ -63,3 +63,19 @@ def test_band_classes():
     assert choose_band_classes(demand, overhead_bands=0.0) == [16, 24, 40, 72]
     assert choose_band_classes(demand, overhead_bands=1e6) == [72]
     assert choose_band_classes([]) == []
+
+
+def test_quintic_coeff_matches_oracle():
+    """The host spline fit of the projector radials (ions.quintic_coeff)
+    equals the oracle's restatement of get_coeff, batched and single, and
+    refuses fewer than four samples."""
+    from oracle.projectors import quintic_coeff as o_coeff
+    from qimpy_amd.ions import quintic_coeff
+
+    rng = np.random.default_rng(2)
+    G = np.arange(60) * 0.02
+    samples = np.stack([np.exp(-G**2 * a) * np.cos(b * G) for a, b in rng.uniform(0.5, 3, (4, 2))])
+    assert np.abs(quintic_coeff(samples) - o_coeff(samples)).max() < 1e-13
+    assert np.abs(quintic_coeff(samples[1]) - o_coeff(samples)[:, 1]).max() < 1e-13
+    with pytest.raises(ValueError):
+        quintic_coeff(samples[:, :3])

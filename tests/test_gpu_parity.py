@@ -619,3 +619,66 @@ def test_dot128_matches_fp64_reference(gpu, dtype, shape):
     lazy = dot128(Ad[:, :M].conj(), Ad[:, M:])  # torch's lazy conjugate view
     ref_c = torch.matmul(A[:, :M], A[:, M:].transpose(-2, -1))
     assert (lazy.cpu() - ref_c).abs().max().item() <= tol * scale
+
+
+def _radial_samples(l_rf, n_G, dG, seed):
+    """Smooth radial functions (Gaussian times cosine) sampled on the nodes."""
+    rng = np.random.default_rng(seed)
+    G = np.arange(n_G) * dG
+    return np.stack([np.exp(-0.5 * a * G**2) * np.cos(b * G)
+                     for a, b in rng.uniform([0.5, 0.2], [2.0, 1.5], (len(l_rf), 2))])
+
+
+@pytest.mark.parametrize("l_rf", [(0, 1, 2, 3, 2), (4, 6), (5,)])
+def test_spline_projectors_vs_oracle(gpu, l_rf):
+    """qmx_projectors (spline radials, real solid harmonics up to l = 6,
+    translation phases, padding) against the oracle's projectors built from
+    scipy's harmonics and B-splines on the same blip coefficients: 1e-12
+    of the largest entry."""
+    from oracle.projectors import projectors as o_spline_projectors
+    from oracle.projectors import quintic_coeff as o_coeff
+    from qimpy_amd.ions import RadialSet, spline_projectors
+
+    spec = _cell_with_box(24)
+    print(f"[spline projectors] l {l_rf}", flush=True)
+    ob = OBasis(spec)
+    el = electrons_from_spec(spec)
+    dG = 0.02
+    n_G = int(np.ceil(np.sqrt(2 * spec.ke_cutoff) / dG)) + 10
+    f = _radial_samples(l_rf, n_G, dG, seed=len(l_rf))
+    radial = RadialSet(l_rf, f, dG)
+    got = spline_projectors(el.basis, torch.tensor(spec.positions), radial).cpu().numpy()
+    want = o_spline_projectors(ob, spec.positions, o_coeff(f), dG, l_rf, spec.volume)
+    assert got.shape == want.shape
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"[spline projectors] max err {err:.2e}", flush=True)
+    assert err < 1e-12
+    pad = np.broadcast_to(ob.pad_mask[:, None, :], got.shape)
+    assert np.all(got[pad] == 0)
+
+
+def test_spline_ions_reproduce_gaussian_projectors(gpu):
+    """Ions with a RadialSet of sampled Gaussians (spacing 0.02) give the
+    analytic Gaussian projectors of the synthetic cell to 1e-8, and the
+    library refuses tables short of the sphere and more than 32 projectors."""
+    from qimpy_amd.ions import Ions, RadialSet, spline_projectors
+
+    spec = _cell_with_box(30)
+    el = electrons_from_spec(spec)
+    b = el.basis
+    dG = 0.02
+    G = np.arange(int(np.ceil(np.sqrt(2 * spec.ke_cutoff) / dG)) + 10) * dG
+    l_rf = [l for rc in spec.proj_rc for l in spec.proj_l]
+    f = np.stack([np.exp(-0.5 * rc**2 * G**2) for rc in spec.proj_rc for _ in spec.proj_l])
+    ions = Ions(spec.positions, (), (), spec.D, radial=RadialSet(l_rf, f, dG))
+    ions.update(b)
+    want = el.ions.beta.coeff
+    assert ions.n_projectors == el.ions.n_projectors
+    err = float((ions.beta.coeff - want).abs().max() / want.abs().max())
+    print(f"[spline ions] max err vs Gaussian {err:.2e}", flush=True)
+    assert err < 1e-8
+    with pytest.raises(ValueError, match="sampled to"):
+        spline_projectors(b, torch.tensor(spec.positions), RadialSet(l_rf, f[:, :20], dG))
+    with pytest.raises(RuntimeError, match="more than 32"):
+        spline_projectors(b, torch.tensor(spec.positions),
+                          RadialSet((3,) * 5, np.repeat(f[:1], 5, axis=0), dG))

@@ -331,3 +331,81 @@ def test_density_cross_identities(small):
     phase = np.exp(2j * np.pi * ((b.iG[0, 6] - b.iG[0, 2]) * r / shape).sum(-1))
     want = 0.75 * b.w_sk[0] / b.volume * c * np.conj(a) * phase
     assert np.abs(collect_density_cross(b, P0, P1, fp) - want).max() < 1e-14
+
+
+def test_harmonics_closed_forms_and_orthonormality():
+    """Real solid harmonics: closed forms for l <= 2 (index l^2 + l + m,
+    (y, z, x) at l = 1) and orthonormality over the unit sphere by
+    Gauss-Legendre in cos(theta) times a uniform phi grid (exact for l <= 3)."""
+    from oracle.projectors import real_solid_harmonics
+
+    rng = np.random.default_rng(0)
+    r = rng.standard_normal((50, 3))
+    x, y, z = r[:, 0], r[:, 1], r[:, 2]
+    r2 = (r**2).sum(-1)
+    pi = np.pi
+    want = [np.full_like(x, 0.5 / np.sqrt(pi)),
+            np.sqrt(3 / (4 * pi)) * y, np.sqrt(3 / (4 * pi)) * z, np.sqrt(3 / (4 * pi)) * x,
+            np.sqrt(15 / (4 * pi)) * x * y, np.sqrt(15 / (4 * pi)) * y * z,
+            np.sqrt(5 / (16 * pi)) * (3 * z * z - r2), np.sqrt(15 / (4 * pi)) * x * z,
+            np.sqrt(15 / (16 * pi)) * (x * x - y * y)]
+    Y = real_solid_harmonics(2, r)
+    assert np.abs(Y - np.array(want)).max() < 1e-13
+    ct, wt = np.polynomial.legendre.leggauss(8)
+    phi = np.arange(16) * 2 * np.pi / 16
+    st = np.sqrt(1 - ct**2)
+    pts = np.stack([st[:, None] * np.cos(phi), st[:, None] * np.sin(phi),
+                    np.broadcast_to(ct[:, None], (8, 16))], -1).reshape(-1, 3)
+    w = (wt[:, None] * np.full(16, 2 * np.pi / 16)).reshape(-1)
+    Y = real_solid_harmonics(3, pts)
+    assert np.abs((Y * w) @ Y.T - np.eye(16)).max() < 1e-13
+
+
+def test_quintic_spline():
+    """Blip coefficients: the spline reproduces its samples at the nodes,
+    meets math/test_quintic_spline.py's bound |f - interp| < dx^4 on 2001
+    points, is mirror-symmetric at 0 (c[-k] = c[k]) and has zero third and
+    fourth derivatives at the last node."""
+    from scipy.interpolate import BSpline
+
+    from oracle.projectors import BLIP_SCALE, blip_eval, quintic_coeff
+
+    dx = 0.1
+    xs = np.arange(0.0, 40.0, dx)
+
+    def f(x):
+        return np.exp(-np.sin(0.01 * x * x)) * np.cos(0.1 * x)
+
+    c = quintic_coeff(f(xs))
+    assert c.shape == (len(xs) + 4,)
+    assert np.abs(blip_eval(c, xs[:-1], dx) - f(xs[:-1])).max() < 1e-12
+    fine = np.linspace(0.0, xs.max() - 1e-6 * dx, 2001)
+    assert np.linalg.norm(blip_eval(c, fine, dx) - f(fine)) < dx**4
+    assert c[0] == c[4] and c[1] == c[3]
+    knots = (np.arange(len(c) + 6) - 5.0) * dx
+    s = BSpline(knots, c * BLIP_SCALE, 5)
+    scale = np.abs(s(xs, 2)).max()
+    for d in (3, 4):
+        assert abs(s(xs[-1], d)) < 1e-9 * scale / dx ** (d - 2)
+    # batched samples keep their trailing shape
+    cb = quintic_coeff(np.stack([f(xs), 2 * f(xs)]))
+    assert cb.shape == (len(xs) + 4, 2) and np.allclose(cb[:, 1], 2 * c)
+
+
+def test_spline_projectors_match_gaussian(small):
+    """Projectors from sampled Gaussian radials (spacing 0.02, as the
+    reference's RadialFunction.DG) equal the analytic Gaussian projectors of
+    the synthetic cells to the spline's accuracy."""
+    from oracle.hamiltonian import projectors as gauss_projectors
+    from oracle.projectors import projectors, quintic_coeff
+
+    spec, b = small
+    dG = 0.02
+    G = np.arange(int(np.ceil(np.sqrt(2 * spec.ke_cutoff) * 1.2 / dG)) + 5) * dG
+    l_rf = [l for rc in spec.proj_rc for l in spec.proj_l]
+    samples = np.stack([np.exp(-0.5 * rc**2 * G**2) for rc in spec.proj_rc for _ in spec.proj_l])
+    coeff = quintic_coeff(samples)
+    want = gauss_projectors(b)
+    got = projectors(b, spec.positions, coeff, dG, l_rf, spec.volume)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-8 * np.abs(want).max()
