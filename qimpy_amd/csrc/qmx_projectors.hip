@@ -4,9 +4,11 @@
 //                                   * exp(-2 pi i (iG + k) . tau_atom) * scale
 // with R a quintic blip spline (math/quintic_spline.py:56-128) and Y_lm the
 // real solid harmonics (math/spherical_harmonics.py:53-137). One thread per
-// plane wave g of k-point blockIdx.y: it evaluates the radial functions and
-// harmonics of its G + k once into LDS slots of its own, then writes every
-// (atom, projector) entry, consecutive threads on consecutive g.
+// plane wave g of k-point blockIdx.y and atom group blockIdx.z: it evaluates
+// the radial functions and harmonics of its G + k once into LDS slots of its
+// own, then writes every (atom, projector) entry of its atoms, consecutive
+// threads on consecutive g. Atom groups fill the chip when the plane waves
+// of the k-points alone give few workgroups.
 //
 // Harmonics: the associated Legendre recurrence in Cartesian form. With
 // Q_l^m = r^l P_l^m(cos t) / (r sin t)^m (no Condon-Shortley phase),
@@ -23,6 +25,7 @@
 // that a blip is 1 at its centre), expanded on the host into a polynomial in f.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -66,7 +69,7 @@ __device__ __forceinline__ double solid_harmonic(int l, int m, double x, double 
 
 __global__ void __launch_bounds__(kProjThreads)
     k_projectors(const int* __restrict__ iG, const double* __restrict__ kvec,
-                 const int* __restrict__ n, const double* __restrict__ pos, int n_atoms,
+                 const int* __restrict__ n, const double* __restrict__ pos, int n_atoms, int apg,
                  const double* __restrict__ coeff, int n_coeff, int n_rf, double inv_dG,
                  double scale, ProjDesc d, double2* __restrict__ out, int64_t ldp, int64_t ldk,
                  int n_tot) {
@@ -74,11 +77,12 @@ __global__ void __launch_bounds__(kProjThreads)
   const int tid = threadIdx.x;
   const int g = blockIdx.x * kProjThreads + tid;
   const int k = blockIdx.y;
+  const int a0 = blockIdx.z * apg, a1 = min(n_atoms, a0 + apg);
   if (g >= n_tot) return;
   double2* o = out + (int64_t)k * ldk + g;
   const int np = d.n_proj;
   if (g >= n[k]) {  // padding
-    for (int a = 0; a < n_atoms; ++a)
+    for (int a = a0; a < a1; ++a)
       for (int p = 0; p < np; ++p) o[(int64_t)(a * np + p) * ldp] = double2{0.0, 0.0};
     return;
   }
@@ -109,7 +113,7 @@ __global__ void __launch_bounds__(kProjThreads)
     for (int j = 0; j < 6; ++j) R += w[j] * cf[(int64_t)j * n_rf];
     val[p][tid] = scale * d.norm[p] * R * solid_harmonic(d.l[p], d.m[p], Gc[0], Gc[1], Gc[2], r2);
   }
-  for (int a = 0; a < n_atoms; ++a) {
+  for (int a = a0; a < a1; ++a) {
     const double dot = f[0] * pos[3 * a] + f[1] * pos[3 * a + 1] + f[2] * pos[3 * a + 2];
     double s, c;
     sincospi(-2.0 * dot, &s, &c);  // exp(-2 pi i (iG + k) . tau)
@@ -197,9 +201,13 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
   for (int i = 0; i < 9; ++i) d.G[i] = Gbasis[i];
   if (nk == 0 || n_tot == 0 || n_atoms == 0 || n_proj == 0) return QMX_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  dim3 grid((unsigned)((n_tot + kProjThreads - 1) / kProjThreads), (unsigned)nk);
+  const int64_t gx = (n_tot + kProjThreads - 1) / kProjThreads;
+  int64_t gz = std::min<int64_t>(n_atoms, std::max<int64_t>(1, (1024 + gx * nk - 1) / (gx * nk)));
+  const int64_t apg = (n_atoms + gz - 1) / gz;  // atoms per group
+  gz = (n_atoms + apg - 1) / apg;
+  dim3 grid((unsigned)gx, (unsigned)nk, (unsigned)gz);
   hipLaunchKernelGGL(k_projectors, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
-                     (int)n_atoms, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
+                     (int)n_atoms, (int)apg, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
                      static_cast<double2*>(out), ldp, ldk, (int)n_tot);
   QMX_LAUNCHED();
   return QMX_OK;
