@@ -174,6 +174,19 @@ int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k
                    double dG, double G_max, double scale, void* out, int64_t ldp, int64_t ldk,
                    void* stream);
 
+/* Gradient operator along Cartesian direction dir (Basis.apply_gradient,
+ * dft/electrons/_basis_ops.py:15-26, with get_gradient = i (G + k),
+ * _basis.py:240-254), scaled by alpha:
+ *   out(k, b, g) = [accumulate ? out : 0] + alpha i G[k][g][dir] C(k, b, g)
+ * G: Cartesian G + k of every basis entry, [nk][n_tot][3], real of the
+ * working precision (float for QMX_C64, double for QMX_C128). C and out have
+ * the strides of qmx_apply_local; out may equal C. With qmx_apply_local it
+ * gives the meta-GGA term of H, -1/2 sum_dir grad(V_tau grad C)
+ * (_hamiltonian.py:13-20). */
+int qmx_apply_gradient(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t nk,
+                       int64_t nb, int64_t n_tot, const void* G, int dir, double alpha, void* out,
+                       int64_t out_ldb, int64_t out_ldk, int accumulate, void* stream);
+
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
  * The result is multiplied by `scale` (1/N for QimPy's forward transform,

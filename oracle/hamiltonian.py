@@ -13,7 +13,13 @@ Restates:
 - ``apply_nonlocal``: dft/electrons/_hamiltonian.py:21-24, i.e.
   ``beta @ (D_all @ (beta ^ C))`` with the _dot/_matmul index conventions of
   _wavefunction_dot.py:66-84,168-185;
-- ``hamiltonian``: _hamiltonian.py:7-29 (tau and +U branches off).
+- ``apply_gradient``: _basis_ops.py:15-26 with get_gradient = i (G + k)
+  (_basis.py:240-254); ``apply_tau``: the need_tau branch of _hamiltonian.py:
+  13-20, -1/2 sum_dir grad_dir apply_potential(V_tau, grad_dir C). Pinned
+  (tests/test_oracle.py): a constant V_tau = c gives c KE C (the sign, the
+  factor 1/2 and all three directions), Hermiticity for a real V_tau, and
+  <C|H_tau|C> = 1/2 sum_dir <grad C|V_tau|grad C> >= 0 for V_tau >= 0;
+- ``hamiltonian``: _hamiltonian.py:7-29 (+U branch off; tau when V_tau is set).
 """
 from __future__ import annotations
 
@@ -73,6 +79,19 @@ def apply_potential(basis: Basis, V: np.ndarray, C: np.ndarray) -> np.ndarray:
     return out
 
 
+def apply_gradient(basis: Basis, C: np.ndarray, i_dir: int) -> np.ndarray:
+    Gk = (basis.iG + basis.k[:, None, :]) @ basis.Gbasis.T  # (K, G, 3)
+    return C * (1j * Gk[:, None, :, i_dir])  # C (K, B, G)
+
+
+def apply_tau(basis: Basis, V_tau: np.ndarray, C: np.ndarray) -> np.ndarray:
+    out = np.zeros_like(C)
+    for i_dir in range(3):
+        out -= 0.5 * apply_gradient(
+            basis, apply_potential(basis, V_tau, apply_gradient(basis, C, i_dir)), i_dir)
+    return out
+
+
 def dot(C1: np.ndarray, C2: np.ndarray) -> np.ndarray:
     """(K, B1, B2) = conj(C1) @ C2^T per k (_wavefunction_dot.py:66-84)."""
     return np.matmul(C1.conj(), C2.transpose(0, 2, 1))
@@ -91,15 +110,19 @@ def apply_nonlocal(beta: np.ndarray, D: np.ndarray, C: np.ndarray) -> np.ndarray
 class Hamiltonian:
     """H = T + V_loc + beta D beta^H on one cell, FP64."""
 
-    def __init__(self, basis: Basis, V: np.ndarray | None = None):
+    def __init__(self, basis: Basis, V: np.ndarray | None = None,
+                 V_tau: np.ndarray | None = None):
         self.basis = basis
         spec = basis.spec
         self.V = spec.V if V is None else V
+        self.V_tau = V_tau
         self.beta = projectors(basis)
         self.D = spec.D.astype(np.float64)
 
     def __call__(self, C: np.ndarray) -> np.ndarray:
         HC = apply_ke(self.basis, C)
         HC += apply_potential(self.basis, self.V, C)
+        if self.V_tau is not None:
+            HC += apply_tau(self.basis, self.V_tau, C)
         HC += apply_nonlocal(self.beta, self.D, C)
         return HC

@@ -93,6 +93,8 @@ class Basis:
         )
         self._ke = {torch.float64: 0.5 * Gk.square().sum(dim=-1)}  # get_ke :256-270
         self._ke[torch.float32] = self._ke[torch.float64].to(torch.float32)
+        self._gk = {torch.float64: Gk.contiguous()}  # Cartesian G + k (get_gradient / i)
+        self._gk[torch.float32] = self._gk[torch.float64].to(torch.float32)
 
         # Sparse-FFT plan (index tables on the device):
         geom, keep = _lib.make_geom(self.grid.shape, self.n_np, self.fft_index_np)
@@ -147,7 +149,9 @@ class Basis:
         cached = self._V_cache.get(key)
         if cached is None:
             cached = data.to(device=rc.device, dtype=rtype).contiguous()
-            self._V_cache = {key: cached}  # keep only the latest potential
+            if len(self._V_cache) >= 4:  # keep the latest few (V_ks and V_tau alternate)
+                self._V_cache.pop(next(iter(self._V_cache)))
+            self._V_cache[key] = cached
         return cached
 
     def apply_local(self, coeff: torch.Tensor, V, with_ke: bool, out: Optional[torch.Tensor] = None,
@@ -177,6 +181,47 @@ class Basis:
                 B, int(accumulate), ws.data_ptr(), ws_bytes, stream,
             )
         del esize
+        return out
+
+    def gradient(self, coeff: torch.Tensor, i_dir: int, alpha: float = 1.0,
+                 out: Optional[torch.Tensor] = None, accumulate: bool = False) -> torch.Tensor:
+        """out = [out +] alpha i (G + k)_dir C on (S, K, B, X, G) coefficients
+        (qmx_apply_gradient)."""
+        if coeff.dtype not in DTYPE_CODE:
+            raise TypeError(f"unsupported wavefunction dtype {coeff.dtype}")
+        S, K, B, X, G = coeff.shape
+        if K != self.nk or G != self.n_tot:
+            raise ValueError(f"coefficients {tuple(coeff.shape)} do not match the basis")
+        if coeff.stride(-1) != 1 or coeff.stride(3) < G or coeff.stride(2) < X * G:
+            coeff = coeff.contiguous()
+        if out is None:
+            out = torch.empty((S, K, B, X, G), dtype=coeff.dtype, device=coeff.device)
+        gk = self._gk[REAL_OF[coeff.dtype]]
+        code = DTYPE_CODE[coeff.dtype]
+        for s in range(S):
+            for x in range(X):
+                cs, os_ = coeff[s, :, :, x], out[s, :, :, x]
+                _lib.call("qmx_apply_gradient", code, cs.data_ptr(), cs.stride(1), cs.stride(0),
+                          K, B, G, gk.data_ptr(), int(i_dir), float(alpha), os_.data_ptr(),
+                          os_.stride(1), os_.stride(0), int(accumulate), rc.stream_ptr())
+        return out
+
+    def apply_gradient(self, C, i_dir: int):
+        """Basis.apply_gradient (_basis_ops.py:15-26): i (G + k)_dir C."""
+        from .wavefunction import Wavefunction
+
+        return Wavefunction(self, coeff=self.gradient(C.coeff, i_dir),
+                            band_division=C.band_division)
+
+    def apply_tau(self, V_tau, coeff: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out -= 1/2 sum_dir grad_dir(V_tau grad_dir C), the meta-GGA term of
+        H (_hamiltonian.py:13-20): per direction one gradient, one local apply
+        and one accumulating gradient scaled by -1/2."""
+        tmp = None
+        for i_dir in range(3):
+            tmp = self.gradient(coeff, i_dir, out=tmp)
+            tmp = self.apply_local(tmp, V_tau, with_ke=False, out=tmp)
+            self.gradient(tmp, i_dir, alpha=-0.5, out=out, accumulate=True)
         return out
 
     def apply_ke(self, C):

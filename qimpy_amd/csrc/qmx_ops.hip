@@ -155,6 +155,33 @@ __global__ void k_randomize(C2* __restrict__ C, int64_t ldb, int64_t ldk, int n_
 
 }  // namespace
 
+namespace {
+
+// out[k][b][g] = [accumulate ? out : 0] + alpha i G[k][g][dir] C[k][b][g]:
+// the gradient operator of Basis.apply_gradient (dft/electrons/
+// _basis_ops.py:15-26; get_gradient = i (iG + k) Gbasis^T, _basis.py:240-254),
+// scaled and optionally accumulated. One thread per plane wave; out may be C.
+template <typename C2, typename R>
+__global__ void __launch_bounds__(256)
+    k_apply_gradient(const C2* C, int64_t ldb, int64_t ldk, C2* out, int64_t oldb, int64_t oldk,
+                     int n_tot, const R* __restrict__ G, int dir, R alpha, int accumulate) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int64_t b = blockIdx.y, k = blockIdx.z;
+  if (g >= n_tot) return;
+  const R s = alpha * G[(k * n_tot + g) * 3 + dir];
+  const C2 c = C[k * ldk + b * ldb + g];
+  C2 v{-s * c.y, s * c.x};  // i s c
+  C2* o = out + k * oldk + b * oldb + g;
+  if (accumulate) {
+    const C2 prev = *o;
+    v.x += prev.x;
+    v.y += prev.y;
+  }
+  *o = v;
+}
+
+}  // namespace
+
 #define QMX_DISPATCH(dtype, KERNEL_CALL_F32, KERNEL_CALL_F64)          \
   do {                                                                \
     if ((dtype) == QMX_C64) {                                         \
@@ -246,6 +273,28 @@ int qmx_randomize(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk, int6
                hipLaunchKernelGGL(k_randomize<double2>, grid, dim3(256), 0, st, (double2*)C, ldb,
                                   ldk, (int)n_tot, n_max, k_offset, n, ke, ke_ldk, seed, b_start,
                                   b_stop));
+  return QMX_OK;
+}
+
+int qmx_apply_gradient(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t nk,
+                       int64_t nb, int64_t n_tot, const void* G, int dir, double alpha, void* out,
+                       int64_t out_ldb, int64_t out_ldk, int accumulate, void* stream) {
+  QMX_CHECK_ARG(C && G && out, "qmx_apply_gradient: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < 65536 && n_tot >= 0 &&
+                    n_tot < (1 << 30),
+                "qmx_apply_gradient: bad sizes");
+  QMX_CHECK_ARG(dir >= 0 && dir < 3, "qmx_apply_gradient: dir must be 0, 1 or 2");
+  QMX_CHECK_ARG(ldb >= n_tot && out_ldb >= n_tot, "qmx_apply_gradient: bad strides");
+  if (nk == 0 || nb == 0 || n_tot == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)((n_tot + 255) / 256), (unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL((k_apply_gradient<float2, float>), grid, dim3(256), 0, st,
+                                  (const float2*)C, ldb, ldk, (float2*)out, out_ldb, out_ldk,
+                                  (int)n_tot, (const float*)G, dir, (float)alpha, accumulate),
+               hipLaunchKernelGGL((k_apply_gradient<double2, double>), grid, dim3(256), 0, st,
+                                  (const double2*)C, ldb, ldk, (double2*)out, out_ldb, out_ldk,
+                                  (int)n_tot, (const double*)G, dir, alpha, accumulate));
   return QMX_OK;
 }
 

@@ -56,11 +56,15 @@ class Electrons:
         filter_order: int = 10,
         init_threshold: float = 1e-1,
         residual_threshold: Optional[float] = None,
+        V_tau: Optional[torch.Tensor] = None,
     ):
         self.basis = basis
         self.ions = basis.ions
         self.V_ks = torch.as_tensor(V, device=rc.device, dtype=torch.float64).contiguous()
         self.V_max = float(self.V_ks.max().item())
+        #: meta-GGA potential dE/dtau on the box (tau_tilde.grad), or None
+        self.V_tau = (None if V_tau is None else
+                      torch.as_tensor(V_tau, device=rc.device, dtype=torch.float64).contiguous())
         self.dtype = dtype
         nbe = _n_bands_extra(n_bands) if n_bands_extra is None else n_bands_extra
         self.fillings = SimpleNamespace(n_bands=n_bands, n_bands_extra=nbe, n_bands_min=n_bands)
@@ -83,6 +87,8 @@ class Electrons:
     def hamiltonian(self, C: Wavefunction) -> Wavefunction:
         basis = C.basis
         HC = basis.apply_ke_potential(self.V_ks, C)
+        if self.V_tau is not None:  # need_tau branch (_hamiltonian.py:13-20)
+            basis.apply_tau(self.V_tau, C.coeff, HC.coeff)
         ions = basis.ions
         if ions is not None and ions.n_projectors:
             beta_C = C.proj  # (S, K, P, B), cached on C

@@ -682,3 +682,37 @@ def test_spline_ions_reproduce_gaussian_projectors(gpu):
     with pytest.raises(RuntimeError, match="more than 32"):
         spline_projectors(b, torch.tensor(spec.positions),
                           RadialSet((3,) * 5, np.repeat(f[:1], 5, axis=0), dG))
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_gradient_and_tau_branch(cfg0_el, dtype):
+    """qmx_apply_gradient (i (G + k)_dir C) and the meta-GGA term of H,
+    -1/2 sum_dir grad(V_tau grad C), against the oracle: complex128 to
+    1e-12, complex64 to 2e-5 of the largest entry."""
+    from oracle.hamiltonian import apply_gradient as o_grad
+    from oracle.hamiltonian import apply_tau as o_tau
+    spec, ob, el = cfg0_el
+    b = el.basis
+    C = rand_coeff(ob, 5, seed=12)
+    wf = to_wf(b, C, dtype)
+    tol = 1e-12 if dtype == C128 else 2e-5
+    for d in range(3):
+        assert rel(b.apply_gradient(wf, d).coeff[0, :, :, 0], o_grad(ob, C, d)) < tol
+    V_tau = np.random.default_rng(3).uniform(0.2, 1.0, ob.shape)
+    out = torch.zeros_like(wf.coeff)
+    b.apply_tau(torch.tensor(V_tau, device="cuda"), wf.coeff, out)
+    err = rel(out[0, :, :, 0], o_tau(ob, V_tau, C))
+    print(f"[tau branch] {dtype} {err:.2e}", flush=True)
+    assert err < tol
+
+
+def test_davidson_with_tau_vs_dense(small_el):
+    """A band solve whose H carries the meta-GGA term reaches the dense
+    eigenvalues of T + V + 1/2 (G + k) . (G' + k) V~tau(G - G') + nonlocal."""
+    spec, ob, _ = small_el
+    V_tau = np.random.default_rng(4).uniform(0.1, 0.5, ob.shape)
+    exact = dense_eigenvalues(ob, spec.n_bands, V_tau=V_tau)
+    el, rep = _solve(spec, C128, 80, 1e-11, V_tau=V_tau)
+    d = np.abs(rep.eig[0].cpu().numpy() - exact).max()
+    print(f"[davidson tau] iterations {rep.n_iterations}, max |eig - dense| {d:.2e}", flush=True)
+    assert d < 1e-9

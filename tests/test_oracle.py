@@ -409,3 +409,50 @@ def test_spline_projectors_match_gaussian(small):
     got = projectors(b, spec.positions, coeff, dG, l_rf, spec.volume)
     assert got.shape == want.shape
     assert np.abs(got - want).max() < 1e-8 * np.abs(want).max()
+
+
+def test_tau_branch_closed_forms(small):
+    """Oracle meta-GGA term -1/2 sum_dir grad(V_tau grad C): a constant
+    V_tau = c gives c KE C; Hermitian for a real V_tau; <C|H_tau|C> >= 0 for
+    V_tau >= 0."""
+    from oracle.hamiltonian import apply_tau
+
+    spec, b = small
+    K, G = b.fft_index.shape
+    rng = np.random.default_rng(9)
+    C = rng.standard_normal((K, 3, G)) + 1j * rng.standard_normal((K, 3, G))
+    b.constrain(C)
+    got = apply_tau(b, np.full(b.shape, 0.7), C)
+    want = 0.7 * b.KE[:, None, :] * C
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    V = 1.0 + 0.5 * rng.uniform(-1, 1, b.shape)  # positive
+    A = rng.standard_normal((K, 2, G)) + 1j * rng.standard_normal((K, 2, G))
+    b.constrain(A)
+    HA, HC = apply_tau(b, V, A), apply_tau(b, V, C)
+    lhs = np.einsum("kag,kbg->kab", A.conj(), HC)
+    rhs = np.einsum("kbg,kag->kab", C.conj(), HA).conj()
+    assert np.abs(lhs - rhs).max() < 1e-11 * np.abs(lhs).max()
+    diag = np.einsum("kbg,kbg->kb", C.conj(), HC)
+    assert np.all(diag.real > 0) and np.abs(diag.imag).max() < 1e-10 * diag.real.max()
+
+
+def test_tau_branch_vs_dense(small):
+    """Oracle apply_tau (through the FFT) against the dense brute force
+    1/2 (G + k) . (G' + k) V~tau(G - G') on the in-sphere plane waves."""
+    from oracle.dense import dense_hamiltonian
+    from oracle.hamiltonian import apply_tau
+
+    spec, b = small
+    K, G = b.fft_index.shape
+    rng = np.random.default_rng(10)
+    V = rng.uniform(0.2, 1.0, b.shape)
+    C = rng.standard_normal((K, 2, G)) + 1j * rng.standard_normal((K, 2, G))
+    b.constrain(C)
+    got = apply_tau(b, V, C)
+    zero = np.zeros(b.shape)
+    for ik in range(K):
+        n = int(b.n[ik])
+        H0 = dense_hamiltonian(b, ik, V=zero, D=np.zeros_like(spec.D))
+        Ht = dense_hamiltonian(b, ik, V=zero, D=np.zeros_like(spec.D), V_tau=V)
+        want = C[ik, :, :n] @ (Ht - H0).T
+        assert np.abs(got[ik, :, :n] - want).max() < 1e-12 * np.abs(want).max()
