@@ -187,6 +187,18 @@ int qmx_apply_gradient(int dtype, const void* C, int64_t ldb, int64_t ldk, int64
                        int64_t nb, int64_t n_tot, const void* G, int dir, double alpha, void* out,
                        int64_t out_ldb, int64_t out_ldk, int accumulate, void* stream);
 
+/* Radial transforms of RadialFunction.transform (math/_radial_function.py:
+ * 62-123) without the spline fit:
+ *   out[i][iG] = sum_r w[r] f[i][r] r^(2 l_i) j_l(G r) / (G r)^l,  G = iG dG,
+ * with f the real-space functions (r^l removed) on the radial grid r, w the
+ * integration weights including 4 pi r^2 dr (the reference's Simpson rule,
+ * :84-89), j_l(x) / x^l as in math/spherical_bessel.py:8-49. r, w, f, out:
+ * device double ([n_r], [n_r], [n_f][n_r], [n_f][nG]); l: host int32 [n_f],
+ * 0 <= l <= 6, n_f <= 64. */
+int qmx_radial_transform(int64_t n_f, int64_t n_r, const double* r, const double* w,
+                         const double* f, const int32_t* l, int64_t nG, double dG, double* out,
+                         void* stream);
+
 /* In-place batched dense c2c 3D FFT of `batch` contiguous boxes.
  * direction -1: forward, exp(-i G r); +1: inverse, exp(+i G r).
  * The result is multiplied by `scale` (1/N for QimPy's forward transform,

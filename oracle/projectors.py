@@ -18,6 +18,12 @@ zeroed on the padding (:67-70), where
   m > 0) and imaginary part times sqrt 2 at index l - m; ``harmonics_tilde``
   multiplies by i^l (math/spherical_harmonics.py:111-137).
 
+``radial_transform`` restates RadialFunction.transform (math/
+_radial_function.py:62-123; Simpson weights at :84-89) with scipy's
+spherical_jn for j_l(x) / x^l (the reference's jl_by_xl, math/
+spherical_bessel.py:8-49); pinned by the Gaussian closed form
+f(r) = exp(-r^2 / 2 s^2) -> (2 pi)^(3/2) s^(2l + 3) exp(-s^2 G^2 / 2) for l <= 3.
+
 Pinned (tests/test_oracle.py): the harmonics against closed forms for
 l <= 2 and orthonormality on the sphere by exact quadrature for l <= 3; the
 spline reproduces its samples at the nodes, meets the reference's own
@@ -115,3 +121,29 @@ def projectors(basis, positions: np.ndarray, coeff: np.ndarray, dG: float,
     beta = beta.transpose(2, 0, 1, 3).reshape(iGk.shape[0], -1, iGk.shape[1])
     beta[np.broadcast_to(basis.pad_mask[:, None, :], beta.shape)] = 0.0
     return beta
+
+
+def jl_by_xl(l: int, x: np.ndarray) -> np.ndarray:
+    x = np.asarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    small = x < 1e-3
+    dfact = np.prod(np.arange(1, 2 * l + 2, 2, dtype=np.float64))  # (2l + 1)!!
+    xs = x[small]
+    out[small] = (1.0 - xs**2 / (2 * (2 * l + 3))) / dfact  # two terms of the series
+    xl = x[~small]
+    out[~small] = scipy.special.spherical_jn(l, xl) / xl**l
+    return out
+
+
+def radial_transform(r: np.ndarray, dr: np.ndarray, f: np.ndarray, l_rf, G_max: float,
+                     dG: float) -> np.ndarray:
+    """f_tilde (n_f, nG), nG = ceil(G_max / dG) + 5, of f (n_f, n_r)."""
+    w = (2.0 / 3.0) * 4.0 * np.pi * r**2 * dr
+    w[1::2] *= 2.0
+    w[0] *= 0.5
+    w[-1] *= 0.5
+    G = np.arange(int(np.ceil(G_max / dG)) + 5) * dG
+    out = np.empty((len(l_rf), len(G)))
+    for i, l in enumerate(l_rf):
+        out[i] = (w * f[i] * r ** (2 * l)) @ jl_by_xl(l, np.outer(r, G))
+    return out

@@ -131,6 +131,71 @@ __global__ void __launch_bounds__(kProjThreads)
   }
 }
 
+// j_l(x) / x^l for l <= 6: the Taylor series sum_i (-x^2/2)^i / (i! (2l + 2i + 1)!!)
+// below x = 1 + 0.7 l (9 + 2l terms), else j_0 = sin x / x,
+// j_1 / x = (j_0 - cos x) / x^2 and the upward recurrence
+// j_l / x^l = ((2l - 1) j_{l-1} / x^{l-1} - j_{l-2} / x^{l-2}) / x^2.
+__device__ double jl_by_xl(int l, double x) {
+  if (x <= 1.0 + 0.7 * l) {
+    double term = 1.0;
+    for (int q = 1; q <= l; ++q) term /= (double)(2 * q + 1);  // 1 / (2l + 1)!!
+    double sum = term;
+    const double x2 = x * x;
+    for (int i = 1; i < 9 + 2 * l; ++i) {
+      term *= -0.5 * x2 / (i * (2.0 * l + 2.0 * i + 1.0));
+      sum += term;
+    }
+    return sum;
+  }
+  double s, c;
+  sincos(x, &s, &c);
+  const double x2 = x * x;
+  double jm2 = s / x;  // j_0
+  if (l == 0) return jm2;
+  double jm1 = (jm2 - c) / x2;  // j_1 / x
+  for (int q = 2; q <= l; ++q) {
+    const double j = ((2 * q - 1) * jm1 - jm2) / x2;
+    jm2 = jm1;
+    jm1 = j;
+  }
+  return jm1;
+}
+
+constexpr int kRadialThreads = 256;
+constexpr int kRadialMaxF = 64;
+
+struct LTable {
+  signed char l[kRadialMaxF];
+};
+
+// f_tilde[i][iG] = sum_r w[r] f[i][r] r^(2 l_i) j_l(G r) / (G r)^l, G = iG dG:
+// one workgroup per (block of 256 G, function i); r staged through LDS.
+__global__ void __launch_bounds__(kRadialThreads)
+    k_radial_transform(const double* __restrict__ r, const double* __restrict__ w,
+                       const double* __restrict__ f, LTable lt, int n_r, int nG, double dG,
+                       double* __restrict__ out) {
+  __shared__ double rs[kRadialThreads], ws[kRadialThreads];
+  const int i = blockIdx.y, l = lt.l[i];
+  const int iG = blockIdx.x * kRadialThreads + threadIdx.x;
+  const double G = iG * dG;
+  double acc = 0.0;
+  for (int r0 = 0; r0 < n_r; r0 += kRadialThreads) {
+    const int ir = r0 + threadIdx.x;
+    if (ir < n_r) {
+      const double rr = r[ir];
+      double r2l = 1.0;
+      for (int q = 0; q < l; ++q) r2l *= rr * rr;
+      rs[threadIdx.x] = rr;
+      ws[threadIdx.x] = w[ir] * f[(int64_t)i * n_r + ir] * r2l;
+    }
+    __syncthreads();
+    const int m = min(kRadialThreads, n_r - r0);
+    for (int q = 0; q < m; ++q) acc += ws[q] * jl_by_xl(l, G * rs[q]);
+    __syncthreads();
+  }
+  if (iG < nG) out[(int64_t)i * nG + iG] = acc;
+}
+
 double factorial(int q) {
   double r = 1.0;
   for (int i = 2; i <= q; ++i) r *= i;
@@ -209,6 +274,36 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
   hipLaunchKernelGGL(k_projectors, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
                      (int)n_atoms, (int)apg, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
                      static_cast<double2*>(out), ldp, ldk, (int)n_tot);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
+extern "C" int qmx_radial_transform(int64_t n_f, int64_t n_r, const double* r, const double* w,
+                                    const double* f, const int32_t* l, int64_t nG, double dG,
+                                    double* out, void* stream) {
+  using qmx::set_error;
+  if (!r || !w || !f || !l || !out) {
+    set_error("qmx_radial_transform: null argument");
+    return QMX_ERR_ARG;
+  }
+  if (n_f < 0 || n_f > kRadialMaxF || n_r < 0 || n_r > (1 << 30) || nG < 0 ||
+      nG > (1 << 30) || !(dG > 0.0)) {
+    set_error("qmx_radial_transform: bad sizes (at most 64 functions)");
+    return QMX_ERR_ARG;
+  }
+  LTable lt{};
+  for (int i = 0; i < n_f; ++i) {
+    if (l[i] < 0 || l[i] > kLMax) {
+      set_error("qmx_radial_transform: l must be in 0..6");
+      return QMX_ERR_ARG;
+    }
+    lt.l[i] = (signed char)l[i];
+  }
+  if (n_f == 0 || nG == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)((nG + kRadialThreads - 1) / kRadialThreads), (unsigned)n_f);
+  hipLaunchKernelGGL(k_radial_transform, grid, dim3(kRadialThreads), 0, st, r, w, f, lt,
+                     (int)n_r, (int)nG, dG, out);
   QMX_LAUNCHED();
   return QMX_OK;
 }

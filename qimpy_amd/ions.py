@@ -93,6 +93,37 @@ def quintic_coeff(samples: np.ndarray) -> np.ndarray:
     return c.reshape((N + 4,) + samples.shape[:-1])
 
 
+def simpson_weights(r: np.ndarray, dr: np.ndarray) -> np.ndarray:
+    """4 pi r^2 dr with Simpson's 1/3 rule on the radial grid (weights 1, 4,
+    2, 4, ..., 1 over 3; math/_radial_function.py:84-89)."""
+    w = (2.0 / 3.0) * 4.0 * np.pi * np.asarray(r, dtype=np.float64) ** 2 * np.asarray(dr)
+    w[1::2] *= 2.0
+    w[0] *= 0.5
+    w[-1] *= 0.5
+    return w
+
+
+def radial_transform(r: np.ndarray, dr: np.ndarray, f: np.ndarray, l_rf, G_max: float,
+                     dG: float = DG_DEFAULT) -> np.ndarray:
+    """f_tilde (n_f, nG) on G = iG dG, nG = ceil(G_max / dG) + 5, of radial
+    functions f (n_f, n_r) with r^l removed (RadialFunction.transform,
+    math/_radial_function.py:62-123, through qmx_radial_transform)."""
+    f = np.atleast_2d(np.asarray(f, dtype=np.float64))
+    l32 = np.ascontiguousarray(np.asarray(l_rf, dtype=np.int32).reshape(-1))
+    if f.shape[0] != len(l32) or f.shape[1] != len(r):
+        raise ValueError("f must be (len(l_rf), len(r))")
+    nG = int(np.ceil(G_max / dG)) + 5
+    dev = rc.device
+    rt = torch.tensor(np.asarray(r, dtype=np.float64), device=dev)
+    wt = torch.tensor(simpson_weights(r, dr), device=dev)
+    ft = torch.tensor(f, device=dev)
+    out = torch.empty((len(l32), nG), dtype=torch.float64, device=dev)
+    _lib.call("qmx_radial_transform", len(l32), len(r), rt.data_ptr(), wt.data_ptr(),
+              ft.data_ptr(), l32.ctypes.data_as(_lib.c_i32p), nG, float(dG), out.data_ptr(),
+              rc.stream_ptr())
+    return out.cpu().numpy()
+
+
 class RadialSet:
     """Radial functions of one species on a uniform reciprocal grid (the
     f_tilde of RadialFunction, math/_radial_function.py:11-123, with G^l
@@ -118,6 +149,13 @@ class RadialSet:
     @property
     def n_proj(self) -> int:
         return len(self.l)
+
+    @classmethod
+    def from_real_space(cls, l_rf, r: np.ndarray, dr: np.ndarray, f: np.ndarray,
+                        G_max: float, dG: float = DG_DEFAULT) -> "RadialSet":
+        """Radial functions f (n_f, n_r) on the grid r (weights dr), r^l
+        removed, transformed up to G_max (radial_transform) and fitted."""
+        return cls(l_rf, radial_transform(r, dr, f, l_rf, G_max, dG), dG)
 
 
 def spline_projectors(basis, positions: torch.Tensor, radial: RadialSet) -> torch.Tensor:

@@ -456,3 +456,21 @@ def test_tau_branch_vs_dense(small):
         Ht = dense_hamiltonian(b, ik, V=zero, D=np.zeros_like(spec.D), V_tau=V)
         want = C[ik, :, :n] @ (Ht - H0).T
         assert np.abs(got[ik, :, :n] - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_radial_transform_gaussian():
+    """Oracle radial transform of exp(-r^2 / 2 s^2) (r^l removed) equals
+    (2 pi)^(3/2) s^(2l + 3) exp(-s^2 G^2 / 2) for l = 0..3 (Simpson on an odd
+    grid to 12 s)."""
+    from oracle.projectors import radial_transform
+
+    s = 0.8
+    r = np.linspace(0.0, 12 * s, 2401)
+    dr = np.full_like(r, r[1] - r[0])
+    l_rf = [0, 1, 2, 3]
+    f = np.stack([np.exp(-0.5 * r**2 / s**2)] * 4)
+    ft = radial_transform(r, dr, f, l_rf, 6.0, 0.02)
+    G = np.arange(ft.shape[1]) * 0.02
+    for i, l in enumerate(l_rf):
+        want = (2 * np.pi) ** 1.5 * s ** (2 * l + 3) * np.exp(-0.5 * s**2 * G**2)
+        assert np.abs(ft[i] - want).max() < 1e-10 * want.max()
