@@ -109,6 +109,38 @@ class Electrons:
         self.C = C if self.dtype == torch.complex128 else C.to(self.dtype)
         self.deig_max = np.inf
 
+    def initialize_lcao(self, orbitals, seed: int = 0) -> None:
+        """LCAO start of a fixed-H solve (Electrons.initialize_wavefunctions
+        with lcao, _electrons.py:255-287, and LCAO.update's subspace
+        diagonalisation, _lcao.py:53-68): the first bands are the atomic
+        orbitals of every ion (`orbitals`, a RadialSet of the orbital radials,
+        built by qmx_projectors as get_atomic_orbitals does), the remaining
+        bands random; then eighg(C^H H C, C^H O C) in FP64 and C = C V. With
+        more orbitals than bands, the lowest eigenvectors of the full orbital
+        subspace are kept (the reference keeps them all). The subspace
+        minimisation with smearing belongs to SCF and is not on this path."""
+        from .ions import spline_projectors
+
+        if self.basis.ions is None:
+            raise ValueError("LCAO needs ions")
+        n_total = self.C.n_bands()
+        psi = spline_projectors(self.basis, self.basis.ions.positions, orbitals)  # (K, A, G)
+        n_atomic = psi.shape[1]
+        S = self.C.coeff.shape[0]
+        if n_atomic < n_total:
+            C = Wavefunction(self.basis, n_bands=n_total, dtype=torch.complex128)
+            C.coeff[:, :, :n_atomic, 0] = psi[None]
+            C.randomize(seed=seed, b_start=n_atomic)
+        else:
+            C = Wavefunction(self.basis, coeff=psi[None, :, :, None, :].expand(
+                S, -1, -1, -1, -1).contiguous())
+        HC = self.hamiltonian(C)
+        eig, V = eighg(C ^ HC, C.dot_O(C).wait())
+        C = C @ V[..., :n_total]
+        self.eig = eig[..., :n_total]
+        self.C = C if self.dtype == torch.complex128 else C.to(self.dtype)
+        self.deig_max = np.inf
+
     def get_Eband(self) -> float:
         n = self.fillings.n_bands
         return float((self.basis.w_sk * self.eig[..., :n]).sum().item())

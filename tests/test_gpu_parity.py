@@ -763,3 +763,30 @@ def test_projectors_from_real_space_gaussians(gpu):
     err = float((ions.beta.coeff - want).abs().max() / want.abs().max())
     print(f"[real-space gaussians] max err {err:.2e}", flush=True)
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("l_rf", [(0,), (0, 1, 1)])
+def test_lcao_start_vs_dense(small_el, l_rf):
+    """LCAO start (atomic orbitals from qmx_projectors, the rest random, FP64
+    subspace eighg): orthonormal bands whose Ritz values bound the dense
+    eigenvalues from above (variational), then a Davidson solve from it
+    reaches the dense eigenvalues. l = (0,) gives fewer orbitals than bands,
+    (0, 1, 1) more."""
+    from qimpy_amd.ions import RadialSet
+    spec, ob, _ = small_el
+    exact = dense_eigenvalues(ob, spec.n_bands)
+    el = electrons_from_spec(spec, dtype=C128, n_iterations=80, eig_threshold=1e-11)
+    G = np.arange(int(np.ceil(np.sqrt(2 * spec.ke_cutoff) / 0.02)) + 10) * 0.02
+    f = np.stack([np.exp(-0.5 * (0.8 + 0.4 * i) ** 2 * G**2) for i in range(len(l_rf))])
+    el.initialize_lcao(RadialSet(l_rf, f))
+    n_total = el.C.n_bands()
+    O = (el.C ^ el.C).wait()
+    assert (O - torch.eye(n_total, device="cuda")).abs().max().item() < 1e-10
+    eig0 = el.eig[0].cpu().numpy()
+    assert np.all(np.diff(eig0, axis=-1) >= -1e-12)
+    assert np.all(eig0[:, : spec.n_bands] >= exact - 1e-10)
+    rep = el.solve()
+    d = np.abs(rep.eig[0].cpu().numpy() - exact).max()
+    print(f"[lcao {l_rf}] start max {np.abs(eig0[:, :spec.n_bands] - exact).max():.2e},"
+          f" iterations {rep.n_iterations}, max |eig - dense| {d:.2e}", flush=True)
+    assert d < 1e-9
