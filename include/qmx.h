@@ -372,6 +372,7 @@ enum {
   QMX_KERNEL_PLANE = 7,   /* fused plane pass: inverse y, x, * V, forward x, y */
   QMX_KERNEL_VZMAJOR = 8, /* potential boxes to z-major order */
   QMX_KERNEL_APPLY = 9,   /* whole local apply, dense-transform byte convention */
+  QMX_KERNEL_PROJ = 10,   /* qmx_projectors (complex128 output: reported as 10 + C128) */
   QMX_KERNEL_C128 = 16,   /* added to the id of a complex128 launch */
   QMX_N_KERNEL_IDS = 32
 };

@@ -15,6 +15,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libqmx.so"
 
 QMX_C64, QMX_C128 = 0, 1
+QMX_KERNEL_PROJ, QMX_KERNEL_C128 = 10, 16
 
 c_i32p = ct.POINTER(ct.c_int32)
 c_i64p = ct.POINTER(ct.c_int64)
@@ -192,6 +193,7 @@ _BASE_NAMES = {
     3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
     7: "k_plane_potential", 8: "k_potential_zmajor",
     9: "local_apply",  # the whole apply (a span of launches, not one kernel)
+    10: "k_projectors",
 }
 #: Profile ids: the kernel id for complex64 launches, + 16 for complex128.
 KERNEL_NAMES = {**{i: f"{n}<c64>" for i, n in _BASE_NAMES.items()},

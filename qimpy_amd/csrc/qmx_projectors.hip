@@ -31,6 +31,11 @@
 
 #include "qmx_common.h"
 
+namespace qmx {
+void prof_begin(hipStream_t st, hipEvent_t* a);
+void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes);
+}  // namespace qmx
+
 namespace {
 
 constexpr int kProjThreads = 128;
@@ -271,10 +276,14 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
   const int64_t apg = (n_atoms + gz - 1) / gz;  // atoms per group
   gz = (n_atoms + apg - 1) / apg;
   dim3 grid((unsigned)gx, (unsigned)nk, (unsigned)gz);
+  hipEvent_t ev;
+  qmx::prof_begin(st, &ev);
   hipLaunchKernelGGL(k_projectors, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
                      (int)n_atoms, (int)apg, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
                      static_cast<double2*>(out), ldp, ldk, (int)n_tot);
   QMX_LAUNCHED();
+  // compulsory bytes: the complex128 output
+  qmx::prof_end(st, ev, QMX_KERNEL_PROJ + QMX_KERNEL_C128, 16.0 * nk * n_atoms * n_proj * n_tot);
   return QMX_OK;
 }
 

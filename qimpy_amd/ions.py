@@ -16,6 +16,8 @@ block diagonal per atom (_ions_update.py:229-243); the cells make it diagonal.
 """
 from __future__ import annotations
 
+from typing import Optional
+
 import numpy as np
 import torch
 
@@ -158,12 +160,19 @@ class RadialSet:
         return cls(l_rf, radial_transform(r, dr, f, l_rf, G_max, dG), dG)
 
 
-def spline_projectors(basis, positions: torch.Tensor, radial: RadialSet) -> torch.Tensor:
+def spline_projectors(basis, positions: torch.Tensor, radial: RadialSet,
+                      out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """beta (K, n_atoms * n_proj, n_tot) complex128 of one species on
-    `basis` (qmx_projectors; _ions_projectors.py:28-72), padding zeroed."""
+    `basis` (qmx_projectors; _ions_projectors.py:28-72), padding zeroed;
+    written into `out` (contiguous, that shape) when given."""
     K, G = basis.nk, basis.n_tot
     A = int(positions.shape[0])
-    out = torch.empty((K, A * radial.n_proj, G), dtype=torch.complex128, device=rc.device)
+    shape = (K, A * radial.n_proj, G)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex128, device=rc.device)
+    elif (tuple(out.shape) != shape or out.dtype != torch.complex128
+          or not out.is_contiguous()):
+        raise ValueError(f"out must be a contiguous complex128 tensor of shape {shape}")
     if A == 0 or radial.n_proj == 0 or K == 0:
         return out.zero_()
     iG = getattr(basis, "_iG_i32", None)
