@@ -126,3 +126,17 @@ def test_heev_small_workspace_contract(n):
     assert one % 256 == 0
     for batch in (0, 3, 100):
         assert int(lib.qmx_heev_small_workspace_bytes(n, batch)) == batch * one
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why,
+    before touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
