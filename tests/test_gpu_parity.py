@@ -790,3 +790,28 @@ def test_lcao_start_vs_dense(small_el, l_rf):
     print(f"[lcao {l_rf}] start max {np.abs(eig0[:, :spec.n_bands] - exact).max():.2e},"
           f" iterations {rep.n_iterations}, max |eig - dense| {d:.2e}", flush=True)
     assert d < 1e-9
+
+
+def test_cfg4_chefsi_full_size(gpu):
+    """The 256-atom CheFSI solve at full size (144^3 box, 1024 bands, filter
+    order 20, ten outer iterations at most), by size-independent properties:
+    both forms finish within the iteration limit, their eigenvalues are
+    sorted, and the mixed form (complex64 filter, residual 3e-5, FP64
+    Rayleigh-Ritz) is within 1e-7 Ha of the plain FP64 solve everywhere."""
+    spec = systems.make_cell(4)
+    out = {}
+    for dt, kw in ((C128, {}), (C64, {"residual_threshold": 3e-5})):
+        print(f"[cfg4 chefsi] {dt}", flush=True)
+        el = electrons_from_spec(spec, dtype=dt, **kw)
+        el.initialize_wavefunctions(seed=0)
+        rep = el.solve()
+        eig = rep.eig.cpu().numpy()
+        print(f"[cfg4 chefsi] {dt}: {rep.n_iterations} outer iterations", flush=True)
+        assert rep.n_iterations <= spec.n_iterations
+        assert np.all(np.diff(eig, axis=-1) >= -1e-10)
+        out[dt] = eig
+        del el, rep
+        torch.cuda.empty_cache()
+    d = np.abs(out[C64] - out[C128]).max()
+    print(f"[cfg4 chefsi] max |eig mixed - plain| {d:.2e}", flush=True)
+    assert d < 1e-7
