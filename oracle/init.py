@@ -8,7 +8,8 @@ Restates dft/electrons/_wavefunction_init.py for S = X = 1 on one process:
   b_start + 1 draws (:98-99), one draw per band, zero padding, bandwidth
   limit 1 / (1 + ((4/3) KE)^6) (:120-125);
 - ``randomize_selected``: state 1 + seed + g + n_max (b + n_bands k), one
-  discarded draw (:145-188).
+  discarded draw, padding of every band zeroed, bandwidth limit on the
+  selected bands (:145-188).
 """
 from __future__ import annotations
 
@@ -68,6 +69,7 @@ def randomize_selected(
         )
         randn(x)
         vals = randn(x)
-    vals[basis.pad_mask[i_k]] = 0.0
-    vals *= _bandwidth_limit(basis.KE[i_k])
     C[i_k, i_band] = vals
+    # The padding of every band is zeroed, selected or not (:176-177):
+    C[np.broadcast_to(basis.pad_mask[:, None, :], C.shape)] = 0.0
+    C[i_k, i_band] *= _bandwidth_limit(basis.KE[i_k])

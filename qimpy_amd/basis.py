@@ -126,6 +126,17 @@ class Basis:
         """KE of each plane wave (nk x len(slice)), cached per precision."""
         return self._ke[dtype][:, basis_slice]
 
+    def set_local_path(self, path: int) -> None:
+        """Kernel sequence of the local apply (qmx_plan_set_local_path): 0
+        the fused plane pass where a plane fits LDS, else the separate
+        y, x and y passes; 1 always the separate passes."""
+        _lib.call("qmx_plan_set_local_path", self._plan, int(path))
+
+    def local_path(self, dtype: torch.dtype) -> str:
+        """'plane' or 'separate': the kernel sequence the apply takes."""
+        code = int(_lib.lib().qmx_plan_local_path(self._plan, DTYPE_CODE[dtype]))
+        return {2: "plane", 1: "separate"}[code]
+
     # ---------------------------------------------------------------- operators
     def workspace(self, dtype: torch.dtype, n_bands: int) -> tuple[torch.Tensor, int]:
         need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dtype], n_bands))

@@ -34,9 +34,13 @@ bucket's convergence data.
 
 Each job converges on its own n_bands: the plain (complex128) form by the
 reference's test, eigenvalue change below eig_threshold
-(_davidson.py:126-139); the mixed (complex64) form by residual norms
-||HC - eig C|| below residual_threshold, after which an FP64 Rayleigh-Ritz
-(complex128 H C and eighg) gives its eigenvalues. A job starts as in the
+(_davidson.py:126-139); the mixed (complex64) form when the Kato-Temple
+estimate of every wanted band (qimpy_amd.accuracy, from the FP64-accumulated
+residual norms ||HC - eig C|| and the Ritz gaps) is below half its
+eigenvalue tolerance. Both finish with an FP64 Rayleigh-Ritz (complex128
+overlap, projected H and eigenproblem; the mixed form applies H in
+complex128 first), whose Ritz values are the results and whose residual
+norms give each result's error estimate. A job starts as in the
 reference: xorshift bands with its own seed and basis
 (_wavefunction_init.py:27-125), orthonormalised in FP64, then rotated to
 Ritz vectors, so its result does not depend on the other jobs of its bucket.
@@ -54,6 +58,7 @@ import torch
 from . import _lib, gemm, rc
 from .basis import DTYPE_CODE, REAL_OF
 from .ions import build_projectors
+from .accuracy import eig_error_estimate
 from .linalg import dagger, eigh_grouped, eigh_nosync
 from .systems import signed_freqs
 
@@ -61,6 +66,9 @@ C64, C128 = torch.complex64, torch.complex128
 
 #: Largest sparse-FFT workspace of one bucket; bands beyond it are blocked.
 WORKSPACE_CAP = 4 << 30
+#: A mixed-precision job stops when every wanted band's error estimate is
+#: below this fraction of its tolerance.
+KT_SAFETY = 0.5
 
 
 # ---------------------------------------------------------------- geometry
@@ -130,6 +138,9 @@ class CellResult:
     job_id: int
     eig: np.ndarray  #: float64 (K, n_bands)
     residual_norm: np.ndarray  #: float64 (K, n_bands)
+    #: largest Kato-Temple error estimate (Ha) of the wanted bands after the
+    #: final FP64 Rayleigh-Ritz (qimpy_amd.accuracy)
+    err_bound: float
     Eband: float
     iterations: int
     converged: bool
@@ -315,8 +326,10 @@ class Bucket:
     DROP_SHIFT = 1e6
 
     def __init__(self, shape, nb: int, dtype: torch.dtype, *, residual_threshold: float,
-                 eig_threshold: float, max_iterations: int):
+                 eig_threshold: float, max_iterations: int,
+                 eig_tolerance: Optional[float] = None):
         self.shape = tuple(int(s) for s in shape)
+        self.eig_tolerance = eig_tolerance
         self.N = int(np.prod(self.shape))
         self.nb = int(nb)
         self.dtype = dtype
@@ -449,9 +462,17 @@ class Bucket:
                       j.seed, 0, nb, rc.stream_ptr())
             j.admitted = time.perf_counter()
         lam, W = yield _dot(C, C)
-        keep = lam > 1e-14 * lam[..., -1:].clamp_min(1e-300)
+        keep = lam > 1e-10 * lam[..., -1:].clamp_min(1e-300)
         U = W * torch.where(keep, lam.clamp_min(1e-300).rsqrt(), torch.zeros_like(lam))[..., None, :]
-        C = gemm.rotate(U, C).to(self.dtype)
+        C = gemm.rotate(U, C)
+        # Bandwidth-limited random bands are nearly dependent: a direction
+        # kept at overlap eigenvalue lambda is orthonormal only to about
+        # eps lambda_max / lambda (1e-6 at the cut). A second pass (Loewdin on
+        # the now near-identity overlap, error O(1e-18)) restores C^H C = I to
+        # rounding, which every later step assumes.
+        O1 = _dot(C, C)
+        O1.diagonal(dim1=-2, dim2=-1).add_(torch.where(keep, 0.0, 1.0))  # dropped (zero) columns
+        C = gemm.rotate(_lowdin(O1), C).to(self.dtype)
         HC = self.HX[a0:, :nb]
         self.hamiltonian(C, HC, a0, Kn)
         Hs = _dot(C, HC)
@@ -487,7 +508,7 @@ class Bucket:
             fail[0] += (info != 0).sum()
         self._info = []
         fail[1] = (~torch.isfinite(self.eig)).sum() + (~torch.isfinite(self.rnorm)).sum()
-        return torch.cat([self.rnorm.flatten(), self.deig.flatten(), fail])
+        return torch.cat([self.rnorm.flatten(), self.deig.flatten(), self.eig.flatten(), fail])
 
     def converged_jobs(self, packed: np.ndarray) -> list[CellJob]:
         nb = self.nb
@@ -497,12 +518,20 @@ class Bucket:
         if packed[-1] != 0:
             raise RuntimeError(f"non-finite eigenvalues or residuals ({int(packed[-1])}) in bucket"
                                f" {self.shape}, {nb} bands")
-        rnorm = packed[: self.Kb * nb].reshape(self.Kb, nb)
-        deig = packed[self.Kb * nb : 2 * self.Kb * nb].reshape(self.Kb, nb)
+        n = self.Kb * nb
+        rnorm = packed[:n].reshape(self.Kb, nb)
+        deig = packed[n : 2 * n].reshape(self.Kb, nb)
+        eig = packed[2 * n : 3 * n].reshape(self.Kb, nb)
         done = []
         for j, a in zip(self.jobs, self.k0):
             rows = slice(a, a + j.K)
-            if self.mixed:
+            if self.mixed and self.eig_tolerance is not None:
+                # Kato-Temple estimate of every wanted band against half the
+                # tolerance: the FP64 Rayleigh-Ritz that follows sets the
+                # final eigenvalues and their own estimate.
+                est = eig_error_estimate(eig[rows], rnorm[rows], j.n_bands)
+                ok = est.max() < KT_SAFETY * self.eig_tolerance
+            elif self.mixed:
                 ok = rnorm[rows, : j.n_bands].max() < self.residual_threshold
             else:
                 ok = deig[rows, : j.n_bands].max() < self.eig_threshold
@@ -560,34 +589,36 @@ class Bucket:
     # ------------------------------------------------------------ results
     def finish_steps(self, job: CellJob):
         """Device results of a converged (or exhausted) job, as a generator
-        that yields one Hermitian eigenproblem and returns a _Pending. The
-        mixed form recomputes the job's subspace in FP64 first (complex128
-        H C, FP64 overlap and eigenproblem, the final Rayleigh quotients).
-        The packed result ends with the count of unconverged QL eigensolves
-        (filled in by the solver)."""
+        that yields one Hermitian eigenproblem and returns a _Pending: an
+        FP64 Rayleigh-Ritz on the job's nb bands (overlap, projected H and
+        its eigenproblem in complex128) and the residual norms of the
+        rotated bands, from which the host forms each band's error
+        estimate. The mixed form first applies H in complex128 to its
+        promoted bands; the plain form uses its H C, which every rotation
+        has kept consistent with C. The packed result holds the nb
+        eigenvalues and residual norms of every k-point, then a slot for the
+        count of failed eigensolves (filled in by the solver)."""
         a = self.k0[self.jobs.index(job)]
         rows = slice(a, a + job.K)
-        nb, B = self.nb, job.n_bands
+        nb = self.nb
         if self.mixed:
             C = self.X[rows, :nb].to(C128)
             HC = torch.empty_like(C)
             beta = job.projectors(self.Gb) if job.P else None
             D = job.device_tables()["D"][None, :].expand(job.K, -1) if job.P else None
             self.hamiltonian(C, HC, a, job.K, beta, D)
-            S = _lowdin(_dot(C, C))  # C is orthonormal to float32 rounding
-            Hh = S @ _dot(C, HC) @ S
-            E, W = yield 0.5 * (Hh + dagger(Hh)), B
-            Vs = S @ W[:, :, :B]
-            C = gemm.rotate(Vs, C)
-            HC = gemm.rotate(Vs, HC)
-            eig = E[:, :B]
-            rnorm = torch.linalg.vector_norm(HC - C * eig[:, :, None].to(C128), dim=-1)
         else:
-            eig = self.eig[rows, :B].clone()
-            rnorm = self.rnorm[rows, :B].clone()
+            C, HC = self.X[rows, :nb], self.HX[rows, :nb]
+        S = _lowdin(_dot(C, C))  # C is orthonormal to float32 rounding or better
+        Hh = S @ _dot(C, HC) @ S
+        E, W = yield 0.5 * (Hh + dagger(Hh))
+        Vs = S @ W
+        C = gemm.rotate(Vs, C)
+        HC = gemm.rotate(Vs, HC)
+        rnorm = torch.linalg.vector_norm(HC - C * E[:, :, None].to(C128), dim=-1)
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
-        return _Pending(job, torch.cat([eig.flatten(), rnorm.flatten(),
+        return _Pending(job, torch.cat([E.flatten(), rnorm.flatten(),
                                         torch.zeros(1, dtype=torch.float64, device=rc.device)]), ev)
 
 
@@ -631,12 +662,18 @@ class BatchSolver:
     ticks until every submitted job is done.
     """
 
-    def __init__(self, dtype: torch.dtype = C64, *, residual_threshold: float = 3e-4,
+    def __init__(self, dtype: torch.dtype = C64, *, eig_tolerance: Optional[float] = 1e-6,
+                 residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
                  band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
                  max_k_per_bucket: int = 2048, groups: int = 1):
         rc.require_gpu()
         self.dtype = dtype
+        #: mixed form: stop a job when every wanted band's Kato-Temple error
+        #: estimate is below KT_SAFETY * eig_tolerance (Ha); None stops on
+        #: residual norms below residual_threshold instead. The plain form
+        #: stops on the reference's eigenvalue change below eig_threshold.
+        self.eig_tolerance = eig_tolerance
         self.residual_threshold = residual_threshold
         self.eig_threshold = eig_threshold
         self.max_iterations = max_iterations
@@ -685,7 +722,8 @@ class BatchSolver:
         b = self.buckets.get(key)
         if b is None:
             b = Bucket(key[0], key[1], self.dtype, residual_threshold=self.residual_threshold,
-                       eig_threshold=self.eig_threshold, max_iterations=self.max_iterations)
+                       eig_threshold=self.eig_threshold, max_iterations=self.max_iterations,
+                       eig_tolerance=self.eig_tolerance)
             self.buckets[key] = b
         return b
 
@@ -736,11 +774,13 @@ class BatchSolver:
             K, B = job.K, job.n_bands
             if h[-1] != 0:
                 raise RuntimeError(f"batched QL eigensolver did not converge (job {job.job_id})")
-            eig = h[: K * B].reshape(K, B).copy()
-            rnorm = h[K * B : 2 * K * B].reshape(K, B).copy()
+            nb = (len(h) - 1) // (2 * K)
+            eig_all = h[: K * nb].reshape(K, nb)
+            r_all = h[K * nb : 2 * K * nb].reshape(K, nb)
             res = CellResult(
-                job_id=job.job_id, eig=eig, residual_norm=rnorm,
-                Eband=float(2.0 * (job.wk[:, None] * eig).sum()), iterations=job.iterations,
+                job_id=job.job_id, eig=eig_all[:, :B].copy(), residual_norm=r_all[:, :B].copy(),
+                err_bound=float(eig_error_estimate(eig_all, r_all, B).max()),
+                Eband=float(2.0 * (job.wk[:, None] * eig_all[:, :B]).sum()), iterations=job.iterations,
                 converged=getattr(job, "converged", False), arrival=job.arrival,
                 admitted=(job.admitted - self.t0) if job.admitted else job.arrival,
                 finished=self._t0_event.elapsed_time(p.event) * 1e-3)

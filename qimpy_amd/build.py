@@ -32,19 +32,30 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every source (in parallel, one hipcc per file) and link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIBDIR.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = CSRC / src
         if not path.exists():
             continue
         obj = LIBDIR / (path.stem + ".o")
-        cmd = [HIPCC, *FLAGS, "-c", str(path), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([HIPCC, *FLAGS, "-c", str(path), "-o", str(obj)])
         objs.append(str(obj))
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        done = list(pool.map(lambda c: subprocess.run(c, capture_output=not verbose), cmds))
+    for cmd, r in zip(cmds, done):
+        if r.returncode != 0:
+            if r.stderr:
+                sys.stderr.write(r.stderr.decode(errors="replace"))
+            raise subprocess.CalledProcessError(r.returncode, cmd)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:

@@ -128,6 +128,11 @@ struct qmx_plan {
   void* h_stage;  // pinned staging copy of the block
   size_t block_bytes;    // capacity of d_block and h_stage
   hipEvent_t uploaded;   // recorded after the copy out of h_stage
+  // Stream ordering of the plan's users (plan_enter / plan_leave): the
+  // event after the latest call that read the tables, and its stream.
+  hipEvent_t last_use;
+  hipStream_t last_stream;
+  bool entered;
 };
 
 namespace {
@@ -1163,6 +1168,7 @@ struct PoolBlock {
   void* h;
   size_t bytes;
   hipEvent_t uploaded;
+  hipEvent_t last_use;  // after the last kernel of the previous owner that read d
 };
 static std::mutex g_pool_mu;
 static std::vector<PoolBlock> g_pool;
@@ -1193,10 +1199,11 @@ static int pool_acquire(size_t need, PoolBlock* out) {
     return QMX_OK;
   }
   const size_t bytes = (need + kPoolGrain - 1) / kPoolGrain * kPoolGrain;
-  *out = PoolBlock{nullptr, nullptr, bytes, nullptr};
+  *out = PoolBlock{nullptr, nullptr, bytes, nullptr, nullptr};
   QMX_HIP(hipHostMalloc(&out->h, bytes, hipHostMallocDefault));
   QMX_HIP(hipMalloc(&out->d, bytes));
   QMX_HIP(hipEventCreateWithFlags(&out->uploaded, hipEventDisableTiming));
+  QMX_HIP(hipEventCreateWithFlags(&out->last_use, hipEventDisableTiming));
   return QMX_OK;
 }
 
@@ -1218,14 +1225,18 @@ static void pool_release(const PoolBlock& blk) {
     (void)hipFree(b.d);
     (void)hipHostFree(b.h);
     (void)hipEventDestroy(b.uploaded);
+    (void)hipEventDestroy(b.last_use);
   }
 }
 
 // The integer tables of a plan go to the device as one block: staged in
-// pinned host memory and copied asynchronously on the null stream (which
-// orders with every blocking stream), so creating a plan does not wait for
-// the work already queued on the device. Device work that still reads a
-// reused block was queued before the copy, so the copy follows it.
+// pinned host memory and copied asynchronously on the null stream, so
+// creating a plan does not wait for the work already queued on the device.
+// A reused block may still be read by kernels of its previous plan on any
+// stream (non-blocking streams do not order with the null stream): the copy
+// first waits for the event recorded after that plan's last call. Every
+// entry point that reads a plan waits for `uploaded` on its own stream
+// (plan_enter).
 static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>& src,
                          const std::vector<int**>& dst) {
   std::vector<size_t> off(src.size());
@@ -1243,6 +1254,7 @@ static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>
   p->h_stage = blk.h;
   p->block_bytes = blk.bytes;
   p->uploaded = blk.uploaded;
+  p->last_use = blk.last_use;
   auto* h = static_cast<char*>(p->h_stage);
   auto* d = static_cast<char*>(p->d_block);
   for (size_t i = 0; i < src.size(); ++i) {
@@ -1250,14 +1262,42 @@ static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>
     std::memcpy(h + off[i], src[i]->data(), src[i]->size() * sizeof(int));
     *dst[i] = reinterpret_cast<int*>(d + off[i]);
   }
+  QMX_HIP(hipStreamWaitEvent(nullptr, p->last_use, 0));
   QMX_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, nullptr));
   QMX_HIP(hipEventRecord(p->uploaded, nullptr));
   return QMX_OK;
 }
 
+// Every entry point that reads a plan's device tables brackets its launches
+// on stream st with these: before them, st waits for the tables' upload and,
+// when the previous call ran on another stream, for that call's work (so a
+// plan moved between streams stays ordered); after them, last_use marks the
+// point a reused block's next upload must wait for.
+static int plan_enter(const qmx_plan* cp, hipStream_t st) {
+  auto* p = const_cast<qmx_plan*>(cp);
+  if (p->entered && p->last_stream == st) return QMX_OK;
+  QMX_HIP(hipStreamWaitEvent(st, p->uploaded, 0));
+  if (p->entered) QMX_HIP(hipStreamWaitEvent(st, p->last_use, 0));
+  p->entered = true;
+  p->last_stream = st;
+  return QMX_OK;
+}
+
+static int plan_leave(const qmx_plan* p, hipStream_t st, int rc) {
+  if (p->last_use) {
+    const hipError_t e = hipEventRecord(p->last_use, st);
+    if (e != hipSuccess && rc == QMX_OK) {
+      set_error(std::string("hipEventRecord: ") + hipGetErrorString(e));
+      return QMX_ERR_HIP;
+    }
+  }
+  return rc;
+}
+
 static void free_plan(qmx_plan* p) {
   if (!p) return;
-  if (p->d_block) pool_release(PoolBlock{p->d_block, p->h_stage, p->block_bytes, p->uploaded});
+  if (p->d_block)
+    pool_release(PoolBlock{p->d_block, p->h_stage, p->block_bytes, p->uploaded, p->last_use});
   delete[] p->h_nsum;
   delete p;
 }
@@ -1489,17 +1529,18 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
   QMX_CHECK_ARG(ldb >= plan->n_tot && out_ldb >= plan->n_tot, "qmx_apply_local: ldb < n_tot");
   QMX_CHECK_ARG(nk == 1 || (ldk > 0 && out_ldk > 0), "qmx_apply_local: bad k stride");
   QMX_CHECK_ARG(!ke || nk == 1 || ke_ldk >= plan->n_tot, "qmx_apply_local: bad ke stride");
+  QMX_CHECK_ARG(dtype == QMX_C64 || dtype == QMX_C128,
+                "qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = plan_enter(plan, st);
+  if (rc) return rc;
   if (dtype == QMX_C64)
-    return apply_local_impl<float>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk, V,
-                                   ke, ke_ldk, n_bands, accumulate, workspace, workspace_bytes,
-                                   st);
-  if (dtype == QMX_C128)
-    return apply_local_impl<double>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk,
-                                    V, ke, ke_ldk, n_bands, accumulate, workspace,
-                                    workspace_bytes, st);
-  set_error("qmx_apply_local: dtype must be QMX_C64 or QMX_C128");
-  return QMX_ERR_ARG;
+    rc = apply_local_impl<float>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk, V,
+                                 ke, ke_ldk, n_bands, accumulate, workspace, workspace_bytes, st);
+  else
+    rc = apply_local_impl<double>(plan, (int)k0, (int)nk, C, ldb, ldk, out, out_ldb, out_ldk, V,
+                                  ke, ke_ldk, n_bands, accumulate, workspace, workspace_bytes, st);
+  return plan_leave(plan, st, rc);
 }
 
 int64_t qmx_density_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
@@ -1555,11 +1596,15 @@ static int collect_density_entry(const char* name, const qmx_plan* plan, int64_t
     QMX_LAUNCHED();
     return QMX_OK;
   }
+  int rc = plan_enter(plan, st);
+  if (rc) return rc;
   if (dtype == QMX_C64)
-    return collect_density_impl<float>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
-                                       ldp, rho, accumulate, workspace, workspace_bytes, st);
-  return collect_density_impl<double>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
+    rc = collect_density_impl<float>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
+                                     ldp, rho, accumulate, workspace, workspace_bytes, st);
+  else
+    rc = collect_density_impl<double>(plan, (int)k0, (int)nk, C0, C1, ldb, ldk, n_bands, prefac,
                                       ldp, rho, accumulate, workspace, workspace_bytes, st);
+  return plan_leave(plan, st, rc);
 }
 
 int qmx_collect_density(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, const void* C,

@@ -218,8 +218,8 @@ class Wavefunction:
         vals = randn(x)
         ke = basis.get_ke()[i_k]
         vals = vals * (1.0 / (1.0 + ((4.0 / 3) * ke) ** 6))
-        vals[torch.arange(G, device=dev)[None, :] >= basis.n[i_k][:, None]] = 0.0
         self.coeff[i_spin, i_k, i_band, 0] = vals.to(self.coeff.dtype)
+        self.constrain()  # the padding of every band, as the reference (:176-177)
         self._proj_invalidate()
 
     # ------------------------------------------------------------ algebra
