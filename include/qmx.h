@@ -360,6 +360,42 @@ int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, in
             int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc,
             int64_t sc, int accumulate, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Band-space rotation and back-projection on the matrix cores:
+ *   out[b][i][g] = alpha sum_k op(M)[b][i][k] W[b][k][g] + beta out[b][i][g]
+ * op(M) = M (m x K, row stride ldm) or, with trans_m, M^T (M stored K x m);
+ * W: K rows of G plane waves (row stride ldw, unit stride in g); out: m rows
+ * (row stride ldo); batch strides sm, sw, so in elements. dtype is the type
+ * of W and out (QMX_C64, QMX_C128), m_dtype that of M (either; it is
+ * rounded to the arithmetic type). variant 0: FP32 MFMA for complex64,
+ * FP64 MFMA for complex128; variant 1 (complex64 only): BF16 x 3 split
+ * precision (each FP32 operand as a BF16 hi + lo pair, three BF16 products
+ * per real product, FP32 accumulation), relative error ~2^-16 per product.
+ * out must not overlap W or M. Replaces _matmul (the C @ M of
+ * Wavefunction, dft/electrons/_wavefunction_dot.py:138-191), the nonlocal
+ * back-projection beta (D beta^H C) (_hamiltonian.py:21-24) and the
+ * Davidson rotations (_davidson.py:212-215). */
+int qmx_gemm(int dtype, int m_dtype, int variant, int trans_m, int64_t batch, int64_t m,
+             int64_t K, int64_t G, double alpha, const void* M, int64_t ldm, int64_t sm,
+             const void* W, int64_t ldw, int64_t sw, double beta, void* out, int64_t ldo,
+             int64_t so, void* stream);
+
+/* The same product in BLAS shape (row-major, batched):
+ *   C[b] (m x n) = alpha op(A[b]) (m x k) B[b] (k x n) + beta C[b],
+ * op(A) = A^T when transA; B untransposed (transB must be 0: its rows of n
+ * plane waves are the long axis). A, B and C share the entry point's type.
+ * The qmx_gemm_{c64, c64_bf16x3, c128} family of the band algebra
+ * (_wavefunction_dot.py:138-191, _hamiltonian.py:21-24). */
+int qmx_gemm_c64(int transA, int transB, int64_t m, int64_t n, int64_t k, double alpha,
+                 const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
+                 double beta, void* C, int64_t ldc, int64_t sC, int64_t batch, void* stream);
+int qmx_gemm_c64_bf16x3(int transA, int transB, int64_t m, int64_t n, int64_t k, double alpha,
+                        const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb,
+                        int64_t sB, double beta, void* C, int64_t ldc, int64_t sC, int64_t batch,
+                        void* stream);
+int qmx_gemm_c128(int transA, int transB, int64_t m, int64_t n, int64_t k, double alpha,
+                  const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
+                  double beta, void* C, int64_t ldc, int64_t sC, int64_t batch, void* stream);
+
 /* Kernel ids reported by the profiler. */
 enum {
   QMX_KERNEL_PASS_A = 0, /* sphere -> columns, inverse z */
@@ -373,6 +409,8 @@ enum {
   QMX_KERNEL_VZMAJOR = 8, /* potential boxes to z-major order */
   QMX_KERNEL_APPLY = 9,   /* whole local apply, dense-transform byte convention */
   QMX_KERNEL_PROJ = 10,   /* qmx_projectors (complex128 output: reported as 10 + C128) */
+  QMX_KERNEL_ROTATE = 11,         /* qmx_gemm, FP32 / FP64 MFMA */
+  QMX_KERNEL_ROTATE_BF16X3 = 12,  /* qmx_gemm, BF16 x 3 on MFMA */
   QMX_KERNEL_C128 = 16,   /* added to the id of a complex128 launch */
   QMX_N_KERNEL_IDS = 32
 };

@@ -92,6 +92,14 @@ SIGNATURES = {
     "qmx_canonical_basis": (i32, [i32, vp, vp, i64, i64, ct.c_double, vp, vp]),
     "qmx_subspace_assemble": (i32, [vp, vp, vp, vp, i64, i64, ct.c_double, ct.c_double, vp, vp,
                                     vp]),
+    "qmx_gemm": (i32, [i32, i32, i32, i32, i64, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64,
+                       i64, ct.c_double, vp, i64, i64, vp]),
+    "qmx_gemm_c64": (i32, [i32, i32, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64, i64,
+                           ct.c_double, vp, i64, i64, i64, vp]),
+    "qmx_gemm_c64_bf16x3": (i32, [i32, i32, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64,
+                                  i64, ct.c_double, vp, i64, i64, i64, vp]),
+    "qmx_gemm_c128": (i32, [i32, i32, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64, i64,
+                            ct.c_double, vp, i64, i64, i64, vp]),
     "qmx_profile_enable": (i32, [i32]),
     "qmx_profile_read": (i32, [vp, vp, vp, i32]),
     "qmx_last_error": (ct.c_char_p, []),
@@ -193,7 +201,7 @@ _BASE_NAMES = {
     3: "k_planes_to_cols", 4: "k_cols_to_sphere", 5: "k_fft", 6: "k_gemm",
     7: "k_plane_potential", 8: "k_potential_zmajor",
     9: "local_apply",  # the whole apply (a span of launches, not one kernel)
-    10: "k_projectors",
+    10: "k_projectors", 11: "k_rotate", 12: "k_rotate_bf16x3",
 }
 #: Profile ids: the kernel id for complex64 launches, + 16 for complex128.
 KERNEL_NAMES = {**{i: f"{n}<c64>" for i, n in _BASE_NAMES.items()},

@@ -438,9 +438,9 @@ class Bucket:
         if self.Pb:
             if beta is None:
                 beta, D = self.beta[k0 : k0 + nk], self.D[k0 : k0 + nk]
-            proj = _dot(beta, Cin)  # (nk, P, B)
+            proj = _dot(beta, Cin)  # (nk, P, B), complex128
             proj *= D[:, :, None].to(proj.real.dtype)
-            out.baddbmm_(proj.to(dt).transpose(-2, -1), beta)
+            gemm.rotate(proj, beta, out=out, beta=1.0)  # out += beta^T (D P)
 
     # ------------------------------------------------------------ job start
     def init_steps(self, new: list[CellJob]):
@@ -478,9 +478,8 @@ class Bucket:
         Hs = _dot(C, HC)
         Hs.diagonal(dim1=-2, dim2=-1).add_(torch.where(keep, 0.0, self.DROP_SHIFT))
         E, Vr = yield 0.5 * (Hs + dagger(Hs))
-        Vr = Vr.to(self.dtype)
-        torch.matmul(Vr.transpose(-2, -1), C, out=self.X[a0:, :nb])
-        self.HX[a0:, :nb] = torch.matmul(Vr.transpose(-2, -1), HC)
+        gemm.rotate(Vr, C, out=self.X[a0:, :nb])
+        self.HX[a0:, :nb] = gemm.rotate(Vr, HC)  # HC is these rows: rotate out of place
         self.eig[a0:] = E
 
     # ------------------------------------------------------------ iteration
@@ -553,14 +552,14 @@ class Bucket:
         # overflow float32) is zeroed and then dropped with the null space:
         _normalize_constrain(R, self.norm, self.n32)
         for _ in range(2):  # R <- (1 - C C^H) R, twice
-            R.baddbmm_(_dot(C, R).to(dt).transpose(-2, -1), C, alpha=-1)
+            gemm.rotate(_dot(C, R), C, out=R, alpha=-1.0, beta=1.0)
         # Canonical orthonormalisation of R (drops near-dependent directions):
         lam, W = yield _dot(R, R)
         tol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
         Ut = torch.empty((self.Kb, nb, nb), dtype=dt, device=rc.device)
         _lib.call("qmx_canonical_basis", DTYPE_CODE[dt], _c(W), _c(lam), self.Kb, nb, tol, _c(Ut),
                   rc.stream_ptr())
-        R.copy_(torch.matmul(Ut, R))
+        R.copy_(gemm.rotate(Ut.transpose(-2, -1), R))  # Ut @ R
         self.hamiltonian(R, HR, 0, self.Kb)
         XO = _dot(self.X, R)  # (Kb, 2 nb, nb): [C^H R; R^H R]
         XH = _dot(self.X, HR)
@@ -574,9 +573,9 @@ class Bucket:
         S.diagonal(dim1=-2, dim2=-1).add_(1.0)
         # The eigensolver reads the lower triangle of S H S.
         E, W2 = yield S @ H @ S, nb  # only the nb lowest eigenvectors
-        Vt = (S @ W2[:, :, :nb]).to(dt).transpose(-2, -1)
-        torch.matmul(Vt, self.X, out=self.X2[:, :nb])
-        torch.matmul(Vt, self.HX, out=self.HX2[:, :nb])
+        V = S @ W2[:, :, :nb]  # (Kb, 2 nb, nb) complex128
+        gemm.rotate(V, self.X, out=self.X2[:, :nb])
+        gemm.rotate(V, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
         self.HX, self.HX2 = self.HX2, self.HX
         E = E[:, :nb]

@@ -94,7 +94,7 @@ class Electrons:
             beta_C = C.proj  # (S, K, P, B), cached on C
             DP = ions.D_diag.to(beta_C.real.dtype)[:, None] * beta_C
             beta = ions.beta_coeff(C.coeff.dtype).flatten(-2)  # (1, K, P, G)
-            HC.coeff.flatten(-2).add_(gemm.rotate(DP, beta))
+            gemm.rotate(DP, beta, out=HC.coeff.flatten(-2), beta=1.0)  # HC += beta (D P)
         return HC
 
     # --------------------------------------------------------------- set-up

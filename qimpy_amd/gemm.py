@@ -6,11 +6,15 @@ Two shapes cover the path (reference: _wavefunction_dot.py):
   small output: ``dot128`` runs it on the library's split-K MFMA kernel
   (qmx_dot) with an FP64 result;
 - ``rotate(M, C)``: (..., Nout, G) = M(..., Nin, Nout)^T @ C(..., Nin, G) —
-  subspace rotations C V and the nonlocal beta (D P) (_matmul, :168-185), a
-  bandwidth-bound batched library GEMM in the precision of C.
+  subspace rotations C V, the nonlocal beta (D P) and Gram-Schmidt updates
+  (_matmul, :168-185; _hamiltonian.py:21-24) on the library's MFMA kernel
+  (qmx_gemm): FP32 MFMA for complex64, FP64 MFMA for complex128, or BF16 x 3
+  split precision for complex64 when enabled (``set_bf16x3``).
 Both operands keep the basis index G contiguous.
 """
 from __future__ import annotations
+
+from typing import Optional
 
 import torch
 
@@ -82,7 +86,86 @@ def dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     return dot128(A, B).to(A.dtype)
 
 
-def rotate(M: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
-    if M.dtype != C.dtype:
+#: Arithmetic of complex64 rotations: 0 FP32 MFMA, 1 BF16 x 3 split precision.
+_variant_c64 = 0
+
+
+def set_bf16x3(on: bool) -> None:
+    """Run complex64 rotations as BF16 x 3 split-precision products (three
+    BF16 MFMAs per real product, FP32 accumulation) instead of FP32 MFMA."""
+    global _variant_c64
+    _variant_c64 = 1 if on else 0
+
+
+def bf16x3_enabled() -> bool:
+    return bool(_variant_c64)
+
+
+def _batch_view(X: torch.Tensor, batch: tuple) -> tuple[torch.Tensor, int]:
+    """X broadcast to the leading dims `batch`, with the one batch stride
+    that walks them in row-major order (0 for a single batch or an operand
+    broadcast along every batch axis); a contiguous copy when the leading
+    axes do not fold into one uniformly strided axis."""
+    X = X.expand(batch + X.shape[-2:])
+    axes = [i for i in range(len(batch)) if batch[i] != 1]
+    if not axes:
+        return X, 0
+    step = X.stride(axes[-1])
+    expect = step
+    for i in reversed(axes):
+        if X.stride(i) != expect:
+            X = X.contiguous()
+            return X, X.shape[-2] * X.shape[-1]
+        expect *= batch[i]
+    return X, step
+
+
+def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
+           alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+    """out = alpha M^T @ C + beta out over the last two axes, batched over the
+    leading axes (broadcast), on qmx_gemm. M: (..., Nin, Nout) complex (any
+    precision, any strides with one unit-stride axis); C: (..., Nin, G)
+    complex64 or complex128 with unit stride along G; out: (..., Nout, G) of
+    C's type with unit stride along G (allocated when None; beta is then 0).
+    out must not overlap M or C."""
+    if C.dtype not in _DTYPE:
+        raise TypeError(f"rotate: complex64 or complex128 operand, not {C.dtype}")
+    if not M.is_complex():
         M = M.to(C.dtype)
-    return torch.matmul(M.transpose(-2, -1), C)
+    M, C = _resolved(M), _resolved(C)
+    Nin, Nout = M.shape[-2:]
+    if C.shape[-2] != Nin:
+        raise ValueError(f"rotate: M {tuple(M.shape)} does not match C {tuple(C.shape)}")
+    G = C.shape[-1]
+    batch = torch.broadcast_shapes(M.shape[:-2], C.shape[:-2])
+    if out is None:
+        out = torch.empty(batch + (Nout, G), dtype=C.dtype, device=C.device)
+        beta = 0.0
+    elif out.shape != batch + (Nout, G) or out.dtype != C.dtype:
+        raise ValueError(f"rotate: out {tuple(out.shape)} {out.dtype} != {batch + (Nout, G)}")
+    if C.stride(-1) != 1:
+        C = C.contiguous()
+    if M.stride(-1) == 1:
+        trans, ldm = 1, M.stride(-2)  # op(M)[i][k] = M[k][i], i contiguous
+    elif M.stride(-2) == 1:
+        trans, ldm = 0, M.stride(-1)
+    else:
+        M = M.contiguous()
+        trans, ldm = 1, M.stride(-2)
+    if out.stride(-1) != 1:
+        raise ValueError("rotate: out needs unit stride along G")
+    nb = 1
+    for d in batch:
+        nb *= int(d)
+    Mb, sm = _batch_view(M, batch)
+    Cb, sw = _batch_view(C, batch)
+    ob, so = _batch_view(out, batch)
+    if ob.data_ptr() != out.data_ptr() or (nb > 1 and so == 0):
+        raise ValueError("rotate: out must have one uniform batch stride")
+    if Mb is not M:  # a copy changed its strides
+        trans, ldm = (1, Mb.stride(-2)) if Mb.stride(-1) == 1 else (0, Mb.stride(-1))
+    variant = _variant_c64 if C.dtype == torch.complex64 else 0
+    _lib.call("qmx_gemm", _DTYPE[C.dtype], _DTYPE[Mb.dtype] if Mb.dtype in _DTYPE else 1, variant,
+              trans, nb, Nout, Nin, G, float(alpha), Mb.data_ptr(), ldm, sm, Cb.data_ptr(),
+              Cb.stride(-2), sw, float(beta), out.data_ptr(), out.stride(-2), so, rc.stream_ptr())
+    return out
