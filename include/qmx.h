@@ -81,8 +81,11 @@ int qmx_plan_info(const qmx_plan* plan, int64_t* info);
 
 /* Kernel sequence of qmx_apply_local. 0 (default): the fused plane pass
  * (one kernel for the y and x transforms and the potential, each x-y plane
- * in LDS) wherever a plane fits, else separate y, x and y passes; 1: always
- * the separate passes; 2: same as 0. qmx_plan_local_path returns the path
+ * in LDS; a kernel specialised for the plane size where one exists:
+ * square planes of 24..64 points per side) wherever a plane fits, else
+ * separate y, x and y passes; 1: always the separate passes; 2: same as 0;
+ * 3: the plane pass, always the kernel for general sizes (for comparisons).
+ * qmx_plan_local_path returns the path
  * taken for a precision (2 plane, 1 separate, -1 bad argument). Set it
  * before sizing the workspace. No reference counterpart (the reference
  * calls a library 3D FFT, _basis_ops.py:152-205). */

@@ -47,6 +47,7 @@ Ritz vectors, so its result does not depend on the other jobs of its bucket.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as ct
 import time
 from dataclasses import dataclass, field
@@ -205,6 +206,24 @@ class CellJob:
             self._dev = dict(k=h["k"], iG=h["iG"], ke=ke, V=h["V"], n=h["n"], pos=h["pos"],
                              D=h["D"])
         return self._dev
+
+    def fresh(self, arrival: Optional[float] = None) -> "CellJob":
+        """A new request for the same cell: shares this job's tables (host
+        and staged device copies), with its solve state reset."""
+        job = copy.copy(self)
+        job.arrival = self.arrival if arrival is None else float(arrival)
+        job.iterations = 0
+        job.admitted = None
+        job.result = None
+        job.converged = False
+        job._beta = None
+        return job
+
+    def stage(self) -> None:
+        """Copy the job's inputs to the device now and drop the pinned host
+        copies (the copies are ordered on the current stream)."""
+        self.device_tables()
+        self._host = None
 
     def release(self) -> None:
         self._dev = None

@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include "qmx_fft_core.h"
+#include "qmx_plane_fixed.h"
 
 namespace qmx {
 
@@ -888,7 +889,12 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
       const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
       const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
       const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
-      if (ntP == kPlaneThreadsSmall)
+      if (p->local_path != 3 && ntP == kPlaneThreadsSmall && plane_fixed_available(nx, ny, ncol)) {
+        const int rc = launch_plane_fixed(sizeof(R) == 8, nx, gP, shP, st, T1, Vt, vslot,
+                                          p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
+                                          tw_of<C2>(p, 1), ncol, nxa, cur);
+        if (rc) return rc;
+      } else if (ntP == kPlaneThreadsSmall)
         hipLaunchKernelGGL((k_plane_potential<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>), gP,
                            dim3(kPlaneThreadsSmall), shP, st, T1, Vt, vslot, p->d_col_ix,
                            p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0), tw_of<C2>(p, 1), rxc,
@@ -1500,7 +1506,7 @@ int64_t qmx_plan_workspace_bytes(const qmx_plan* p, int dtype, int64_t n_bands) 
 }
 
 int qmx_plan_set_local_path(qmx_plan* p, int path) {
-  QMX_CHECK_ARG(p && path >= 0 && path <= 2, "qmx_plan_set_local_path: path must be 0, 1 or 2");
+  QMX_CHECK_ARG(p && path >= 0 && path <= 3, "qmx_plan_set_local_path: path must be 0..3");
   p->local_path = path;
   return QMX_OK;
 }

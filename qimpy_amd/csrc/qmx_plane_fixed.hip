@@ -1,0 +1,227 @@
+// The fused plane pass of the local apply with the plane size fixed at
+// compile time, for the cubic boxes of the batched and served workloads
+// (24, 30, 32, 36, 40, 48, 60 and 64 points per axis; planes up to 64 x 64).
+//
+// Same work as k_plane_potential (qmx_local.hip): one workgroup of 256
+// threads per (z, band, k) x-y plane held in LDS; scatter the plane's active
+// columns, inverse y-transform on the active x rows, inverse x-transform on
+// every y column, multiply by V(x, y, z), forward x, forward y, gather back.
+// Each 1D transform is two in-place Stockham passes whose radices, strides,
+// butterfly counts and LDS pitch are constants here, so the butterfly
+// addressing reduces to constant offsets from one base per butterfly and the
+// index divisions to multiplications by constants. The runtime-general
+// kernel spends about as many VALU instructions on that addressing
+// (including quarter-rate 32-bit multiplies) as on the transforms.
+#include "qmx_fft_core.h"
+#include "qmx_plane_fixed.h"
+
+using namespace qmx;
+
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kColPer = 4;   // active columns staged per thread (ncol <= 1024)
+constexpr int kVPer = 16;    // potential values per thread (planes <= 4096 points)
+
+// First and second radix of a length-N transform (composite radices run in
+// registers, dft_any).
+template <int N> struct Split;
+template <> struct Split<24> { static constexpr int A = 8, B = 3; };
+template <> struct Split<30> { static constexpr int A = 10, B = 3; };
+template <> struct Split<32> { static constexpr int A = 8, B = 4; };
+template <> struct Split<36> { static constexpr int A = 9, B = 4; };
+template <> struct Split<40> { static constexpr int A = 10, B = 4; };
+template <> struct Split<48> { static constexpr int A = 16, B = 3; };
+template <> struct Split<60> { static constexpr int A = 15, B = 4; };
+template <> struct Split<64> { static constexpr int A = 16, B = 4; };
+
+// One in-place Stockham pass of radix R (sub-transform length NS before it)
+// over nl lines of length N. ROWS: line l is row rows[l] of the plane, its
+// points contiguous (consecutive threads take consecutive butterflies of a
+// line); otherwise line l is column l (NL columns, stride LD between points;
+// consecutive threads take consecutive columns). Values are held in
+// registers across the barrier that separates the reads from the writes.
+template <typename C2, int S, int N, int R, int NS, int NLMAX, bool ROWS, int LD>
+__device__ __forceinline__ void fixed_pass(C2* buf, const C2* __restrict__ tw, int nl,
+                                           const int* __restrict__ rows) {
+  constexpr int STRIDE = N / R, TWSTEP = N / (NS * R);
+  constexpr int NB = (STRIDE * NLMAX + kNT - 1) / kNT;
+  constexpr int PS = ROWS ? 1 : LD;  // distance between the points of a line
+  const int total = STRIDE * nl;
+  C2 v[NB][R];
+  int in0[NB], out0[NB];
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int t = threadIdx.x + it * kNT;
+    int j, lo;
+    if constexpr (ROWS) {
+      const int l = t / STRIDE;
+      j = t - l * STRIDE;
+      lo = t < total ? rows[l] * LD : 0;
+    } else {
+      j = t / NLMAX;
+      lo = t - j * NLMAX;
+    }
+    const int k = j % NS;
+    in0[it] = j * PS + lo;
+    out0[it] = ((j - k) * R + k) * PS + lo;
+    if (t < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[in0[it] + r * STRIDE * PS];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C2 w = tw[r * k * TWSTEP];
+          if constexpr (S > 0) w.y = -w.y;
+          v[it][r] = cmul(v[it][r], w);
+        }
+      }
+      dft_any<C2, R, S>(v[it], tw, N);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int t = threadIdx.x + it * kNT;
+    if (t < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[out0[it] + r * NS * PS] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename C2, int S, int N, int NLMAX, bool ROWS, int LD>
+__device__ __forceinline__ void fixed_fft(C2* buf, const C2* tw, int nl, const int* rows) {
+  constexpr int A = Split<N>::A, B = Split<N>::B;
+  static_assert(A * B == N, "radix split");
+  fixed_pass<C2, S, N, A, 1, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+  fixed_pass<C2, S, N, B, A, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+}
+
+template <typename C2, typename R, int NX, int NY>
+__global__ void __launch_bounds__(kNT)
+    k_plane_fixed(C2* __restrict__ T1, const R* __restrict__ Vt, const int* __restrict__ vslot,
+                  const int* __restrict__ col_ix, const int* __restrict__ col_iy,
+                  const int* __restrict__ x_list, const C2* __restrict__ twx_g,
+                  const C2* __restrict__ twy_g, int ncol, int nxa, int nb) {
+  constexpr int LD = NY | 1, NXY = NX * NY;
+  static_assert(NXY <= kNT * kVPer, "plane too large for the potential prefetch");
+  extern __shared__ __align__(16) unsigned char smem_fixed[];
+  const int nz = gridDim.x;
+  C2* P = reinterpret_cast<C2*>(smem_fixed);
+  C2* twx = P + NX * LD;
+  C2* twy = twx + NX;
+  int* xl = reinterpret_cast<int*>(twy + NY);
+  const int z = blockIdx.x, ib = blockIdx.y, k = blockIdx.z;
+  const int tid = threadIdx.x;
+  C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
+  C2 cv[kColPer];
+  int co[kColPer];
+#pragma unroll
+  for (int q = 0; q < kColPer; ++q) {
+    const int c = tid + q * kNT;
+    if (c < ncol) {
+      cv[q] = row[c];
+      co[q] = col_ix[c] * LD + col_iy[c];
+    }
+  }
+  for (int i = tid; i < NX; i += kNT) twx[i] = twx_g[i];
+  for (int i = tid; i < NY; i += kNT) twy[i] = twy_g[i];
+  for (int i = tid; i < nxa; i += kNT) xl[i] = x_list[i];
+  for (int i = tid; i < NX * LD; i += kNT) P[i] = C2{R(0), R(0)};
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kColPer; ++q)
+    if (tid + q * kNT < ncol) P[co[q]] = cv[q];
+  __syncthreads();
+  fixed_fft<C2, +1, NY, NX, true, LD>(P, twy, nxa, xl);
+  // The potential of the plane, in registers across the inverse x transform.
+  const R* Vz = Vt + ((int64_t)vslot[k] * nz + z) * NXY;
+  constexpr int VQ = (NXY + kNT - 1) / kNT;
+  R vv[VQ];
+#pragma unroll
+  for (int q = 0; q < VQ; ++q) {
+    const int i = tid + q * kNT;
+    vv[q] = i < NXY ? Vz[i] : R(0);
+  }
+  fixed_fft<C2, +1, NX, NY, false, LD>(P, twx, NY, nullptr);
+#pragma unroll
+  for (int q = 0; q < VQ; ++q) {
+    const int i = tid + q * kNT;
+    if (i < NXY) {
+      const int x = i / NY, y = i - x * NY;
+      P[x * LD + y] = cscale(P[x * LD + y], vv[q]);
+    }
+  }
+  __syncthreads();
+  fixed_fft<C2, -1, NX, NY, false, LD>(P, twx, NY, nullptr);
+  fixed_fft<C2, -1, NY, NX, true, LD>(P, twy, nxa, xl);
+#pragma unroll
+  for (int q = 0; q < kColPer; ++q)
+    if (tid + q * kNT < ncol) row[tid + q * kNT] = P[co[q]];
+}
+
+template <typename C2, typename R, int N>
+int launch_n(dim3 grid, size_t lds, hipStream_t st, void* T1, const void* Vt, const int* vslot,
+             const int* col_ix, const int* col_iy, const int* x_list, const void* twx,
+             const void* twy, int ncol, int nxa, int nb) {
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_plane_fixed<C2, R, N, N>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_set = lds;
+  }
+  hipLaunchKernelGGL((k_plane_fixed<C2, R, N, N>), grid, dim3(kNT), lds, st,
+                     static_cast<C2*>(T1), static_cast<const R*>(Vt), vslot, col_ix, col_iy,
+                     x_list, static_cast<const C2*>(twx), static_cast<const C2*>(twy), ncol, nxa,
+                     nb);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
+template <typename C2, typename R>
+int launch_any(int n, dim3 grid, size_t lds, hipStream_t st, void* T1, const void* Vt,
+               const int* vslot, const int* col_ix, const int* col_iy, const int* x_list,
+               const void* twx, const void* twy, int ncol, int nxa, int nb) {
+#define QMX_FIXED(N) \
+  case N: return launch_n<C2, R, N>(grid, lds, st, T1, Vt, vslot, col_ix, col_iy, x_list, twx, twy, ncol, nxa, nb);
+  switch (n) {
+    QMX_FIXED(24)
+    QMX_FIXED(30)
+    QMX_FIXED(32)
+    QMX_FIXED(36)
+    QMX_FIXED(40)
+    QMX_FIXED(48)
+    QMX_FIXED(60)
+    QMX_FIXED(64)
+  }
+#undef QMX_FIXED
+  set_error("plane_fixed: no kernel for this plane size");
+  return QMX_ERR_UNSUPPORTED;
+}
+
+}  // namespace
+
+namespace qmx {
+
+bool plane_fixed_available(int nx, int ny, int ncol) {
+  if (nx != ny || ncol > kColPer * kNT) return false;
+  switch (nx) {
+    case 24: case 30: case 32: case 36: case 40: case 48: case 60: case 64: return true;
+  }
+  return false;
+}
+
+int launch_plane_fixed(bool c128, int n, dim3 grid, size_t lds, hipStream_t st, void* T1,
+                       const void* Vt, const int* vslot, const int* col_ix, const int* col_iy,
+                       const int* x_list, const void* twx, const void* twy, int ncol, int nxa,
+                       int nb) {
+  if (c128)
+    return launch_any<double2, double>(n, grid, lds, st, T1, Vt, vslot, col_ix, col_iy, x_list,
+                                       twx, twy, ncol, nxa, nb);
+  return launch_any<float2, float>(n, grid, lds, st, T1, Vt, vslot, col_ix, col_iy, x_list, twx,
+                                   twy, ncol, nxa, nb);
+}
+
+}  // namespace qmx

@@ -38,14 +38,33 @@ class ServeReport:
         return {k: v for k, v in self.__dict__.items() if k != "results"}
 
 
+def stage_requests(specs: Sequence, arrivals: Sequence[float], workers: int = 16) -> list:
+    """Jobs of the requests with their inputs resident in HBM: sphere
+    tables, potential box, k-points, positions and couplings built on the
+    host by `workers` threads and copied to the device (the pinned host
+    copies are dropped). Served with serve(..., jobs=...), a request then
+    enters the solver at its arrival time with nothing left to upload."""
+    def build(i):
+        job = CellJob(specs[i], job_id=i, arrival=float(arrivals[i]))
+        job.stage()
+        return job
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        jobs = list(pool.map(build, range(len(specs))))
+    torch.cuda.synchronize()
+    return jobs
+
+
 def serve(specs: Sequence, arrivals: Sequence[float], solver: BatchSolver, *,
           on_result: Optional[Callable[[CellResult], None]] = None,
-          progress: Optional[Callable[[str], None]] = None, build_workers: int = 4) -> ServeReport:
+          progress: Optional[Callable[[str], None]] = None, build_workers: int = 4,
+          jobs: Optional[Sequence[CellJob]] = None) -> ServeReport:
     """Serve requests `specs[i]` arriving at `arrivals[i]` seconds after the
-    solver's clock start (ascending). A request's job (sphere tables, on the
-    host) is built by a pool of `build_workers` threads from its arrival on,
-    overlapping the solver's device work; built jobs enter the solver in
-    arrival order."""
+    solver's clock start (ascending). Without `jobs`, a request's job
+    (sphere tables, on the host) is built by a pool of `build_workers`
+    threads from its arrival on, overlapping the solver's device work, and
+    its tables are copied to the device at admission; with `jobs` (from
+    stage_requests) the staged job of request i enters at its arrival.
+    Built jobs enter the solver in arrival order."""
     order = np.argsort(np.asarray(arrivals, dtype=np.float64), kind="stable")
     n = len(order)
     nxt = 0
@@ -56,8 +75,11 @@ def serve(specs: Sequence, arrivals: Sequence[float], solver: BatchSolver, *,
             now = solver.clock()
             while nxt < n and arrivals[order[nxt]] <= now:
                 i = int(order[nxt])
-                building.append(pool.submit(CellJob, specs[i], job_id=i,
-                                            arrival=float(arrivals[i])))
+                if jobs is not None:
+                    solver.submit([jobs[i]])
+                else:
+                    building.append(pool.submit(CellJob, specs[i], job_id=i,
+                                                arrival=float(arrivals[i])))
                 nxt += 1
             while building and building[0].done():
                 solver.submit([building.popleft().result()])

@@ -151,13 +151,23 @@ def _make_potential(
     seed: int, shape: Sequence[int], Gbasis: np.ndarray, sigma: float = 1.0,
     amplitude: float = 0.5,
 ) -> np.ndarray:
-    """Gaussian-smoothed white noise, zero mean, max|V| = amplitude."""
+    """Gaussian-smoothed white noise, zero mean, max|V| = amplitude: the
+    noise times exp(-sigma^2 G^2 / 2) in reciprocal space (real transforms
+    on the half box, G^2 from the reciprocal metric)."""
+    import scipy.fft
+
+    shape = tuple(int(s) for s in shape)
     n = int(np.prod(shape))
     noise = splitmix64_normal(seed, _S_V, n).reshape(shape)
-    iG = np.stack(np.meshgrid(*[signed_freqs(s) for s in shape], indexing="ij"), -1)
-    G2 = ((iG @ Gbasis.T) ** 2).sum(-1)
-    Vt = np.fft.fftn(noise) * np.exp(-0.5 * sigma**2 * G2)
-    V = np.fft.ifftn(Vt).real
+    g = Gbasis.T @ Gbasis  # metric: G^2 = sum_ab g_ab iG_a iG_b
+    f0 = signed_freqs(shape[0]).astype(np.float64)[:, None, None]
+    f1 = signed_freqs(shape[1]).astype(np.float64)[None, :, None]
+    f2 = np.arange(shape[2] // 2 + 1, dtype=np.float64)[None, None, :]
+    G2 = (g[0, 0] * f0 * f0 + g[1, 1] * f1 * f1 + g[2, 2] * f2 * f2
+          + 2.0 * (g[0, 1] * f0 * f1 + g[0, 2] * f0 * f2 + g[1, 2] * f1 * f2))
+    Vt = scipy.fft.rfftn(noise)
+    Vt *= np.exp(-0.5 * sigma**2 * G2)
+    V = scipy.fft.irfftn(Vt, s=shape)
     V -= V.mean()
     V *= amplitude / np.abs(V).max()
     return np.ascontiguousarray(V)

@@ -83,15 +83,38 @@ def _check_apply(spec, basis, dtype, n_bands, seed, path):
 
 
 @pytest.mark.parametrize("which", ["small", "cfg0"])
-@pytest.mark.parametrize("path", [0, 1], ids=["auto", "separate"])
+@pytest.mark.parametrize("path", [0, 1, 3], ids=["auto", "separate", "plane_general"])
 @pytest.mark.parametrize("dtype", [C128, C64])
 def test_apply_local_paths(gpu, which, path, dtype):
+    """Path 0 takes the plane kernel specialised for the 36^3 box of the
+    diamond cell and the general plane kernel for the small cell's 12x15
+    planes; path 3 forces the general plane kernel, path 1 the separate
+    passes."""
     spec = systems.small_cell(n_bands=5) if which == "small" else systems.make_cell(0)
     print(f"[apply_local_paths] {which} path {path} {dtype}", flush=True)
     basis = Basis.from_spec(spec)
     basis.set_local_path(path)
     assert basis.local_path(dtype) == ("separate" if path == 1 else "plane")
     _check_apply(spec, basis, dtype, 7, 3, basis.local_path(dtype))
+
+
+@pytest.mark.parametrize("g", [24, 30, 32, 36, 40, 48, 60, 64])
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_apply_local_fixed_planes(gpu, g, dtype):
+    """Every square plane size with a specialised plane kernel, against the
+    oracle and against the general plane kernel on the same bands."""
+    spec = systems.small_cell(shape=(g, g, g), n_bands=4, kmesh=(1, 1, 2), triclinic=False,
+                              seed=100 + g)
+    print(f"[apply_local_fixed_planes] {g}^3 {dtype}", flush=True)
+    basis = Basis.from_spec(spec)
+    _check_apply(spec, basis, dtype, 3, g, "plane (fixed)")
+    ob = OBasis(spec)
+    C = to_wf(basis, rand_coeff(ob, 3, seed=g + 1), dtype)
+    V = torch.as_tensor(spec.V, device="cuda")
+    fixed = basis.apply_potential(V, C).coeff
+    basis.set_local_path(3)
+    general = basis.apply_potential(V, C).coeff
+    assert rel(fixed, general.cpu().numpy()) < TOL_OP[dtype]
 
 
 @pytest.mark.parametrize("g", [24, 30, 36, 40, 48, 60])
