@@ -500,7 +500,8 @@ def main():
         out["accuracy"] = acc = accuracy(mixed, plain)
         print(f"[bench] accuracy: {acc}", flush=True)
     if mixed is not None:
-        out["roofline"], out["kernels"] = roofline(prof, wm["wall_s"] + 1e-9)
+        # The profile covers the whole mixed stream (warm-up windows too).
+        out["roofline"], out["kernels"] = roofline(prof, mixed.wall_s + 1e-9)
         del mixed, plain
         torch.cuda.empty_cache()
         if args.stable_requests:

@@ -244,12 +244,19 @@ class CellJob:
         return beta[..., :G]
 
     def hbm_bytes(self, nb: int, dtype: torch.dtype, G: Optional[int] = None) -> int:
-        """Device bytes this job holds inside a bucket of band class nb."""
-        G = G or self.n_tot
+        """Device bytes this job holds inside a bucket of band class nb whose
+        rows are G plane waves long (the bucket pads every job to its
+        largest basis): X and HX double-buffered, the copy of both a restack
+        makes, GEMM outputs, projectors (complex128 and working copies) and
+        the potential."""
+        G = max(G or 0, self.n_tot)
         cs = torch.empty((), dtype=dtype).element_size()
         state = 2 * 2 * (2 * nb) * G * cs * self.K  # X and HX, double-buffered
+        restack = 2 * (2 * nb) * G * cs * self.K
         temps = 3 * nb * G * cs * self.K + 16 * self.P * G * self.K  # GEMM outputs, beta c128
-        return int(state + temps + self.P * G * cs * self.K + 12 * self.N)
+        return int(state + restack + temps + self.P * G * cs * self.K + 12 * self.N)
+
+
 
 
 # ---------------------------------------------------------------- raw kernels
@@ -684,7 +691,7 @@ class BatchSolver:
                  residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
                  band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
-                 max_k_per_bucket: int = 2048, groups: int = 1):
+                 max_k_per_bucket: int = 2048, max_bucket_bytes: float = 16e9, groups: int = 1):
         rc.require_gpu()
         self.dtype = dtype
         #: mixed form: stop a job when every wanted band's Kato-Temple error
@@ -698,6 +705,10 @@ class BatchSolver:
         self.band_classes = sorted(band_classes) if band_classes else None
         self.hbm_budget = hbm_budget
         self.max_k_per_bucket = max_k_per_bucket
+        #: Largest iteration state (X, HX, double-buffered) of one bucket:
+        #: beyond it a larger batch adds no parallelism the device lacks.
+        self.max_bucket_bytes = max_bucket_bytes
+        self._csize = torch.empty((), dtype=dtype).element_size()
         self.buckets: dict = {}
         self.queue: list[CellJob] = []
         self.results: dict[int, CellResult] = {}
@@ -759,6 +770,12 @@ class BatchSolver:
         taken: dict = {}
         rest = []
         load = None
+        # The budget holds against the bytes live tensors really hold (a
+        # host-side counter of the caching allocator; its cached free blocks
+        # are reused, so they do not count) plus the estimates of the jobs
+        # admitted in this call.
+        reserved = torch.cuda.memory_allocated(rc.device)
+        pending = 0
         for job in self.queue:
             if now is not None and job.arrival > now:
                 rest.append(job)
@@ -774,12 +791,18 @@ class BatchSolver:
                 rest.append(job)
                 continue
             b = self.buckets.get(key)
-            k_in = (b.Kb if b else 0) + sum(j.K for j in taken.get(key, []))
-            need = job.hbm_bytes(key[1], self.dtype)
-            if (self.hbm_in_use + need > self.hbm_budget and self.hbm_in_use > 0) or (
-                    k_in + job.K > self.max_k_per_bucket and k_in > 0):
+            joining = taken.get(key, [])
+            k_in = (b.Kb if b else 0) + sum(j.K for j in joining)
+            G = max([b.Gb if b else 0] + [j.n_tot for j in joining] + [job.n_tot])
+            need = job.hbm_bytes(key[1], self.dtype, G)
+            in_bucket = 4 * (2 * key[1]) * G * self._csize * (k_in + job.K)  # X, HX, buffers
+            over = max(self.hbm_in_use, reserved + pending) + need > self.hbm_budget
+            if (over and self.hbm_in_use > 0) or (
+                    k_in > 0 and (k_in + job.K > self.max_k_per_bucket
+                                  or in_bucket > self.max_bucket_bytes)):
                 rest.append(job)
                 continue
+            pending += need
             self.hbm_in_use += need
             self._held[job.job_id] = need
             taken.setdefault(key, []).append(job)
