@@ -381,42 +381,81 @@ class Bucket:
             _retired_plans.append((_stamp[0], plan))  # destroyed once no work uses it
         self._plan = None
 
+    #: Capacity step of the bucket buffers (k-points), and the factor of
+    #: headroom they are allocated with.
+    K_QUANTUM, K_GROW = 16, 1.25
+
+    def _buffers(self, Kb: int, Gb: int, Pb: int) -> bool:
+        """Make the bucket's buffers hold Kb k-points of Gb plane waves and Pb
+        projectors: X and HX twice over (the iteration state and the buffers
+        the next rotation writes), the projectors, couplings and kinetic
+        energies. They are reused while the jobs fit, so a restack allocates
+        nothing of the iteration state in the steady state of a server (and
+        the caching allocator sees few sizes instead of a new one per
+        restack); the basis length never shrinks and the k-point capacity
+        shrinks only below a third. Returns True when new buffers were
+        allocated (the previous ones, if any, stay readable until replaced)."""
+        kcap, gcap, pcap = getattr(self, "_cap", (0, 0, 0))
+        g_need = max(gcap, -(-Gb // 64) * 64)
+        p_need = max(pcap, -(-Pb // 8) * 8)
+        if Kb <= kcap and g_need == gcap and p_need == pcap and 3 * Kb >= kcap:
+            return False
+        kcap = max(self.K_QUANTUM, -(-int(np.ceil(Kb * self.K_GROW)) // self.K_QUANTUM)
+                   * self.K_QUANTUM)
+        dev, nb, dt = rc.device, self.nb, self.dtype
+        self._cap = (kcap, g_need, p_need)
+        self._full = [torch.zeros((kcap, 2 * nb, g_need), dtype=dt, device=dev) for _ in range(4)]
+        self._beta_full = torch.zeros((kcap, p_need, g_need), dtype=dt, device=dev)
+        self._D_full = torch.zeros((kcap, p_need), dtype=self.rdtype, device=dev)
+        self._ke_full = torch.zeros((kcap, g_need), dtype=torch.float64, device=dev)
+        return True
+
     def _restack(self, keep: list[CellJob], new: list[CellJob]) -> None:
-        """Rebuild the stacked tensors for jobs `keep` (state carried over)
-        followed by `new` (state zero, to be initialised)."""
+        """Rebuild the stacked state for jobs `keep` (state carried over)
+        followed by `new` (state to be initialised by init_steps) in the
+        bucket's reusable buffers."""
         dev = rc.device
         nb, dt = self.nb, self.dtype
         jobs = keep + new
         if not jobs:
             self.jobs, self.k0, self.Kb = [], [], 0
             for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "norm", "beta", "D",
-                         "ke", "ke_r", "n32", "V", "V64", "workspace"):
+                         "ke", "ke_r", "n32", "V", "V64", "workspace", "_full", "_beta_full",
+                         "_D_full", "_ke_full"):
                 setattr(self, name, None)
+            self._cap = (0, 0, 0)
             self._destroy_plan()
             return
-        Gb = max(j.n_tot for j in jobs)
-        Pb = max(j.P for j in jobs)
         Kb = sum(j.K for j in jobs)
         Kkeep = sum(j.K for j in keep)
-        X = torch.zeros((Kb, 2 * nb, Gb), dtype=dt, device=dev)
-        HX = torch.zeros_like(X)
+        old_X, old_HX = getattr(self, "X", None), getattr(self, "HX", None)
+        if not self._buffers(Kb, max(j.n_tot for j in jobs), max(j.P for j in jobs)):
+            # same buffers: the new state goes into the spare pair, which
+            # becomes current ([X, HX, X2, HX2] bases)
+            f = self._full
+            self._full = [f[2], f[3], f[0], f[1]]
+        kcap, Gb, Pb = self._cap
+        Xn, HXn, X2n, HX2n = (b[:Kb] for b in self._full)
         eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
         deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
         if keep:
             where = {id(j): a for j, a in zip(self.jobs, self.k0)}
             idx = torch.cat([torch.arange(where[id(j)], where[id(j)] + j.K, device=dev) for j in keep])
-            g = min(Gb, self.Gb)
-            X[:Kkeep, :nb, :g] = self.X.index_select(0, idx)[:, :nb, :g]
-            HX[:Kkeep, :nb, :g] = self.HX.index_select(0, idx)[:, :nb, :g]
+            g = old_X.shape[-1]
+            Xn[:Kkeep, :nb, :g] = old_X[idx, :nb]
+            HXn[:Kkeep, :nb, :g] = old_HX[idx, :nb]
             eig[:Kkeep] = self.eig.index_select(0, idx)
             deig[:Kkeep] = self.deig.index_select(0, idx)
         k0, acc = [], 0
         for j in jobs:
             k0.append(acc)
             acc += j.K
-        ke = torch.zeros((Kb, Gb), dtype=torch.float64, device=dev)
-        beta = torch.zeros((Kb, Pb, Gb), dtype=dt, device=dev)
-        D = torch.zeros((Kb, Pb), dtype=self.rdtype, device=dev)
+        ke = self._ke_full[:Kb]
+        beta = self._beta_full[:Kb]
+        D = self._D_full[:Kb]
+        ke.zero_()
+        beta.zero_()
+        D.zero_()
         V64 = torch.empty((len(jobs), self.N), dtype=torch.float64, device=dev)
         fi = np.zeros((Kb, Gb), dtype=np.int64)
         v_slot = np.zeros(Kb, dtype=np.int32)
@@ -435,8 +474,8 @@ class Bucket:
         self._plan = _lib.create_plan(geom)
         del keepalive
         self.jobs, self.k0, self.Kb, self.Gb, self.Pb = jobs, k0, Kb, Gb, Pb
-        self.X, self.HX, self.eig, self.deig = X, HX, eig, deig
-        self.X2, self.HX2 = torch.empty_like(X), torch.empty_like(HX)
+        self.X, self.HX, self.X2, self.HX2 = Xn, HXn, X2n, HX2n
+        self.eig, self.deig = eig, deig
         self.ke = ke
         self.ke_r = ke.to(self.rdtype)
         self.n32 = _to_device(n.astype(np.int32))
@@ -447,7 +486,10 @@ class Bucket:
         need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dt], 2 * nb))
         one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], 1))
         size = max(one, min(need, WORKSPACE_CAP))
-        self.workspace = torch.empty(size, dtype=torch.uint8, device=dev)
+        ws = getattr(self, "workspace", None)
+        if ws is None or ws.numel() < size or ws.numel() > 2 * size + (256 << 20):
+            size = -(-size // (64 << 20)) * (64 << 20)  # 64 MB steps: few distinct sizes
+            self.workspace = torch.empty(size, dtype=torch.uint8, device=dev)
 
     # ------------------------------------------------------------ operators
     def hamiltonian(self, Cin: torch.Tensor, out: torch.Tensor, k0: int, nk: int,
@@ -604,6 +646,8 @@ class Bucket:
         gemm.rotate(V, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
         self.HX, self.HX2 = self.HX2, self.HX
+        f = self._full
+        self._full = [f[2], f[3], f[0], f[1]]
         E = E[:, :nb]
         self.deig = (E - self.eig).abs()
         self.eig = E.contiguous()
