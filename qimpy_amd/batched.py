@@ -250,7 +250,7 @@ class CellJob:
         makes, GEMM outputs, projectors (complex128 and working copies) and
         the potential."""
         G = max(G or 0, self.n_tot)
-        cs = torch.empty((), dtype=dtype).element_size()
+        cs = 8 if dtype == C64 else 16
         state = 2 * 2 * (2 * nb) * G * cs * self.K  # X and HX, double-buffered
         restack = 2 * (2 * nb) * G * cs * self.K
         temps = 3 * nb * G * cs * self.K + 16 * self.P * G * self.K  # GEMM outputs, beta c128
@@ -789,6 +789,9 @@ class BatchSolver:
         return c if c <= limit else job.n_bands_max
 
     def submit(self, jobs: Sequence[CellJob]) -> None:
+        """Queue jobs (FIFO); their bucket key is fixed here, once."""
+        for job in jobs:
+            job._key = (job.shape, self.band_class(job))
         self.queue.extend(jobs)
 
     def _bucket(self, key) -> Bucket:
@@ -817,20 +820,25 @@ class BatchSolver:
         # The budget holds against the bytes live tensors really hold (a
         # host-side counter of the caching allocator; its cached free blocks
         # are reused, so they do not count) plus the estimates of the jobs
-        # admitted in this call.
+        # admitted in this call. The queue is FIFO: once the budget is spent
+        # no later job fits, and a full bucket takes no later job of its key.
         reserved = torch.cuda.memory_allocated(rc.device)
         pending = 0
-        for job in self.queue:
+        full: set = set()
+        for pos, job in enumerate(self.queue):
             if now is not None and job.arrival > now:
+                rest.extend(self.queue[pos:])  # arrivals are in order
+                break
+            key = job._key
+            if key in full:
                 rest.append(job)
                 continue
-            key = (job.shape, self.band_class(job))
             if key not in self._group_of:
                 if load is None:
                     load = self._group_loads()
                 g = int(np.argmin(load))
                 self._group_of[key] = g
-                load[g] += float(job.K) * job.n.max() * key[1]
+                load[g] += float(job.K) * job.n_tot * key[1]
             if self._group_of[key] != group:
                 rest.append(job)
                 continue
@@ -839,11 +847,14 @@ class BatchSolver:
             k_in = (b.Kb if b else 0) + sum(j.K for j in joining)
             G = max([b.Gb if b else 0] + [j.n_tot for j in joining] + [job.n_tot])
             need = job.hbm_bytes(key[1], self.dtype, G)
+            if max(self.hbm_in_use, reserved + pending) + need > self.hbm_budget and (
+                    self.hbm_in_use > 0):
+                rest.extend(self.queue[pos:])
+                break
             in_bucket = 4 * (2 * key[1]) * G * self._csize * (k_in + job.K)  # X, HX, buffers
-            over = max(self.hbm_in_use, reserved + pending) + need > self.hbm_budget
-            if (over and self.hbm_in_use > 0) or (
-                    k_in > 0 and (k_in + job.K > self.max_k_per_bucket
-                                  or in_bucket > self.max_bucket_bytes)):
+            if k_in > 0 and (k_in + job.K > self.max_k_per_bucket
+                             or in_bucket > self.max_bucket_bytes):
+                full.add(key)
                 rest.append(job)
                 continue
             pending += need
