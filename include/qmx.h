@@ -363,6 +363,22 @@ int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, in
             int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc,
             int64_t sc, int accumulate, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* qmx_dot without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); no counterpart in the reference. The
+ * chunk partials live in a fixed device array of qmx_dot_noalloc_max_bytes()
+ * (32 MiB) bytes that each GPU holds for the process lifetime; a product
+ * whose partials need more is refused with QMX_ERR_NOMEM (qmx_dot_workspace_
+ * bytes gives the need). It calls no hipMalloc, hipFree or allocating
+ * library routine and records no profiling events. Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (concurrent
+ * streams, threads) race; calls in one stream are ordered. Same arguments,
+ * kernels and results as qmx_dot otherwise. */
+int64_t qmx_dot_noalloc_max_bytes(void);
+int qmx_dot_noalloc(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
+                    int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out,
+                    int64_t ldc, int64_t sc, int accumulate, void* stream);
+
 /* Band-space rotation and back-projection on the matrix cores:
  *   out[b][i][g] = alpha sum_k op(M)[b][i][k] W[b][k][g] + beta out[b][i][g]
  * op(M) = M (m x K, row stride ldm) or, with trans_m, M^T (M stored K x m);

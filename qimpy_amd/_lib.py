@@ -92,6 +92,9 @@ SIGNATURES = {
     "qmx_canonical_basis": (i32, [i32, vp, vp, i64, i64, ct.c_double, vp, vp]),
     "qmx_subspace_assemble": (i32, [vp, vp, vp, vp, i64, i64, ct.c_double, ct.c_double, vp, vp,
                                     vp]),
+    "qmx_dot_noalloc_max_bytes": (i64, []),
+    "qmx_dot_noalloc": (i32, [i32, vp, i64, i64, vp, i64, i64, i64, i64, i64, i64, vp, i64, i64,
+                              i32, vp]),
     "qmx_gemm": (i32, [i32, i32, i32, i32, i64, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64,
                        i64, ct.c_double, vp, i64, i64, vp]),
     "qmx_gemm_c64": (i32, [i32, i32, i64, i64, i64, ct.c_double, vp, i64, i64, vp, i64, i64,

@@ -14,6 +14,7 @@
 // into the complex128 result, so complex64 inputs are accumulated in FP32
 // only within a chunk.
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -462,14 +463,14 @@ int launch_block(const DotPlan& p, dim3 grid, hipStream_t st, const void* A, int
 template <typename T>
 int dot_impl(const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb, int64_t sb,
              int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc, int64_t sc,
-             int accumulate, void* ws, int64_t ws_bytes, hipStream_t st) {
+             int accumulate, void* ws, int64_t ws_bytes, hipStream_t st, bool profile) {
   using C2 = typename Cplx<T>::T;
   const DotPlan p = dot_plan(batch, M, N, G);
   const int64_t need = batch * p.nchunks * M * N * (int64_t)sizeof(C2);
   QMX_CHECK_ARG(ws && ws_bytes >= need, "qmx_dot: workspace too small");
   QMX_CHECK_ARG(batch * p.mtiles * p.ntiles < (1ll << 31), "qmx_dot: too many tiles");
   hipEvent_t ev = nullptr;
-  prof_begin(st, &ev);
+  if (profile) prof_begin(st, &ev);  // events only while profiling is on
   dim3 grid((unsigned)(batch * p.mtiles * p.ntiles), (unsigned)p.nchunks);
   if (p.block) {
     auto go = [&](auto kmax) {
@@ -512,9 +513,10 @@ int64_t qmx_dot_workspace_bytes(int dtype, int64_t batch, int64_t M, int64_t N, 
   return std::max<int64_t>(1, batch * p.nchunks * M * N * (dtype == QMX_C64 ? 8 : 16));
 }
 
-int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
-            int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc,
-            int64_t sc, int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
+static int dot_entry(int dtype, const void* A, int64_t lda, int64_t sa, const void* B,
+                     int64_t ldb, int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G,
+                     void* out, int64_t ldc, int64_t sc, int accumulate, void* workspace,
+                     int64_t workspace_bytes, void* stream, bool profile) {
   QMX_CHECK_ARG(out, "qmx_dot: null output");
   QMX_CHECK_ARG(batch >= 0 && M >= 0 && N >= 0 && G >= 0, "qmx_dot: negative size");
   QMX_CHECK_ARG(M < (1 << 20) && N < (1 << 20), "qmx_dot: M, N too large");
@@ -524,12 +526,66 @@ int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, in
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == QMX_C64)
     return dot_impl<float>(A, lda, sa, B, ldb, sb, batch, M, N, G, out, ldc, sc, accumulate,
-                           workspace, workspace_bytes, st);
+                           workspace, workspace_bytes, st, profile);
   if (dtype == QMX_C128)
     return dot_impl<double>(A, lda, sa, B, ldb, sb, batch, M, N, G, out, ldc, sc, accumulate,
-                            workspace, workspace_bytes, st);
+                            workspace, workspace_bytes, st, profile);
   set_error("qmx_dot: dtype must be QMX_C64 or QMX_C128");
   return QMX_ERR_ARG;
+}
+
+int qmx_dot(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
+            int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc,
+            int64_t sc, int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
+  return dot_entry(dtype, A, lda, sa, B, ldb, sb, batch, M, N, G, out, ldc, sc, accumulate,
+                   workspace, workspace_bytes, stream, true);
+}
+
+// ------------------------------------------------------------ no allocation
+// qmx_dot_noalloc keeps its chunk partials in a fixed device array instead
+// of a caller workspace. Each GPU holds its own copy of the array for the
+// process lifetime; host code never uses its name as an address: the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current.
+constexpr int64_t kDotScratchBytes = int64_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kDotScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int dot_scratch(void** out) {
+  int dev = 0;
+  QMX_HIP(hipGetDevice(&dev));
+  QMX_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "qmx_dot_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    QMX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    QMX_CHECK_ARG(ptr != nullptr, "qmx_dot_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return QMX_OK;
+}
+
+int64_t qmx_dot_noalloc_max_bytes(void) { return kDotScratchBytes; }
+
+int qmx_dot_noalloc(int dtype, const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
+                    int64_t sb, int64_t batch, int64_t M, int64_t N, int64_t G, void* out,
+                    int64_t ldc, int64_t sc, int accumulate, void* stream) {
+  const int64_t need = qmx_dot_workspace_bytes(dtype, batch, M, N, G);
+  QMX_CHECK_ARG(need >= 0, "qmx_dot_noalloc: bad dtype or negative size");
+  if (need > kDotScratchBytes) {
+    set_error("qmx_dot_noalloc: the chunk partials need " + std::to_string(need) +
+              " bytes, more than its fixed scratch of " + std::to_string(kDotScratchBytes) +
+              " bytes (use qmx_dot with a workspace)");
+    return QMX_ERR_NOMEM;
+  }
+  void* ws = nullptr;
+  const int rc = dot_scratch(&ws);
+  if (rc) return rc;
+  return dot_entry(dtype, A, lda, sa, B, ldb, sb, batch, M, N, G, out, ldc, sc, accumulate, ws,
+                   kDotScratchBytes, stream, false);
 }
 
 }  // extern "C"
