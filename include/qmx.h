@@ -229,6 +229,19 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb,
                          int64_t out_ldk, int64_t nk, int64_t nb, int64_t n_tot,
                          double* norm2, double* rnorm2, void* stream);
 
+/* Linear combination of three wavefunction blocks of the same layout:
+ *   out(k, b, g) = a[k, b] X(k, b, g) + b[k, b] Y(k, b, g) + c[k, b] Z(k, b, g)
+ * with the coefficients device doubles at [k * cf_ldk + b * cf_ldb]
+ * (cf_ldb = 0 for one coefficient per k-point); b or c NULL drops its term
+ * (Y, Z may then be NULL); out may alias X, Y or Z. All blocks have the
+ * strides ldb, ldk. The axpy-type wavefunction arithmetic
+ * (_wavefunction_arithmetic.py:9-109) fused per step: one Chebyshev
+ * filter step Y_{n+1} = (2 s'/e)(H Y_n - c Y_n) - s s' Y_{n-1} of CheFSI
+ * (_chefsi.py:122-140) is a single call. */
+int qmx_axpby(int dtype, int64_t nk, int64_t nb, int64_t n_tot, const double* a, const double* b,
+              const double* c, int64_t cf_ldk, int64_t cf_ldb, const void* X, const void* Y,
+              const void* Z, void* out, int64_t ldb, int64_t ldk, void* stream);
+
 /* Scale each band by scale[k, b] (double, or 1 if NULL) and zero the
  * padding g >= n[k] (n: device int32 [nk]) — Wavefunction.constrain
  * (_wavefunction.py:264-275) fused with the band normalisation of

@@ -60,6 +60,7 @@ SIGNATURES = {
         [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp, vp],
     ),
     "qmx_scale_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
+    "qmx_axpby": (i32, [i32, i64, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, vp, i64, i64, vp]),
     "qmx_apply_gradient": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, i32, ct.c_double, vp, i64,
                                  i64, i32, vp]),
     "qmx_randomize": (

@@ -180,6 +180,35 @@ __global__ void __launch_bounds__(256)
   *o = v;
 }
 
+// out = a X + b Y + c Z on (k, band, g) rows, a, b, c per (k, band) in
+// double (c, Z optional); one workgroup per row, out may alias an input.
+template <typename C2>
+__global__ void __launch_bounds__(kRowThreads)
+    k_axpby(const double* __restrict__ ca, const double* __restrict__ cb,
+            const double* __restrict__ cc, int64_t cf_ldk, int64_t cf_ldb, const C2* X,
+            const C2* Y, const C2* Z, C2* out, int64_t ldb, int64_t ldk, int n_tot) {
+  using R = decltype(C2{}.x);
+  const int b = blockIdx.x, k = blockIdx.y;
+  const int64_t cf = k * cf_ldk + (int64_t)b * cf_ldb;
+  const int64_t row = k * ldk + (int64_t)b * ldb;
+  const R a = (R)ca[cf], bb = cb ? (R)cb[cf] : R(0), c = (cc && Z) ? (R)cc[cf] : R(0);
+  for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+    const C2 x = X[row + g];
+    C2 v{a * x.x, a * x.y};
+    if (cb) {
+      const C2 y = Y[row + g];
+      v.x += bb * y.x;
+      v.y += bb * y.y;
+    }
+    if (cc && Z) {
+      const C2 z = Z[row + g];
+      v.x += c * z.x;
+      v.y += c * z.y;
+    }
+    out[row + g] = v;
+  }
+}
+
 }  // namespace
 
 #define QMX_DISPATCH(dtype, KERNEL_CALL_F32, KERNEL_CALL_F64)          \
@@ -295,6 +324,29 @@ int qmx_apply_gradient(int dtype, const void* C, int64_t ldb, int64_t ldk, int64
                hipLaunchKernelGGL((k_apply_gradient<double2, double>), grid, dim3(256), 0, st,
                                   (const double2*)C, ldb, ldk, (double2*)out, out_ldb, out_ldk,
                                   (int)n_tot, (const double*)G, dir, alpha, accumulate));
+  return QMX_OK;
+}
+
+int qmx_axpby(int dtype, int64_t nk, int64_t nb, int64_t n_tot, const double* a, const double* b,
+              const double* c, int64_t cf_ldk, int64_t cf_ldb, const void* X, const void* Y,
+              const void* Z, void* out, int64_t ldb, int64_t ldk, void* stream) {
+  QMX_CHECK_ARG(nk >= 0 && nb >= 0 && n_tot >= 0, "qmx_axpby: negative size");
+  if (nk == 0 || nb == 0 || n_tot == 0) return QMX_OK;
+  QMX_CHECK_ARG(a && X && out, "qmx_axpby: null argument");
+  QMX_CHECK_ARG(!b || Y, "qmx_axpby: coefficient b without Y");
+  QMX_CHECK_ARG(ldb >= n_tot && (nk == 1 || ldk >= ldb * nb), "qmx_axpby: bad strides");
+  QMX_CHECK_ARG(nk < 65536 && nb < (1ll << 31), "qmx_axpby: too many rows");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_axpby<float2>, grid, dim3(kRowThreads), 0, st, a, b, c,
+                                  cf_ldk, cf_ldb, static_cast<const float2*>(X),
+                                  static_cast<const float2*>(Y), static_cast<const float2*>(Z),
+                                  static_cast<float2*>(out), ldb, ldk, (int)n_tot),
+               hipLaunchKernelGGL(k_axpby<double2>, grid, dim3(kRowThreads), 0, st, a, b, c,
+                                  cf_ldk, cf_ldb, static_cast<const double2*>(X),
+                                  static_cast<const double2*>(Y), static_cast<const double2*>(Z),
+                                  static_cast<double2*>(out), ldb, ldk, (int)n_tot));
   return QMX_OK;
 }
 

@@ -1,12 +1,29 @@
 """Davidson diagonalization (reference: dft/electrons/_davidson.py).
 
-Same algorithm and side effects as the reference: it mutates
-``electrons.C``, ``electrons.eig`` (float64 (S, K, n)) and
-``electrons.deig_max``. The residual, inverse-kinetic preconditioner and
-expansion norms run as one fused HIP pass (``qmx_precond_residual``), the
-normalisation and padding constraint as another (``qmx_scale_constrain``);
-H*psi, the overlaps and rotations run on the GPU in the wavefunction
-precision, and eighg/eigh always in FP64.
+Drop-in for QimPy's Davidson: the same constructor, ``__call__`` /
+``diagonalize(n_iterations=None, eig_threshold=None)``, and the same side
+effects on ``electrons.C``, ``electrons.eig`` (float64 (S, K, n)) and
+``electrons.deig_max``; the same sequence of steps per iteration, so the
+iteration count and eigenvalue trajectory follow the reference's
+(_davidson.py:141-235, checked against the oracle's restatement). The
+implementation is organised differently:
+
+- ``_Targets`` fixes, once per call, the band counts, the iteration limit,
+  the eigenvalue threshold and whether the call is an inner loop (an
+  explicit limit or threshold from a caller that is not a helper, whose
+  target is ``n_bands_min``);
+- each iteration is ``_expansion`` (the preconditioned residuals of the
+  unconverged bands, fused into one HIP pass, then regularised and
+  normalised), ``_bordered`` (the projected overlap and Hamiltonian of
+  [C, C_exp] from one product each, with C's own blocks known: identity and
+  diag(eig)), the FP64 generalized eigensolve, and ``_rotate`` (C and H C
+  into the new Ritz basis, truncated to n_bands + n_bands_extra);
+- convergence counts the leading bands whose eigenvalue change is below the
+  threshold in every k-point and spin (_davidson.py:126-139).
+
+The mixed form (complex64 bands) adds what float32 overlaps need: the
+expansion projected out of span(C) twice, a canonical fallback of the
+generalized eigensolve, and optionally a stop on residual norms.
 """
 from __future__ import annotations
 
@@ -53,6 +70,44 @@ class _PhaseTimer:
         return self.times
 
 
+@dataclass
+class _Targets:
+    """What one call of the diagonalizer aims for."""
+
+    n_bands: int  #: bands reported (fillings.n_bands)
+    n_keep: int  #: bands carried between iterations (n_bands + n_bands_extra)
+    n_goal: int  #: converged leading bands that end the call
+    n_iterations: int
+    threshold: float
+
+    @classmethod
+    def of(cls, solver: "Davidson", n_iterations: Optional[int],
+           eig_threshold: Optional[float]) -> "_Targets":
+        f = solver.electrons.fillings
+        # A subclass calling in as a helper, or a caller giving neither a
+        # limit nor a threshold, solves to n_bands; an inner SCF loop (an
+        # explicit limit or threshold) stops at n_bands_min.
+        helper = type(solver) is not Davidson
+        inner = not helper and not (n_iterations is None and eig_threshold is None)
+        return cls(n_bands=f.n_bands, n_keep=f.n_bands + f.n_bands_extra,
+                   n_goal=f.n_bands_min if inner else f.n_bands,
+                   n_iterations=n_iterations or solver.n_iterations,
+                   threshold=eig_threshold or solver.eig_threshold)
+
+
+def _bordered(corner: torch.Tensor, edge: torch.Tensor) -> torch.Tensor:
+    """The Hermitian matrix [[corner, E], [E^H, F]] of an expanded subspace
+    [C, C_exp], from corner = the known C-C block (n_cur x n_cur) and
+    edge = [E; F] = [C, C_exp]^X C_exp ((n_cur + n_exp) x n_exp), in complex128
+    (the block layout of TileExpansion, _davidson.py:240-264)."""
+    edge = edge.to(torch.complex128)
+    n_cur = edge.shape[-2] - edge.shape[-1]
+    top, bottom = edge[..., :n_cur, :], edge[..., n_cur:, :]
+    corner = corner.to(torch.complex128).expand(edge.shape[:-2] + (n_cur, n_cur))
+    return torch.cat((torch.cat((corner, top), dim=-1),
+                      torch.cat((dagger(top), bottom), dim=-1)), dim=-2)
+
+
 class Davidson:
     """Davidson diagonalization of the Hamiltonian in `electrons`."""
 
@@ -61,18 +116,20 @@ class Davidson:
                  residual_threshold: Optional[float] = None):
         self.electrons = electrons
         #: When set, bands count as converged once ||H C - eig C|| (accumulated
-        #: in double) is below this, instead of the eigenvalue change of
-        #: _davidson.py:126-139: in float32 the eigenvalue change has a noise
-        #: floor near 1e-6 Ha that an FP64-sized threshold never passes.
+        #: in double) is below this, instead of the eigenvalue change: in
+        #: float32 the eigenvalue change has a rounding floor near 1e-6 Ha
+        #: that an FP64-sized threshold never passes.
         self.residual_threshold = residual_threshold
-        #: Solve the expanded subspace by canonical orthogonalisation instead
-        #: of Cholesky; defaults to on for complex64 wavefunctions, whose
-        #: float32-accumulated overlaps can lose positive definiteness.
+        #: Solve the expanded subspace with a canonical fallback of the
+        #: Cholesky generalized eigensolve; on by default for complex64
+        #: bands, whose float32 overlaps can lose positive definiteness.
         self.canonical_overlap = canonical_overlap
         self.n_iterations = n_iterations
         self.eig_threshold = eig_threshold
         self._line_prefix = "Davidson"
-        self._norm_cut = np.sqrt(electrons.basis.n_tot * 1e-15)  # :56-58
+        #: Expansion vectors with a norm below this are regularised
+        #: (_davidson.py:56-58).
+        self._norm_cut = np.sqrt(electrons.basis.n_tot * 1e-15)
         self._i_iter = 0
         self.history: list = []
         self.timer = _PhaseTimer()
@@ -80,220 +137,188 @@ class Davidson:
     def __repr__(self) -> str:
         return f"Davidson(n_iterations: {self.n_iterations}, eig_threshold: {self.eig_threshold:g})"
 
+    # ------------------------------------------------------------ pieces
     def precondition(self, Cerr: Wavefunction, KEref: torch.Tensor) -> Wavefunction:
-        """Inverse-kinetic preconditioner (:92-99)."""
-        basis = self.electrons.basis
-        x = basis.get_ke()[None, :, None, None, :] / KEref[..., None, None]
-        x += torch.exp(-x)
-        return Cerr * (1.0 / x).to(Cerr.coeff.real.dtype)
+        """Inverse-kinetic preconditioner (_davidson.py:92-99): divide by
+        x + exp(-x), x = KE / KEref, which leaves x ~ 0 untouched."""
+        x = self.electrons.basis.get_ke()[None, :, None, None, :] / KEref[..., None, None]
+        return Cerr * (1.0 / (x + torch.exp(-x))).to(Cerr.coeff.real.dtype)
 
-    def _residual_expansion(
-        self, C_sel: Wavefunction, HC_sel: Wavefunction, eig_sel: torch.Tensor,
-        KEref: torch.Tensor,
-    ) -> tuple[Wavefunction, torch.Tensor]:
-        """precondition(HC_sel - C_sel eig_sel, KEref) and its band norms (:182-184),
-        fused into one pass over C and HC."""
-        c, h = C_sel.coeff, HC_sel.coeff
-        S, K, B, X, G = c.shape
-        out = torch.empty((S, K, B, X, G), dtype=c.dtype, device=c.device)
-        norm2 = torch.empty((S, K, B), dtype=torch.float64, device=c.device)
-        rnorm2 = torch.empty((S, K, B), dtype=torch.float64, device=c.device)
-        ke = self.electrons.basis.get_ke()
-        eig = eig_sel.contiguous()
-        kref = KEref.contiguous()
-        for s in range(S):
-            cs, hs = c[s], h[s]
-            if cs.stride() != hs.stride() or cs.stride(-1) != 1:
-                hs = hs.contiguous()
-                cs = cs.contiguous()
-            _lib.call(
-                "qmx_precond_residual", DTYPE_CODE[c.dtype], cs.data_ptr(), hs.data_ptr(),
-                cs.stride(1), cs.stride(0), eig[s].data_ptr(), kref[s].data_ptr(),
-                ke.data_ptr(), ke.stride(0), out[s].data_ptr(), out.stride(2), out.stride(1),
-                K, B, G, norm2[s].data_ptr(), rnorm2[s].data_ptr(), rc.stream_ptr(),
-            )
-        self.residual_norm = rnorm2.sqrt()
-        return Wavefunction(C_sel.basis, coeff=out), norm2.sqrt()
+    def get_Eband(self) -> float:
+        el = self.electrons
+        n = el.fillings.n_bands
+        return float((el.basis.w_sk * el.eig[..., :n]).sum().item())
+
+    def _check_deig(self, deig: torch.Tensor, eig_threshold: float) -> tuple[float, int]:
+        """(largest change over the reported bands, number of leading bands
+        whose change is below the threshold in every spin and k-point)."""
+        n = self.electrons.fillings.n_bands
+        change = deig[..., :n]
+        unconverged = (change > eig_threshold).flatten(0, 1).any(dim=0)
+        first = torch.nonzero(unconverged)
+        return float(change.max().item()), int(first[0, 0].item()) if len(first) else n
+
+    def _report(self, n_eigs_done: int = 0, converged: bool = False) -> None:
+        self.history.append((self._i_iter, self.electrons.deig_max, n_eigs_done))
 
     def _orthogonalize(self, C: Wavefunction) -> bool:
         ortho = self.canonical_overlap
         return (C.coeff.dtype == torch.complex64) if ortho is None else ortho
 
+    def _rotate_to_ritz(self, C: Wavefunction) -> Wavefunction:
+        """Ritz vectors of span(C) (the subspace eigh of _davidson.py:164-168);
+        sets electrons.C and electrons.eig, returns H C in the new basis."""
+        el = self.electrons
+        HC = el.hamiltonian(C)
+        el.eig, V = eigh((C ^ HC).wait().to(torch.complex128))
+        el.C = C @ V
+        return HC @ V
+
+    def _expansion(self, C: Wavefunction, HC: Wavefunction, eig: torch.Tensor
+                   ) -> tuple[Wavefunction, torch.Tensor]:
+        """precondition(HC - C eig, band_ke(C)) and its band norms, fused into
+        one pass over C and HC (qmx_precond_residual); the unpreconditioned
+        residual norms go to self.residual_norm."""
+        KEref = C.band_ke()
+        c, h = C.coeff, HC.coeff
+        S, K, B, X, G = c.shape
+        out = torch.empty((S, K, B, X, G), dtype=c.dtype, device=c.device)
+        norm2 = torch.empty((S, K, B), dtype=torch.float64, device=c.device)
+        rnorm2 = torch.empty_like(norm2)
+        ke = self.electrons.basis.get_ke()
+        eig, KEref = eig.contiguous(), KEref.contiguous()
+        for s in range(S):
+            cs, hs = c[s], h[s]
+            if cs.stride() != hs.stride() or cs.stride(-1) != 1:
+                cs, hs = cs.contiguous(), hs.contiguous()
+            _lib.call("qmx_precond_residual", DTYPE_CODE[c.dtype], cs.data_ptr(), hs.data_ptr(),
+                      cs.stride(1), cs.stride(0), eig[s].data_ptr(), KEref[s].data_ptr(),
+                      ke.data_ptr(), ke.stride(0), out[s].data_ptr(), out.stride(2),
+                      out.stride(1), K, B, G, norm2[s].data_ptr(), rnorm2[s].data_ptr(),
+                      rc.stream_ptr())
+        self.residual_norm = rnorm2.sqrt()
+        return Wavefunction(C.basis, coeff=out), norm2.sqrt()
+
+    def _regularize(self, C: Wavefunction, norm: torch.Tensor, i_iter: int) -> None:
+        """Re-draw expansion vectors whose norm vanished, seeded by the
+        iteration (_davidson.py:101-115); their norm is taken as 1."""
+        i_spin, i_k, i_band = torch.where(norm < self._norm_cut)
+        if len(i_spin):
+            C.randomize_selected(i_spin, i_k, i_band, i_iter)
+            norm[i_spin, i_k, i_band] = 1.0
+
     def _project_out(self, C: Wavefunction, Cexp: Wavefunction, drop_below: float = 1e-3) -> None:
         """Cexp <- (1 - C C^H) Cexp twice (block classical Gram-Schmidt with
         re-orthogonalisation), then renormalise. Directions that keep less
         than `drop_below` of their norm lie in span(C) up to float32 rounding;
-        they are re-drawn at random (as _regularize does for null bands) and
-        projected again. Used for float32 subspaces: the reference's expansion
-        (_davidson.py:183-197) leaves the near-dependence between C and Cexp
-        to its FP64 generalized eigensolve, which float32 overlaps cannot
-        resolve. In exact arithmetic the span, and so the Ritz values, are
-        unchanged."""
+        they are re-drawn at random (as null bands are regularised) and
+        projected again. Float32 subspaces need this: the reference leaves
+        the near-dependence between C and its expansion to its FP64
+        generalized eigensolve, which float32 overlaps cannot resolve. In
+        exact arithmetic the span, and so the Ritz values, are unchanged."""
         for attempt in range(2):
             for _ in range(2):
-                M = (C ^ Cexp).wait()
-                Cexp -= C @ M
+                Cexp -= C @ (C ^ Cexp).wait()
             norm = Cexp.band_norm()
-            low = norm < drop_below
-            i_spin, i_k, i_band = torch.where(low)
+            i_spin, i_k, i_band = torch.where(norm < drop_below)
             if attempt or not len(i_spin):
                 break
             Cexp.randomize_selected(i_spin, i_k, i_band, 1000 + self._i_iter)
             Cexp.scale_constrain_(1.0 / Cexp.band_norm())
         Cexp.scale_constrain_(1.0 / norm.clamp_min(1e-300))
 
-    def _regularize(self, C: Wavefunction, norm: torch.Tensor, i_iter: int) -> None:
-        """Randomize null bands, seeded by the iteration (:101-115)."""
-        low_norm = norm < self._norm_cut
-        i_spin, i_k, i_band = torch.where(low_norm)
-        if not len(i_spin):
-            return
-        C.randomize_selected(i_spin, i_k, i_band, i_iter)
-        norm[i_spin, i_k, i_band] = 1.0
+    def _residual_done(self, n_locked: int, n_bands: int) -> int:
+        """Leading bands (counted from 0) whose residual norm is below
+        residual_threshold in every spin and k-point, among the expanded
+        bands n_locked.. (the bands before them are converged already)."""
+        r = self.residual_norm[..., : max(n_bands - n_locked, 0)]
+        bad = torch.nonzero((r > self.residual_threshold).flatten(0, 1).any(dim=0))
+        return n_locked + (int(bad[0, 0].item()) if len(bad) else n_bands - n_locked)
 
-    def get_Eband(self) -> float:
+    def _solve_projected(self, Hs: torch.Tensor, Os: torch.Tensor, C: Wavefunction):
+        canonical = self.canonical_overlap
+        if canonical is None:
+            canonical = C.coeff.dtype == torch.complex64
+        return (eighg_robust if canonical else eighg)(Hs, Os)
+
+    # ------------------------------------------------------------ iteration
+    def __call__(self, n_iterations: Optional[int] = None,
+                 eig_threshold: Optional[float] = None) -> None:
         el = self.electrons
-        n_bands = el.fillings.n_bands
-        return float((el.basis.w_sk * el.eig[..., :n_bands]).sum().item())
-
-    def _check_deig(self, deig: torch.Tensor, eig_threshold: float) -> tuple[float, int]:
-        n_bands = self.electrons.fillings.n_bands
-        d = deig[..., :n_bands]
-        deig_max = float(d.max().item())
-        pending = torch.where((d > eig_threshold).flatten(0, 1).any(dim=0))[0]
-        n_eigs_done = int(pending[0].item()) if len(pending) else n_bands
-        return deig_max, n_eigs_done
-
-    def _report(self, n_eigs_done: int = 0, converged: bool = False) -> None:
-        self.history.append((self._i_iter, self.electrons.deig_max, n_eigs_done))
-
-    def __call__(self, n_iterations: Optional[int] = None, eig_threshold: Optional[float] = None) -> None:
-        el = self.electrons
-        n_bands = el.fillings.n_bands
-        n_bands_max = n_bands + el.fillings.n_bands_extra
-        helper = type(self) != Davidson
-        inner_loop = not (helper or ((n_iterations is None) and (eig_threshold is None)))
-        n_iterations = n_iterations if n_iterations else self.n_iterations
-        eig_threshold = eig_threshold if eig_threshold else self.eig_threshold
-        n_eigs_target = el.fillings.n_bands_min if inner_loop else n_bands
-
-        if 2 * n_bands_max >= el.basis.n_min:
+        t = _Targets.of(self, n_iterations, eig_threshold)
+        if 2 * t.n_keep >= el.basis.n_min:  # _davidson.py:159-163
             raise ValueError(
-                f"n_bands + n_bands_extra = {n_bands_max} exceeds"
-                f" min(n_basis)/2 = {el.basis.n_min // 2} in Davidson"
+                f"Davidson needs twice n_bands + n_bands_extra ({t.n_keep}) below the smallest"
+                f" basis ({el.basis.n_min} plane waves): at most {(el.basis.n_min - 1) // 2} bands"
             )
         self.history = []
-        HC = el.hamiltonian(el.C)
-        el.eig, V = eigh((el.C ^ HC).wait().to(torch.complex128))
+        HC = self._rotate_to_ritz(el.C)
         el.deig_max = np.inf
-        el.C = el.C @ V
-        HC = HC @ V
         self._i_iter = 0
         self._report()
-        n_eigs_done = 0
-
-        for self._i_iter in range(self._i_iter + 1, n_iterations + 1):
-            n_bands_cur = el.C.n_bands()
-            eig_sel = el.eig[:, :, n_eigs_done:]
-            C_sel = el.C[:, :, n_eigs_done:] if n_eigs_done else el.C
-            HC_sel = HC[:, :, n_eigs_done:] if n_eigs_done else HC
+        n_locked = 0
+        for self._i_iter in range(1, t.n_iterations + 1):
+            n_cur = el.C.n_bands()
+            # 1. Expansion from the residuals of the bands not yet converged.
             tk = self.timer.mark("residual")
-            KEref = C_sel.band_ke()
-            Cexp, norm_exp = self._residual_expansion(C_sel, HC_sel, eig_sel, KEref)
+            active = slice(n_locked, None)
+            C_act = el.C[:, :, active] if n_locked else el.C
+            HC_act = HC[:, :, active] if n_locked else HC
+            Cexp, norm = self._expansion(C_act, HC_act, el.eig[:, :, active])
             self.timer.done(tk)
-            res_thr = self.residual_threshold
-            if res_thr is not None:
-                # residuals of the bands not yet locked (index >= n_eigs_done):
-                r = self.residual_norm[..., : max(n_bands - n_eigs_done, 0)]
-                pending = torch.where((r > res_thr).flatten(0, 1).any(dim=0))[0]
-                n_res_done = n_eigs_done + (
-                    int(pending[0].item()) if len(pending) else n_bands - n_eigs_done)
-                if n_res_done >= n_eigs_target:
-                    self._report(n_res_done, True)
+            if self.residual_threshold is not None:
+                n_res = self._residual_done(n_locked, t.n_bands)
+                if n_res >= t.n_goal:
+                    self._report(n_res, True)
                     break
-            self._regularize(Cexp, norm_exp, self._i_iter)
-            Cexp.scale_constrain_(1.0 / norm_exp)
+            self._regularize(Cexp, norm, self._i_iter)
+            Cexp.scale_constrain_(1.0 / norm)
             if self._orthogonalize(el.C):
                 tk = self.timer.mark("project")
                 self._project_out(el.C, Cexp)
                 self.timer.done(tk)
-            n_bands_exp = Cexp.n_bands()
-            n_bands_new = n_bands_cur + n_bands_exp
-
+            n_exp = Cexp.n_bands()
+            # 2. The expanded subspace and its projected overlap and H.
             Cnew = el.C.cat(Cexp, clear=True)
-            Cexp = Cnew[:, :, n_bands_cur:]
-
+            Cexp = Cnew[:, :, n_cur:]
             tk = self.timer.mark("overlap")
-            C_OC = torch.eye(n_bands_cur, device=rc.device, dtype=torch.float64)[None, None]
-            C_OC_new = TileExpansion(C_OC, Cnew.dot_O(Cexp))
+            O_blk = _bordered(torch.eye(n_cur, device=rc.device, dtype=torch.float64),
+                              Cnew.dot_O(Cexp).wait())
             self.timer.done(tk)
-
             tk = self.timer.mark("hamiltonian")
             HCexp = el.hamiltonian(Cexp)
             self.timer.done(tk)
             del Cexp
             tk = self.timer.mark("overlap")
-            C_HC = torch.diag_embed(el.eig)
-            C_HC_new = TileExpansion(C_HC, Cnew ^ HCexp)
+            H_blk = _bordered(torch.diag_embed(el.eig), (Cnew ^ HCexp).wait())
             self.timer.done(tk)
+            # 3. Ritz vectors of the expanded subspace, truncated.
             tk = self.timer.mark("eighg")
-
-            canonical = self.canonical_overlap
-            if canonical is None:
-                canonical = Cnew.coeff.dtype == torch.complex64
-            if canonical:
-                eig_new, V_new = eighg_robust(C_HC_new, C_OC_new.wait())
-            else:
-                eig_new, V_new = eighg(C_HC_new, C_OC_new.wait())
+            eig_new, V = self._solve_projected(H_blk, O_blk, Cnew)
             self.timer.done(tk)
-            n_bands_next = min(n_bands_new, n_bands_max)
-            V_new = V_new[..., :n_bands_next]
-            Vcur, Vexp = V_new.split((n_bands_cur, n_bands_exp), dim=2)
-
+            n_next = min(n_cur + n_exp, t.n_keep)
+            V = V[..., :n_next]
             tk = self.timer.mark("rotate")
-            el.C = Cnew @ V_new
+            el.C = Cnew @ V
             del Cnew
-            HC = HC @ Vcur
-            HC += HCexp @ Vexp
+            HC = HC @ V[:, :, :n_cur]
+            HC += HCexp @ V[:, :, n_cur:]
             del HCexp
             self.timer.done(tk)
-            deig = torch.abs(el.eig - eig_new[..., :n_bands_cur])
-            el.eig = eig_new[..., :n_bands_next]
-
+            # 4. Convergence by the eigenvalue change of the current bands.
+            deig = (el.eig - eig_new[..., :n_cur]).abs()
+            el.eig = eig_new[..., :n_next]
             tk = self.timer.mark("check")
-            el.deig_max, n_eigs_done = self._check_deig(deig, eig_threshold)
+            el.deig_max, n_locked = self._check_deig(deig, t.threshold)
             self.timer.done(tk)
-            if res_thr is not None:
-                n_eigs_done = n_res_done  # lock by residual; test at the next start
-            converged = res_thr is None and n_eigs_done >= n_eigs_target
-            self._report(n_eigs_done, converged)
-            if converged:
+            if self.residual_threshold is not None:
+                n_locked = n_res  # lock by residual; tested again at the next start
+            done = self.residual_threshold is None and n_locked >= t.n_goal
+            self._report(n_locked, done)
+            if done:
                 break
         self.last_HC = HC  # H C of the final subspace, for residual reports
-        if helper:
-            self._HC = HC
+        if type(self) is not Davidson:
+            self._HC = HC  # for the subclass that called in as a helper
 
     diagonalize = __call__
-
-
-@dataclass
-class TileExpansion:
-    """[[C_XC, C_XCexp], [C_XCexp^H, Cexp_XCexp]] from Cnew^X(Cexp) (:240-264),
-    assembled in complex128 for the FP64 subspace solve."""
-
-    C_XC: torch.Tensor
-    Cnew_XCexp: object
-
-    def wait(self) -> torch.Tensor:
-        Cnew_XCexp = self.Cnew_XCexp.wait().to(torch.complex128)
-        n_spins, nk, n_bands_new, n_bands_exp = Cnew_XCexp.shape
-        n_bands_cur = n_bands_new - n_bands_exp
-        C_XCexp, Cexp_XCexp = Cnew_XCexp.split((n_bands_cur, n_bands_exp), dim=2)
-        result = torch.zeros(
-            (n_spins, nk, n_bands_new, n_bands_new), device=C_XCexp.device, dtype=torch.complex128
-        )
-        result[:, :, :n_bands_cur, :n_bands_cur] += self.C_XC
-        result[:, :, :n_bands_cur, n_bands_cur:] = C_XCexp
-        result[:, :, n_bands_cur:, :n_bands_cur] = dagger(C_XCexp)
-        result[:, :, n_bands_cur:, n_bands_cur:] = Cexp_XCexp
-        return result

@@ -298,13 +298,18 @@ def test_band_reductions_and_precond(cfg0_el):
     assert rel(ke, np.einsum("kbg,kg->kb", np.abs(C) ** 2, ob.KE)) < 1e-14
     eig = np.random.default_rng(7).standard_normal((len(ob.k), 5))
     d = el.diagonalize
-    Cexp, nrm = d._residual_expansion(
-        W, to_wf(el.basis, HC, C128), torch.tensor(eig, device="cuda")[None], torch.tensor(ke, device="cuda")[None])
-    x = ob.KE[:, None, :] / ke[..., None]
-    x = x + np.exp(-x)
-    ref = (HC - C * eig[..., None]) / x
+    # The fused residual + preconditioner pass (KEref = band_ke of C) and the
+    # Python preconditioner against the oracle's restatement, whose
+    # conventions the reference's own vectors pin (test_golden_reference).
+    from oracle.davidson import precondition as o_precondition
+    Cexp, nrm = d._expansion(W, to_wf(el.basis, HC, C128), torch.tensor(eig, device="cuda")[None])
+    ref = o_precondition(ob, HC - C * eig[..., None], ke)
     assert rel(Cexp.coeff[0, :, :, 0], ref) < 1e-14
     assert rel(nrm[0], np.sqrt((np.abs(ref) ** 2).sum(-1))) < 1e-14
+    rn = d.residual_norm[0].cpu().numpy()
+    assert rel(rn, np.sqrt((np.abs(HC - C * eig[..., None]) ** 2).sum(-1))) < 1e-14
+    P = d.precondition(to_wf(el.basis, HC, C128), torch.tensor(ke, device="cuda")[None])
+    assert rel(P.coeff[0, :, :, 0], o_precondition(ob, HC, ke)) < 1e-14
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
