@@ -260,20 +260,6 @@ int qmx_randomize(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk,
                   const double* ke, int64_t ke_ldk, int64_t seed, int64_t b_start,
                   int64_t b_stop, void* stream);
 
-/* Householder reduction of one Hermitian complex128 matrix to real
- * tridiagonal form, T = Q^H A Q, in one cooperative (grid-synchronised)
- * launch: the latency-bound core of the subspace eigensolve eigh(U^H H U)
- * in eighg (math/_linalg.py:181) and the initial eigh of Davidson
- * (_davidson.py:165). A is row-major with leading dimension lda and only its
- * lower triangle is read (UPLO = L). Outputs: d[n] and e[n-1] (real), and Q
- * (n x n) in column-major order, as a tridiagonal divide-and-conquer with
- * "original" eigenvectors expects. n_groups / n_threads (256, 512, 1024)
- * set the launch shape; 0 picks the defaults. */
-int64_t qmx_hetrd_workspace_bytes(int64_t n);
-int qmx_hetrd(const void* A, int64_t n, int64_t lda, double* d, double* e, void* Q,
-              void* workspace, int64_t workspace_bytes, int n_groups, int n_threads,
-              void* stream);
-
 /* Batched Hermitian eigensolver for small complex128 matrices (n <= 96) by
  * cyclic Jacobi, one workgroup per matrix: the subspace eigensolves
  * eigh(U^H H U) of eighg (math/_linalg.py:181) and Davidson's initial eigh
@@ -338,6 +324,37 @@ int qmx_heev_small_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_m
  * (E and V are then NaN). */
 int64_t qmx_heevx_work_bytes(int64_t n, int64_t n_want);
 int qmx_heevx_grouped(const qmx_heev_desc* desc, int64_t count, int64_t n_max, void* stream);
+
+/* One larger Hermitian complex128 matrix (2 <= n <= 2048): the subspace
+ * eigensolve eigh(U^H H U) inside eighg (math/_linalg.py:144-183) for the
+ * expanded Davidson subspace (_davidson.py:199-206) and the CheFSI
+ * Rayleigh-Ritz (_chefsi.py:156) above n = 256. A (row stride lda,
+ * lower triangle read, UPLO = L) is reduced to real tridiagonal form by one
+ * cooperative launch whose workgroups own rows of A and exchange one column
+ * per step; then all n eigenvalues by multisection, the eigenvectors of the
+ * n_want lowest (0 = all) by inverse iteration with re-orthogonalisation in
+ * clusters, and the back-transformation. E[n] ascending; eigenvector j goes
+ * to V[i * vsi + j * vsj]; *info (device, may be NULL) is 0, or 1 for a
+ * non-finite input (E and V NaN). Work area: qmx_heevx_large_work_bytes(n,
+ * n_want) bytes, 256-byte aligned. n_groups > 0 sets the reduction's
+ * workgroups (0 = chosen from n). */
+int64_t qmx_heevx_large_work_bytes(int64_t n, int64_t n_want);
+int qmx_heevx_large(const void* A, int64_t n, int64_t lda, int64_t n_want, double* E, void* V,
+                    int64_t vsi, int64_t vsj, int* info, void* work, int64_t work_bytes,
+                    int n_groups, void* stream);
+
+/* Inverse Cholesky factor of one Hermitian positive definite complex128
+ * matrix (1 <= n <= 2048): X = L^-1 with O = L L^H, the orthonormalising
+ * U = X^H of ortho_matrix (math/_linalg.py:110-141: torch.linalg.cholesky
+ * then solve_triangular against the identity) used by eighg and Davidson's
+ * orthonormalisation. O (row stride ldo, lower triangle read); X row-major
+ * n x n, lower triangular (zeros above). *info (device, may be NULL) is 0,
+ * or k + 1 for the first non-positive or non-finite pivot k, as LAPACK potrf
+ * and torch.linalg.cholesky_ex. One cooperative launch; work area of
+ * qmx_potrf_inv_work_bytes(n) bytes, 256-byte aligned. */
+int64_t qmx_potrf_inv_work_bytes(int64_t n);
+int qmx_potrf_inv(const void* O, int64_t n, int64_t ldo, void* X, int* info, void* work,
+                  int64_t work_bytes, int n_groups, void* stream);
 
 /* Fused steps of the batched Davidson iteration (no counterpart as kernels
  * in the reference, which runs them as tensor expressions):

@@ -67,8 +67,6 @@ SIGNATURES = {
         i32,
         [i32, vp, i64, i64, i64, i64, i64, i64, vp, vp, i64, i64, i64, i64, vp],
     ),
-    "qmx_hetrd_workspace_bytes": (i64, [i64]),
-    "qmx_hetrd": (i32, [vp, i64, i64, vp, vp, vp, vp, i64, i32, i32, vp]),
     "qmx_heevj_workspace_bytes": (i64, [i64, i64]),
     "qmx_heevj_batched": (i32, [vp, i64, i64, i64, i64, vp, vp, i64, i64, vp, i64, i32, vp]),
     "qmx_heev_small_workspace_bytes": (i64, [i64, i64]),
@@ -89,6 +87,10 @@ SIGNATURES = {
     "qmx_heev_small_grouped": (i32, [vp, i64, i64, vp]),
     "qmx_heevx_work_bytes": (i64, [i64, i64]),
     "qmx_heevx_grouped": (i32, [vp, i64, i64, vp]),
+    "qmx_heevx_large_work_bytes": (i64, [i64, i64]),
+    "qmx_heevx_large": (i32, [vp, i64, i64, i64, vp, vp, i64, i64, vp, vp, i64, i32, vp]),
+    "qmx_potrf_inv_work_bytes": (i64, [i64]),
+    "qmx_potrf_inv": (i32, [vp, i64, i64, vp, vp, vp, i64, i32, vp]),
     "qmx_normalize_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
     "qmx_canonical_basis": (i32, [i32, vp, vp, i64, i64, ct.c_double, vp, vp]),
     "qmx_subspace_assemble": (i32, [vp, vp, vp, vp, i64, i64, ct.c_double, ct.c_double, vp, vp,

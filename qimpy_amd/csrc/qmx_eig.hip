@@ -1,30 +1,8 @@
-// Latency-optimised Householder tridiagonalisation of one Hermitian matrix
-// (FP64 complex) for the subspace eigensolves of Davidson and CheFSI.
-//
-// Library reductions issue several launches per column (a Hermitian
-// matrix-vector product, reductions, a rank-2 update), so an n = 564 reduction
-// costs ~n x 20 us of launch latency. Here one cooperative launch keeps every
-// workgroup resident: rows are distributed cyclically (row i belongs to
-// workgroup i mod G) and each column step is
-//   (a) every workgroup reads column k below the diagonal (published by the
-//       row owners at the end of the previous step) and builds the Householder
-//       vector v, tau and beta redundantly (no barrier);
-//   (b) owners compute y_i = sum_j A_ij v_j for their rows and the partial
-//       v^H y, publish y and add the partial sum;       -> grid barrier
-//   (c) everyone forms w = tau y - |tau|^2 (v^H y) v / 2, owners apply
-//       A -= v w^H + w v^H and Q -= tau (Q v) v^H to their rows and publish
-//       column k+1;                                      -> grid barrier
-// so each column costs two grid barriers. Each wave of a workgroup takes whole
-// rows (wave-level reductions, no block barriers). Rows of A and Q live in
-// global memory private to their owner (L2-resident), shared vectors in LDS.
-// The outputs are the real tridiagonal (d, e) and Q in column-major order,
-// ready for a tridiagonal divide-and-conquer with evect = original, which
-// returns the eigenvectors Q Z of the input.
-#include <hip/hip_cooperative_groups.h>
-
+// Batched Hermitian eigensolvers for small FP64 complex matrices: cyclic
+// Jacobi (n <= 96) and Householder + implicit-shift QL (n <= 256), one
+// workgroup per matrix, for the subspace eigensolves of Davidson and CheFSI.
 #include "qmx_common.h"
 
-namespace cg = cooperative_groups;
 using namespace qmx;
 
 namespace {
@@ -47,229 +25,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
-// Block reduction of one double; all threads get the result.
-template <int kWaves>
-__device__ __forceinline__ double block_sum_all(double v, double* red) {
-  v = wave_sum_d(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  double s = 0;
-#pragma unroll
-  for (int i = 0; i < kWaves; ++i) s += red[i];
-  return s;
-}
-
-template <int kEigThreads>
-__global__ void __launch_bounds__(kEigThreads)
-    k_hetrd_coop(const double2* __restrict__ A, int n, int64_t lda, double2* __restrict__ W,
-                 double2* __restrict__ Q, double2* __restrict__ Qout, double* __restrict__ d,
-                 double* __restrict__ e, double2* colbuf, double2* ybuf, double* dotacc) {
-  constexpr int kWaves = kEigThreads / 64;
-  cg::grid_group grid = cg::this_grid();
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* v = reinterpret_cast<double2*>(smem_raw);
-  double2* w = v + n;
-  double* red = reinterpret_cast<double*>(w + n);
-  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int64_t ldn = n;
-
-  // Own rows from the lower triangle (as LAPACK/torch eigh with UPLO = L):
-  for (int i = g; i < n; i += G) {
-    for (int j = tid; j < n; j += kEigThreads) {
-      double2 a;
-      if (j < i) {
-        a = A[(int64_t)i * lda + j];
-      } else if (j > i) {
-        const double2 t = A[(int64_t)j * lda + i];
-        a = double2{t.x, -t.y};
-      } else {
-        a = double2{A[(int64_t)i * lda + i].x, 0.0};
-      }
-      W[i * ldn + j] = a;
-      Q[i * ldn + j] = double2{i == j ? 1.0 : 0.0, 0.0};
-    }
-    if (tid == 0 && i >= 1) colbuf[i] = W[i * ldn + 0];
-  }
-  for (int i = g * kEigThreads + tid; i < n; i += G * kEigThreads) dotacc[i] = 0.0;
-  grid.sync();
-
-  for (int k = 0; k < n - 1; ++k) {
-    // (a) Householder vector of x = A[k+1:, k], replicated in every workgroup.
-    double s = 0;
-    for (int i = k + 2 + tid; i < n; i += kEigThreads) {
-      const double2 x = colbuf[i];
-      s += x.x * x.x + x.y * x.y;
-    }
-    const double sigma = block_sum_all<kWaves>(s, red);
-    const double2 alpha = colbuf[k + 1];
-    double beta, scale_re = 0, scale_im = 0;
-    double2 tau;
-    if (sigma == 0.0 && alpha.y == 0.0) {
-      tau = double2{0.0, 0.0};
-      beta = alpha.x;
-    } else {
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + sigma);
-      beta = alpha.x >= 0 ? -nrm : nrm;
-      tau = double2{(beta - alpha.x) / beta, -alpha.y / beta};
-      // scale = 1 / (alpha - beta)
-      const double ar = alpha.x - beta, ai = alpha.y, den = ar * ar + ai * ai;
-      scale_re = ar / den;
-      scale_im = -ai / den;
-    }
-    for (int j = tid; j < n; j += kEigThreads) {
-      double2 vj = double2{0.0, 0.0};
-      if (j == k + 1) {
-        vj = double2{1.0, 0.0};
-      } else if (j > k + 1) {
-        const double2 x = colbuf[j];
-        vj = double2{x.x * scale_re - x.y * scale_im, x.x * scale_im + x.y * scale_re};
-      }
-      v[j] = vj;
-    }
-    __syncthreads();
-    if (g == 0 && tid == 0) e[k] = beta;
-
-    // (b) y_i = sum_j W_ij v_j over own rows i > k, and the partial v^H y.
-    double part = 0;
-    {
-      int r = 0;
-      for (int i = g; i < n; i += G, ++r) {
-        if (i <= k || (r % kWaves) != wave) continue;
-        const double2* Wi = W + i * ldn;
-        double yr = 0, yi = 0;
-        for (int j = k + 1 + lane; j < n; j += 64) {
-          const double2 a = Wi[j], b = v[j];
-          yr += a.x * b.x - a.y * b.y;
-          yi += a.x * b.y + a.y * b.x;
-        }
-        yr = wave_sum_d(yr);
-        yi = wave_sum_d(yi);
-        if (lane == 0) {
-          ybuf[i] = double2{yr, yi};
-          const double2 vi = v[i];
-          part += vi.x * yr + vi.y * yi;  // Re(conj(v_i) y_i)
-        }
-      }
-    }
-    if (lane == 0 && part != 0.0) atomicAdd(&dotacc[k], part);
-    grid.sync();
-
-    // (c) w = tau y - |tau|^2 (v^H y) v / 2 ; rank-2 update of own rows and Q.
-    const double vy = dotacc[k];
-    const double t2 = tau.x * tau.x + tau.y * tau.y;
-    for (int j = k + 1 + tid; j < n; j += kEigThreads) {
-      const double2 y = ybuf[j], vj = v[j];
-      w[j] = double2{tau.x * y.x - tau.y * y.y - 0.5 * t2 * vy * vj.x,
-                     tau.x * y.y + tau.y * y.x - 0.5 * t2 * vy * vj.y};
-    }
-    __syncthreads();
-    {
-      int r = 0;
-      for (int i = g; i < n; i += G, ++r) {
-        if ((r % kWaves) != wave) continue;
-        if (i > k) {
-          double2* Wi = W + i * ldn;
-          const double2 vi = v[i], wi = w[i];
-          for (int j = k + 1 + lane; j < n; j += 64) {
-            const double2 vj = v[j], wj = w[j];
-            // W_ij -= v_i conj(w_j) + w_i conj(v_j)
-            double2 a = Wi[j];
-            a.x -= vi.x * wj.x + vi.y * wj.y + wi.x * vj.x + wi.y * vj.y;
-            a.y -= vi.y * wj.x - vi.x * wj.y + wi.y * vj.x - wi.x * vj.y;
-            Wi[j] = a;
-          }
-        }
-        // Q_i <- Q_i (I - tau v v^H): s = Q_i . v, Q_ij -= tau s conj(v_j)
-        double2* Qi = Q + i * ldn;
-        double sr = 0, si = 0;
-        for (int j = k + 1 + lane; j < n; j += 64) {
-          const double2 q = Qi[j], b = v[j];
-          sr += q.x * b.x - q.y * b.y;
-          si += q.x * b.y + q.y * b.x;
-        }
-        sr = wave_sum_d(sr);
-        si = wave_sum_d(si);
-        const double ts_r = tau.x * sr - tau.y * si, ts_i = tau.x * si + tau.y * sr;
-        for (int j = k + 1 + lane; j < n; j += 64) {
-          const double2 b = v[j];
-          double2 q = Qi[j];
-          q.x -= ts_r * b.x + ts_i * b.y;
-          q.y -= ts_i * b.x - ts_r * b.y;
-          Qi[j] = q;
-        }
-        if (lane == 0) {
-          if (i == k) d[k] = W[i * ldn + i].x;
-          if (i >= k + 2) colbuf[i] = W[i * ldn + (k + 1)];
-        }
-      }
-    }
-    grid.sync();
-  }
-  // Last diagonal entry and Q in column-major order.
-  for (int i = g; i < n; i += G) {
-    if (tid == 0 && i == n - 1) d[n - 1] = W[i * ldn + i].x;
-    for (int j = tid; j < n; j += kEigThreads) Qout[(int64_t)j * n + i] = Q[i * ldn + j];
-  }
-}
-
 }  // namespace
-
-extern "C" {
-
-int64_t qmx_hetrd_workspace_bytes(int64_t n) {
-  if (n < 1) return -1;
-  return (2 * n * n + 2 * n) * (int64_t)sizeof(double2) + n * (int64_t)sizeof(double) + 256;
-}
-
-}  // extern "C"
-
-template <int T>
-static int launch_hetrd(const void* A, int64_t n, int64_t lda, double* d, double* e, void* Q,
-                        void* workspace, int n_groups, hipStream_t st) {
-  int dev = 0, n_cu = 0, per_cu = 0;
-  QMX_HIP(hipGetDevice(&dev));
-  QMX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-  const size_t sh = (size_t)(2 * n) * sizeof(double2) + (T / 64) * sizeof(double);
-  QMX_CHECK_ARG(sh <= 64 * 1024, "qmx_hetrd: n too large for the shared vectors");
-  QMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hetrd_coop<T>, T, sh));
-  QMX_CHECK_ARG(per_cu >= 1, "qmx_hetrd: kernel cannot be resident");
-  int G = n_groups > 0 ? n_groups : 16;
-  G = (int)std::min<int64_t>(std::min<int64_t>(G, (int64_t)n_cu * per_cu), n);
-  char* ws = static_cast<char*>(workspace);
-  double2* W = reinterpret_cast<double2*>(ws);
-  double2* Qw = W + n * n;
-  double2* colbuf = Qw + n * n;
-  double2* ybuf = colbuf + n;
-  double* dotacc = reinterpret_cast<double*>(ybuf + n);
-  const double2* Ap = static_cast<const double2*>(A);
-  double2* Qo = static_cast<double2*>(Q);
-  int ni = (int)n;
-  void* args[] = {(void*)&Ap, (void*)&ni, (void*)&lda, (void*)&W, (void*)&Qw, (void*)&Qo,
-                  (void*)&d, (void*)&e, (void*)&colbuf, (void*)&ybuf, (void*)&dotacc};
-  QMX_HIP(hipLaunchCooperativeKernel((const void*)k_hetrd_coop<T>, dim3(G), dim3(T), args,
-                                     (unsigned)sh, st));
-  return QMX_OK;
-}
-
-extern "C" int qmx_hetrd(const void* A, int64_t n, int64_t lda, double* d, double* e, void* Q,
-                         void* workspace, int64_t workspace_bytes, int n_groups, int n_threads,
-                         void* stream) {
-  QMX_CHECK_ARG(A && d && e && Q && workspace, "qmx_hetrd: null argument");
-  QMX_CHECK_ARG(n >= 2 && n <= 16384 && lda >= n, "qmx_hetrd: need 2 <= n <= 16384, lda >= n");
-  QMX_CHECK_ARG(workspace_bytes >= qmx_hetrd_workspace_bytes(n), "qmx_hetrd: workspace too small");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  switch (n_threads) {
-    case 256: return launch_hetrd<256>(A, n, lda, d, e, Q, workspace, n_groups, st);
-    case 512: return launch_hetrd<512>(A, n, lda, d, e, Q, workspace, n_groups, st);
-    case 0:
-    case 1024: return launch_hetrd<1024>(A, n, lda, d, e, Q, workspace, n_groups, st);
-  }
-  qmx::set_error("qmx_hetrd: n_threads must be 256, 512 or 1024 (0 = default)");
-  return QMX_ERR_ARG;
-}
 
 // ---------------------------------------------------------------------------
 // Batched cyclic Jacobi for small Hermitian matrices (n <= 96): one workgroup

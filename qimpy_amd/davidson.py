@@ -235,11 +235,14 @@ class Davidson:
         bad = torch.nonzero((r > self.residual_threshold).flatten(0, 1).any(dim=0))
         return n_locked + (int(bad[0, 0].item()) if len(bad) else n_bands - n_locked)
 
-    def _solve_projected(self, Hs: torch.Tensor, Os: torch.Tensor, C: Wavefunction):
+    def _solve_projected(self, Hs: torch.Tensor, Os: torch.Tensor, C: Wavefunction,
+                         n_want: Optional[int] = None):
         canonical = self.canonical_overlap
         if canonical is None:
             canonical = C.coeff.dtype == torch.complex64
-        return (eighg_robust if canonical else eighg)(Hs, Os)
+        if canonical:
+            return eighg_robust(Hs, Os, n_want=n_want)
+        return eighg(Hs, Os, n_want=n_want)
 
     # ------------------------------------------------------------ iteration
     def __call__(self, n_iterations: Optional[int] = None,
@@ -294,9 +297,9 @@ class Davidson:
             self.timer.done(tk)
             # 3. Ritz vectors of the expanded subspace, truncated.
             tk = self.timer.mark("eighg")
-            eig_new, V = self._solve_projected(H_blk, O_blk, Cnew)
-            self.timer.done(tk)
             n_next = min(n_cur + n_exp, t.n_keep)
+            eig_new, V = self._solve_projected(H_blk, O_blk, Cnew, n_next)
+            self.timer.done(tk)
             V = V[..., :n_next]
             tk = self.timer.mark("rotate")
             el.C = Cnew @ V

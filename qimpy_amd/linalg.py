@@ -7,7 +7,6 @@ these small dense problems always run in complex128 on the device.
 """
 from __future__ import annotations
 
-import ctypes as ct
 from typing import Generic, Optional, Protocol, Sequence, TypeVar
 
 import numpy as np
@@ -36,41 +35,19 @@ class Waitless(Generic[T]):
 #: (qmx_heev_small_batched) kernels.
 JACOBI_MAX_N = 96
 SMALL_QL_MAX_N = 256
-#: Size window of the cooperative tridiagonalisation (qmx_hetrd): below it the
-#: library eigensolver is as fast, above it grid barriers make it slower.
-HETRD_MIN_N, HETRD_MAX_N = 200, 800
-#: Launch shape of qmx_hetrd (workgroups, threads) measured fastest on MI355X.
-HETRD_GROUPS, HETRD_THREADS = 32, 1024
+#: Largest n of the grid-wide eigensolver (qmx_heevx_large), used above
+#: SMALL_QL_MAX_N; larger matrices go to the library eigensolver.
+LARGE_MAX_N = 2048
+#: Workgroups of its reduction (0: the library's choice from n).
+LARGE_GROUPS = 0
+#: Largest n of the grid-wide Cholesky factor inverse (qmx_potrf_inv), and
+#: its workgroups (0: the library's choice from n).
+POTRF_MAX_N = 2048
+POTRF_GROUPS = 0
 
 
-class _RocSolver:
-    """rocSOLVER's tridiagonal divide-and-conquer (zstedc), called on torch's
-    current stream through a rocBLAS handle of this process."""
-
-    EVECT_ORIGINAL = 211
-
-    def __init__(self):
-        self.rb = ct.CDLL("librocblas.so")
-        self.rs = ct.CDLL("librocsolver.so")
-        self.handle = ct.c_void_p()
-        if self.rb.rocblas_create_handle(ct.byref(self.handle)) != 0:
-            raise RuntimeError("rocblas_create_handle failed")
-        self.rs.rocsolver_zstedc.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p,
-                                             ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
-        self.rb.rocblas_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
-
-    def zstedc(self, d: torch.Tensor, e: torch.Tensor, Q: torch.Tensor, info: torch.Tensor):
-        n = d.numel()
-        if self.rb.rocblas_set_stream(self.handle, ct.c_void_p(rc.stream_ptr())) != 0:
-            raise RuntimeError("rocblas_set_stream failed")
-        st = self.rs.rocsolver_zstedc(self.handle, self.EVECT_ORIGINAL, n, d.data_ptr(),
-                                      e.data_ptr(), Q.data_ptr(), n, info.data_ptr())
-        if st != 0:
-            raise RuntimeError(f"rocsolver_zstedc failed with status {st}")
-
-
-_rocsolver = None
-_hetrd_ws: dict = {}
+_large_ws: dict = {}
+_potrf_ws: dict = {}
 
 
 #: Batched on-chip eigensolvers are latency-bound per launch but run all
@@ -200,8 +177,8 @@ def eigh_grouped(mats: list, n_want: Optional[Sequence[int]] = None,
 def eigh_nosync(H: torch.Tensor):
     """(E, V, info) like eigh for complex128 (..., n, n) on the GPU, without
     a host synchronisation: info is None or an int32 tensor per matrix,
-    nonzero where the QL iteration did not converge, for the caller to check
-    at its next synchronisation point."""
+    nonzero where the QL iteration did not converge or the input was not
+    finite, for the caller to check at its next synchronisation point."""
     n = H.shape[-1]
     n_mat = 1
     for b in H.shape[:-2]:
@@ -211,19 +188,48 @@ def eigh_nosync(H: torch.Tensor):
         if method != "library":
             E, V = eigh_small(H, method)
             return E, V, (eigh_small.last_info if method in ("ql", "bisect") else None)
+    if H.is_cuda and SMALL_QL_MAX_N < n <= LARGE_MAX_N:
+        return eigh_large(H)
     E, V = eigh(H)
     return E, V, None
 
 
-def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+def eigh_large(H: torch.Tensor, n_want: Optional[int] = None):
+    """(E, V, info) of complex128 (..., n, n), 2 <= n <= LARGE_MAX_N, on the
+    GPU by qmx_heevx_large, one matrix per call: all n eigenvalues
+    (ascending) and the eigenvectors of the n_want lowest (all by default)
+    in the columns of V, a column-major (n, n_want) view per matrix; info
+    (int32 per matrix) is 1 for a non-finite input (E and V NaN). Reads the
+    lower triangle."""
+    n = H.shape[-1]
+    nw = n if n_want is None else max(1, min(int(n_want), n))
+    batch = H.shape[:-2]
+    Hf = _fp64(H).reshape((-1, n, n))
+    if Hf.stride(-1) != 1 or Hf.stride(-2) != n:
+        Hf = Hf.contiguous()
+    need = int(_lib.lib().qmx_heevx_large_work_bytes(n, nw))
+    ws = _large_ws.get(H.device)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=H.device)
+        _large_ws[H.device] = ws
+    B = Hf.shape[0]
+    E = torch.empty((B, n), dtype=torch.float64, device=H.device)
+    Vt = torch.empty((B, nw, n), dtype=torch.complex128, device=H.device)
+    info = torch.empty(B, dtype=torch.int32, device=H.device)
+    for b in range(B):
+        _lib.call("qmx_heevx_large", Hf[b].data_ptr(), n, n, nw, E[b].data_ptr(), Vt[b].data_ptr(),
+                  1, n, info[b:].data_ptr(), ws.data_ptr(), need, LARGE_GROUPS, rc.stream_ptr())
+    return E.reshape(batch + (n,)), Vt.transpose(-1, -2).reshape(batch + (n, nw)), info
+
+
+def eigh(H: torch.Tensor, n_want: Optional[int] = None) -> tuple[torch.Tensor, torch.Tensor]:
     """Hermitian eigendecomposition in FP64 (lower triangle, ascending), like
     torch.linalg.eigh. On the GPU: n <= SMALL_QL_MAX_N runs the whole batch
-    in one batched Jacobi or Householder + QL launch where the cost model
-    says it beats the library; HETRD_MIN_N <= n <= HETRD_MAX_N reduces each
-    matrix to tridiagonal form in one cooperative launch (qmx_hetrd) and
-    finishes with rocSOLVER's tridiagonal divide-and-conquer; other sizes use
-    the library eigensolver."""
-    global _rocsolver
+    in one batched launch (bisection, Householder + QL or Jacobi) where the
+    cost model says it beats the library; SMALL_QL_MAX_N < n <= LARGE_MAX_N
+    runs the grid-wide solver qmx_heevx_large, which also takes n_want, the
+    number of lowest eigenvectors wanted (V then has n_want columns; smaller
+    n return all n); larger n use the library eigensolver."""
     H = _fp64(H)
     n = H.shape[-1]
     if not H.is_cuda or n == 0:
@@ -239,30 +245,11 @@ def eigh(H: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
             if method in ("ql", "bisect") and bool((eigh_small.last_info != 0).any()):
                 return torch.linalg.eigh(H)  # QL did not converge / non-finite input
             return E, V
-    if not (HETRD_MIN_N <= n <= HETRD_MAX_N):
         return torch.linalg.eigh(H)
-    if _rocsolver is None:
-        _rocsolver = _RocSolver()
-    Hf = H.reshape((-1, n, n))
-    E = torch.empty(Hf.shape[:-1], dtype=torch.float64, device=H.device)
-    V = torch.empty(Hf.shape, dtype=torch.complex128, device=H.device)
-    info = torch.zeros(Hf.shape[0], dtype=torch.int32, device=H.device)
-    need = int(_lib.lib().qmx_hetrd_workspace_bytes(n))
-    ws = _hetrd_ws.get(H.device)
-    if ws is None or ws.numel() < need:
-        ws = torch.empty(need, dtype=torch.uint8, device=H.device)
-        _hetrd_ws[H.device] = ws
-    e = torch.empty(n, dtype=torch.float64, device=H.device)
-    for b in range(Hf.shape[0]):
-        A = Hf[b].contiguous()
-        Q = torch.empty((n, n), dtype=torch.complex128, device=H.device)  # column-major
-        _lib.call("qmx_hetrd", A.data_ptr(), n, n, E[b].data_ptr(), e.data_ptr(), Q.data_ptr(),
-                  ws.data_ptr(), need, HETRD_GROUPS, HETRD_THREADS, rc.stream_ptr())
-        _rocsolver.zstedc(E[b], e, Q, info[b:b + 1])
-        V[b] = Q.transpose(0, 1)
-    if bool((info != 0).any()):
-        raise torch.linalg.LinAlgError(f"tridiagonal eigensolver failed (info {info.tolist()})")
-    return E.reshape(batch + (n,)), V.reshape(batch + (n, n))
+    if n <= LARGE_MAX_N:
+        E, V, _ = eigh_large(H, n_want)
+        return E, V
+    return torch.linalg.eigh(H)
 
 
 def abs_squared(x: torch.Tensor) -> torch.Tensor:
@@ -281,10 +268,48 @@ def _fp64(x: torch.Tensor) -> torch.Tensor:
     return x.to(torch.float64)
 
 
+def cholesky_inverse(O: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """(X, info) for Hermitian positive definite complex128 O (..., n, n) on
+    the GPU, n <= POTRF_MAX_N: X = L^-1 with O = L L^H (lower triangular,
+    from the lower triangle of O, as torch.linalg.cholesky), by
+    qmx_potrf_inv one matrix per call; info (int32 per matrix) is 0 or k + 1
+    for the first non-positive pivot k (X then not meaningful), as
+    torch.linalg.cholesky_ex."""
+    n = O.shape[-1]
+    batch = O.shape[:-2]
+    Of = O.to(torch.complex128).reshape((-1, n, n))
+    if Of.stride(-1) != 1 or Of.stride(-2) != n:
+        Of = Of.contiguous()
+    need = int(_lib.lib().qmx_potrf_inv_work_bytes(n))
+    ws = _potrf_ws.get(O.device)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=O.device)
+        _potrf_ws[O.device] = ws
+    X = torch.empty_like(Of)
+    info = torch.empty(Of.shape[0], dtype=torch.int32, device=O.device)
+    for b in range(Of.shape[0]):
+        _lib.call("qmx_potrf_inv", Of[b].data_ptr(), n, n, X[b].data_ptr(), info[b:].data_ptr(),
+                  ws.data_ptr(), need, POTRF_GROUPS, rc.stream_ptr())
+    return X.reshape(batch + (n, n)), info.reshape(batch)
+
+
+def _own_cholesky(O: torch.Tensor) -> bool:
+    return O.is_cuda and O.is_complex() and 0 < O.shape[-1] <= POTRF_MAX_N
+
+
 def ortho_matrix(O: torch.Tensor, use_cholesky: bool = True) -> torch.Tensor:
+    """U with U^H O U = 1: L^-H of the Cholesky factor O = L L^H (use_cholesky,
+    as math/_linalg.py:110-141; on the GPU by qmx_potrf_inv), else the
+    symmetric O^-1/2."""
     assert O.shape[-2] == O.shape[-1]
     O = _fp64(O)
     if use_cholesky:
+        if _own_cholesky(O):
+            X, info = cholesky_inverse(O)
+            if bool((info != 0).any()):
+                raise torch.linalg.LinAlgError(
+                    f"ortho_matrix: the overlap is not positive definite (info {info.tolist()})")
+            return dagger(X)
         identity = torch.eye(O.shape[-1], device=O.device, dtype=O.dtype).view(
             (1,) * (O.dim() - 2) + O.shape[-2:]
         )
@@ -294,8 +319,10 @@ def ortho_matrix(O: torch.Tensor, use_cholesky: bool = True) -> torch.Tensor:
     return V @ ((1.0 / torch.sqrt(lbda))[..., None] * dagger(V))
 
 
-def eighg(H, O: torch.Tensor, use_cholesky: bool = True):
-    """E (ascending) and V with H V = O V E; H may be a Waitable."""
+def eighg(H, O: torch.Tensor, use_cholesky: bool = True, n_want: Optional[int] = None):
+    """E (ascending) and V with H V = O V E; H may be a Waitable. n_want
+    (optional): only the eigenvectors of the n_want lowest eigenvalues are
+    needed (V may then have just those columns)."""
     rc.compute_stream_wait_current()
     if rc.compute_stream is not None:
         with torch.cuda.stream(rc.compute_stream):
@@ -304,11 +331,12 @@ def eighg(H, O: torch.Tensor, use_cholesky: bool = True):
         U = ortho_matrix(O, use_cholesky)
     Hres = _fp64(H.wait() if hasattr(H, "wait") else H)
     rc.current_stream_wait_compute()
-    E, V = eigh(dagger(U) @ (Hres @ U))
+    E, V = eigh(dagger(U) @ (Hres @ U), n_want)
     return E, U @ V
 
 
-def eighg_canonical(H, O: torch.Tensor, rtol: float = 1e-4, shift: float = 1e6):
+def eighg_canonical(H, O: torch.Tensor, rtol: float = 1e-4, shift: float = 1e6,
+                    n_want: Optional[int] = None):
     """H V = O V E by canonical orthogonalisation, for overlaps that rounding
     has made indefinite or near-singular (subspaces assembled in float32).
 
@@ -327,7 +355,7 @@ def eighg_canonical(H, O: torch.Tensor, rtol: float = 1e-4, shift: float = 1e6):
     U = W * s[..., None, :].to(W.dtype)
     Hr = dagger(U) @ (Hres @ U)
     Hr = Hr + torch.diag_embed(torch.where(keep, 0.0, shift).to(Hr.dtype))
-    E, V = eigh(Hr)
+    E, V = eigh(Hr, n_want)
     return E, U @ V
 
 
@@ -335,17 +363,24 @@ def eighg_canonical(H, O: torch.Tensor, rtol: float = 1e-4, shift: float = 1e6):
 n_canonical_fallbacks = 0
 
 
-def eighg_robust(H, O: torch.Tensor):
+def eighg_robust(H, O: torch.Tensor, n_want: Optional[int] = None):
     """Cholesky eighg when the overlap factorises, canonical orthogonalisation
     (eighg_canonical) when float32 rounding has made it indefinite."""
     global n_canonical_fallbacks
     O = _fp64(O)
-    L, info = torch.linalg.cholesky_ex(O)
-    if bool((info != 0).any()):
-        n_canonical_fallbacks += 1
-        return eighg_canonical(H, O)
-    identity = torch.eye(O.shape[-1], device=O.device, dtype=O.dtype).expand_as(L)
-    U = dagger(torch.linalg.solve_triangular(L, identity, upper=False))
+    if _own_cholesky(O):
+        X, info = cholesky_inverse(O)
+        if bool((info != 0).any()):
+            n_canonical_fallbacks += 1
+            return eighg_canonical(H, O, n_want=n_want)
+        U = dagger(X)
+    else:
+        L, info = torch.linalg.cholesky_ex(O)
+        if bool((info != 0).any()):
+            n_canonical_fallbacks += 1
+            return eighg_canonical(H, O, n_want=n_want)
+        identity = torch.eye(O.shape[-1], device=O.device, dtype=O.dtype).expand_as(L)
+        U = dagger(torch.linalg.solve_triangular(L, identity, upper=False))
     Hres = _fp64(H.wait() if hasattr(H, "wait") else H)
-    E, V = eigh(dagger(U) @ (Hres @ U))
+    E, V = eigh(dagger(U) @ (Hres @ U), n_want)
     return E, U @ V
