@@ -289,6 +289,108 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// ----------------------------------------------------- density, pass C
+// Density on the separate passes (x-y planes too large for the plane pass):
+// after the inverse z- and y-transforms (passes A and B into T2), one
+// workgroup takes one y, one z-tile of Mz (a power of two), one k and a chunk
+// of bands, nbg bands per group (M = nbg * Mz lines, M / 4 per wave). The
+// active x rows of the group's (band, z) lines are scattered into the zeroed
+// plane and transformed along x (inverse), each wave its own lines; then the
+// thread that owns a point (x, z) adds w |psi|^2 of the group's bands in
+// band order to its FP64 accumulator in LDS. After the last group the
+// accumulators are added atomically to the x-major density rhoX[x][y][z].
+// CROSS: T2b holds the second spinor component of the same bands; both are
+// transformed and w psi_1 conj(psi_0) is accumulated, the imaginary part
+// into rhoX[N + ...].
+template <typename C2, typename R, bool CROSS>
+__global__ void __launch_bounds__(kThreads)
+    k_xpass_density(const C2* __restrict__ T2, const C2* __restrict__ T2b,
+                    const double* __restrict__ prefac, int64_t ldp, const int* __restrict__ x_list,
+                    const C2* __restrict__ tw_g, Radices rx, FastDiv fmw, FastDiv fM, int zshift,
+                    int nbg, int ny, int nz, int nxa, int nb, int b0, int bands_per_chunk,
+                    int n_chunks, double* __restrict__ rhoX) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int kNp = CROSS ? 2 : 1;
+  const int nx = rx.n, M = fM.d, ld = odd_ld(M), Mz = 1 << zshift, zmask = Mz - 1;
+  const int mw = fmw.d;
+  C2* tw = reinterpret_cast<C2*>(smem_raw);
+  C2* pa = tw + nx;  // kNp pairs of planes (a, b), nx x ld each
+  double* acc = reinterpret_cast<double*>(pa + 2 * kNp * nx * ld);  // kNp x nx x Mz
+  const int z0 = blockIdx.x * Mz, y = blockIdx.y;
+  const int k = blockIdx.z / n_chunks, chunk = blockIdx.z - k * n_chunks;
+  const int mz = min(Mz, nz - z0);
+  const int bstart = chunk * bands_per_chunk;
+  const int bstop = min(nb, bstart + bands_per_chunk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int npt = nx * Mz;
+  load_twiddles(tw, tw_g, nx);
+  for (int t = threadIdx.x; t < kNp * npt; t += blockDim.x) acc[t] = 0.0;
+  const int64_t plane = (int64_t)ny * nz;
+  const C2* src[2] = {T2, T2b};
+  for (int g0 = bstart; g0 < bstop; g0 += nbg) {
+    __syncthreads();  // the previous group's planes have been read
+    zero_lds(pa, 2 * kNp * nx * ld);
+    __syncthreads();
+    for (int c = 0; c < kNp; ++c) {
+      C2* a = pa + 2 * c * nx * ld;
+      for (int i = threadIdx.x; i < nxa * M; i += blockDim.x) {
+        const int xa = fdiv(i, fM);
+        const int l = i - xa * M;
+        const int ib = g0 + (l >> zshift), zl = l & zmask;
+        if (ib < bstop && zl < mz)
+          a[x_list[xa] * ld + l] =
+              src[c][(((int64_t)k * nb + ib) * nxa + xa) * plane + (int64_t)y * nz + z0 + zl];
+      }
+    }
+    __syncthreads();
+    const int l0 = wave * mw;
+    C2* r[2];
+    for (int c = 0; c < kNp; ++c) {
+      C2* a = pa + 2 * c * nx * ld;
+      r[c] = fft_lds_wave<C2, +1>(a, a + nx * ld, tw, rx, fmw, ld, l0);
+    }
+    __syncthreads();  // every wave's lines are transformed
+    const int ng = min(nbg, bstop - g0);
+    for (int t = threadIdx.x; t < npt; t += blockDim.x) {
+      const int x = t >> zshift, zl = t & zmask;
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < ng; ++j) {
+        const double w = prefac[(int64_t)k * ldp + b0 + g0 + j];
+        const C2 v = r[0][x * ld + (j << zshift) + zl];
+        if constexpr (CROSS) {  // psi_1 conj(psi_0), psi_0 = v, psi_1 = u
+          const C2 u = r[1][x * ld + (j << zshift) + zl];
+          sr += w * ((double)u.x * (double)v.x + (double)u.y * (double)v.y);
+          si += w * ((double)u.y * (double)v.x - (double)u.x * (double)v.y);
+        } else {
+          sr += w * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        }
+      }
+      acc[t] += sr;
+      if constexpr (CROSS) acc[npt + t] += si;
+    }
+  }
+  __syncthreads();
+  const int64_t N = (int64_t)nx * plane;
+  for (int t = threadIdx.x; t < npt; t += blockDim.x) {
+    const int x = t >> zshift, zl = t & zmask;
+    if (zl >= mz) continue;
+    const int64_t o = ((int64_t)x * ny + y) * nz + z0 + zl;
+    if (acc[t] != 0.0) unsafeAtomicAdd(rhoX + o, acc[t]);
+    if constexpr (CROSS)
+      if (acc[npt + t] != 0.0) unsafeAtomicAdd(rhoX + N + o, acc[npt + t]);
+  }
+}
+
+// dst (= or +=) src, n doubles.
+template <bool kAccumulate>
+__global__ void __launch_bounds__(256)
+    k_density_add(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if constexpr (kAccumulate) dst[i] += src[i];
+    else dst[i] = src[i];
+  }
+}
+
 // ---------------------------------------------------------------- pass D
 template <typename C2, typename R>
 __global__ void __launch_bounds__(kThreads)
@@ -990,6 +1092,142 @@ static void launch_density(dim3 grid, size_t sh, hipStream_t st, const C2* T1, c
                      tw_of<C2>(p, 1), rxc, ryc, fny, fnxa, p->ncol, p->nxa, cur, b0, bpg, rhoT);
 }
 
+// Whether the density of this box runs on the plane pass (x-y planes, two
+// for the spinor cross term, in LDS and within the accumulator registers);
+// otherwise on the separate passes.
+static bool density_plane(const qmx_plan* p, int cs, bool cross) {
+  if (!use_plane_path(p, cs)) return false;
+  const size_t shP = plane_lds_bytes(p, cs) + (cross ? (size_t)p->shape[0] * odd_ld(p->shape[1]) * cs : 0);
+  const int ntP = plane_threads(p);
+  const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
+  return shP <= (size_t)kLdsMax && (int64_t)p->shape[0] * p->shape[1] <= (int64_t)ntP * maxeP;
+}
+
+// Workspace per band and k of the density (T1, plus T2 on the separate passes).
+static int64_t density_band_bytes(const qmx_plan* p, int cs, bool cross) {
+  return plane_aware_band_bytes(p, cs, density_plane(p, cs, cross)) * (cross ? 2 : 1);
+}
+
+template <typename C2, typename R, bool CROSS>
+static int set_xdensity_lds(size_t bytes) {
+  static size_t done = 0;
+  if (bytes > done) {
+    QMX_HIP(hipFuncSetAttribute((const void*)k_xpass_density<C2, R, CROSS>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done = bytes;
+  }
+  return QMX_OK;
+}
+
+// The density on the separate passes: passes A and B of the local apply
+// (z- then y-transforms into T2) for each component, then k_xpass_density;
+// rho (= or +=) the x-major accumulator.
+template <typename R>
+int collect_density_separate(const qmx_plan* p, int k0, int nk, const void* Cv, const void* Cv1,
+                             int64_t ldb, int64_t ldk, int64_t nb, const double* prefac,
+                             int64_t ldp, double* rho, int accumulate, void* ws, int64_t ws_bytes,
+                             hipStream_t st, const char* name) {
+  using C2 = typename Cplx<R>::T;
+  const bool cross = Cv1 != nullptr;
+  const int n_comp = cross ? 2 : 1;
+  const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
+  const int ncol = p->ncol, nxa = p->nxa, cs = sizeof(C2);
+  const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
+  const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
+  const int64_t fixed = density_fixed_bytes(p, cross);
+  const int64_t per_band = density_band_bytes(p, cs, cross);
+  if (ws_bytes <= fixed || (ws_bytes - fixed) / (per_band * nk) < 1) {
+    set_error(std::string(name) + ": workspace smaller than the density and one band of every k");
+    return QMX_ERR_ARG;
+  }
+  int64_t blk = (ws_bytes - fixed) / (per_band * nk);
+  blk = std::min<int64_t>(blk, nb);
+  blk = std::min<int64_t>(blk, 65535 / nk);
+  // Pass C tiles: z-tile Mzc (power of two, <= 16), nbg bands per group,
+  // M = nbg * Mzc lines (a multiple of the 4 waves) under the LDS budget.
+  auto need_c = [&](int M, int Mzc) {
+    return (size_t)(2 * n_comp * odd_ld(M) * nx + nx) * cs + (size_t)n_comp * nx * Mzc * 8;
+  };
+  int Mzc = 16, nbg = 0;
+  while (Mzc > 1 && Mzc / 2 >= nz) Mzc /= 2;
+  for (size_t budget : {(size_t)kLdsBudget, (size_t)kLdsMax}) {
+    for (int mz = Mzc; mz >= 1 && !nbg; mz /= 2)
+      for (int g = 8; g >= 1; g /= 2)
+        if ((g * mz) % 4 == 0 && need_c(g * mz, mz) <= budget) {
+          Mzc = mz;
+          nbg = g;
+          break;
+        }
+    if (nbg) break;
+  }
+  if (!nbg) {
+    set_error(std::string(name) + ": x lines of this box exceed LDS");
+    return QMX_ERR_ARG;
+  }
+  const int Mx = nbg * Mzc;
+  int zshift = 0;
+  while ((1 << zshift) < Mzc) ++zshift;
+  const int Mz = max_lines(nz, cs, 0, 64);
+  const int My = even_tile(nz, max_lines(ny, cs, 0, 64));
+  const FastDiv fz = make_fastdiv(Mz), fy = make_fastdiv(My), fxw = make_fastdiv(Mx / 4),
+                fxM = make_fastdiv(Mx);
+  const size_t shA = (size_t)(2 * odd_ld(Mz) * nz + nz) * cs;
+  const size_t shB = (size_t)(2 * odd_ld(My) * ny + ny) * cs;
+  const size_t shC = need_c(Mx, Mzc);
+  int rc = cross ? set_xdensity_lds<C2, R, true>(shC) : set_xdensity_lds<C2, R, false>(shC);
+  if (rc) return rc;
+  const C2* Cs[2] = {static_cast<const C2*>(Cv), static_cast<const C2*>(Cv1)};
+  double* rhoX = static_cast<double*>(ws);
+  const int64_t n_acc = (int64_t)p->N * n_comp;
+  hipLaunchKernelGGL(k_fill_zero, dim3(std::min<int64_t>((n_acc + 255) / 256, 4096)), dim3(256), 0,
+                     st, rhoX, n_acc);
+  QMX_LAUNCHED();
+  const double* pf = prefac + k0 * ldp;
+  for (int64_t b0 = 0; b0 < nb; b0 += blk) {
+    const int cur = (int)std::min<int64_t>(blk, nb - b0);
+    const int64_t t1n = (int64_t)cur * nk * ncol * nz, t2n = (int64_t)cur * nk * nxa * ny * nz;
+    C2* T1 = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
+    C2* T2[2] = {T1 + n_comp * t1n, T1 + n_comp * t1n + t2n};
+    for (int c = 0; c < n_comp; ++c) {
+      C2* T1c = T1 + c * t1n;
+      dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
+      hipLaunchKernelGGL((k_sphere_to_cols<C2, R, false>), gA, dim3(kThreads), shA, st, Cs[c], ldb,
+                         ldk, T1c, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
+                         p->n_tot, (int)b0);
+      QMX_LAUNCHED();
+      dim3 gB((nz + My - 1) / My, nxa, nk * cur);
+      hipLaunchKernelGGL((k_cols_to_planes<C2, R>), gB, dim3(kThreads), shB, st, T1c, T2[c],
+                         p->d_pcol_start, p->d_col_iy, tw_of<C2>(p, 1), p->rad[1], fy, ncol, nxa,
+                         nz);
+      QMX_LAUNCHED();
+    }
+    const int64_t wg_base = (int64_t)((nz + Mzc - 1) / Mzc) * ny * nk;
+    int n_chunks = (int)std::min<int64_t>((cur + nbg - 1) / nbg,
+                                          std::max<int64_t>(1, (2048 + wg_base - 1) / wg_base));
+    n_chunks = std::min<int>(n_chunks, 65535 / nk);
+    int bpc = (cur + n_chunks - 1) / n_chunks;
+    bpc = (bpc + nbg - 1) / nbg * nbg;
+    n_chunks = (cur + bpc - 1) / bpc;
+    dim3 gC((nz + Mzc - 1) / Mzc, ny, nk * n_chunks);
+    if (cross)
+      hipLaunchKernelGGL((k_xpass_density<C2, R, true>), gC, dim3(kThreads), shC, st, T2[0], T2[1],
+                         pf, ldp, p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fxw, fxM, zshift, nbg,
+                         ny, nz, nxa, cur, (int)b0, bpc, n_chunks, rhoX);
+    else
+      hipLaunchKernelGGL((k_xpass_density<C2, R, false>), gC, dim3(kThreads), shC, st, T2[0],
+                         nullptr, pf, ldp, p->d_x_list, tw_of<C2>(p, 0), p->rad[0], fxw, fxM,
+                         zshift, nbg, ny, nz, nxa, cur, (int)b0, bpc, n_chunks, rhoX);
+    QMX_LAUNCHED();
+  }
+  const unsigned gD = (unsigned)std::min<int64_t>((n_acc + 255) / 256, 8192);
+  if (accumulate)
+    hipLaunchKernelGGL(k_density_add<true>, dim3(gD), dim3(256), 0, st, rhoX, rho, n_acc);
+  else
+    hipLaunchKernelGGL(k_density_add<false>, dim3(gD), dim3(256), 0, st, rhoX, rho, n_acc);
+  QMX_LAUNCHED();
+  return QMX_OK;
+}
+
 // Cv1 == nullptr: rho (N doubles) = or += sum_b w_b |psi_b|^2. Otherwise the
 // spinor cross term: rho (2 N doubles, real then imaginary part) = or +=
 // sum_b w_b psi1_b conj(psi0_b), psi0 in Cv, psi1 in Cv1 (same strides).
@@ -1003,15 +1241,10 @@ int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, cons
   const int nx = p->shape[0], ny = p->shape[1], nz = p->shape[2];
   const int ncol = p->ncol, nxa = p->nxa, cs = sizeof(C2);
   const char* name = cross ? "qmx_collect_density_cross" : "qmx_collect_density";
-  if (!use_plane_path(p, cs)) {
-    set_error(std::string(name) + ": the box's x-y planes exceed the plane pass (LDS)");
-    return QMX_ERR_ARG;
-  }
+  if (!density_plane(p, cs, cross))
+    return collect_density_separate<R>(p, k0, nk, Cv, Cv1, ldb, ldk, nb, prefac, ldp, rho,
+                                       accumulate, ws, ws_bytes, st, name);
   const size_t shP = plane_lds_bytes(p, cs) + (cross ? (size_t)nx * odd_ld(ny) * cs : 0);
-  if (shP > (size_t)kLdsMax) {
-    set_error(std::string(name) + ": two x-y planes of this box exceed LDS");
-    return QMX_ERR_ARG;
-  }
   const int* t1idx = p->d_t1idx + (int64_t)k0 * p->n_tot;
   const int* cstart = p->d_cstart + (int64_t)k0 * (ncol + 1);
   const int64_t fixed = density_fixed_bytes(p, cross);
@@ -1040,10 +1273,6 @@ int collect_density_impl(const qmx_plan* p, int k0, int nk, const void* Cv, cons
                  : (small ? set_density_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall, false>(shP)
                           : set_density_lds<C2, R, kPlaneThreads, kPlaneMaxE, false>(shP));
   if (rc) return rc;
-  if ((int64_t)nx * ny > (int64_t)ntP * maxeP) {
-    set_error(std::string(name) + ": plane larger than the accumulator registers");
-    return QMX_ERR_ARG;
-  }
   const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
   const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
   const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
@@ -1552,15 +1781,13 @@ int qmx_apply_local_k(const qmx_plan* plan, int64_t k0, int64_t nk, int dtype, c
 int64_t qmx_density_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
   if (!plan || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
   const int cs = dtype == QMX_C64 ? 8 : 16;
-  return density_fixed_bytes(plan, false) +
-         (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+  return density_fixed_bytes(plan, false) + density_band_bytes(plan, cs, false) * plan->nk * n_bands;
 }
 
 int64_t qmx_density_cross_workspace_bytes(const qmx_plan* plan, int dtype, int64_t n_bands) {
   if (!plan || (dtype != QMX_C64 && dtype != QMX_C128) || n_bands < 0) return -1;
   const int cs = dtype == QMX_C64 ? 8 : 16;
-  return density_fixed_bytes(plan, true) +
-         2 * (int64_t)plan->ncol * plan->shape[2] * cs * plan->nk * n_bands;
+  return density_fixed_bytes(plan, true) + density_band_bytes(plan, cs, true) * plan->nk * n_bands;
 }
 
 static int collect_density_entry(const char* name, const qmx_plan* plan, int64_t k0, int64_t nk,

@@ -13,6 +13,9 @@
   energies from the oracle's dense brute force (tests/golden/
   dense_cells.npz, written by tests/golden/make_dense_fixtures.py), and a
   request alone against the same request in the mix.
+- The density on the separate passes (boxes 24..60^3 forced, the 144^3 box,
+  the complex128 spinor cross term on 96^3) against the oracle, plain and
+  spinor.
 - A plan used on a non-blocking stream straight after creation, and a plan
   block reused while the previous owner's work is still queued.
 
@@ -139,6 +142,63 @@ def test_apply_local_cfg4_box(gpu, dtype):
     basis = Basis.from_spec(spec)
     assert basis.local_path(dtype) == "separate"
     _check_apply(spec, basis, dtype, 2, 44, "separate")
+
+
+def _density_check(basis, ob, dtype, n_bands, seed, spinor):
+    """Plain (and, for spinor, need_Mvec) density of random bands with
+    fractional fillings and one empty band against the oracle, relative to
+    the peak density: complex128 to 1e-12, complex64 to 1e-5."""
+    from oracle.density import collect_density as o_density
+    from oracle.density import collect_density_cross as o_cross
+    tol = 1e-12 if dtype == C128 else 1e-5
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0.0, 1.0, (len(ob.k), n_bands))
+    f[:, 1] = 0.0
+    C0 = rand_coeff(ob, n_bands, seed=seed)
+    if not spinor:
+        ref = o_density(ob, C0, f)
+        got = basis.collect_density(to_wf(basis, C0, dtype), torch.tensor(f)[None])
+        err = rel(got[0], ref)
+        print(f"    density {err:.2e}", flush=True)
+        assert err < tol
+        return
+    C1 = rand_coeff(ob, n_bands, seed=seed + 1)
+    coeff = torch.cat([to_wf(basis, C0, dtype).coeff, to_wf(basis, C1, dtype).coeff], dim=3)
+    r0, r1 = o_density(ob, C0, f), o_density(ob, C1, f)
+    X = o_cross(ob, C0, C1, f)
+    got = basis.collect_density(coeff, torch.tensor(f)[None], need_Mvec=True).cpu().numpy()
+    peak = np.abs(r0 + r1).max()
+    for i, want in enumerate((r0 + r1, 2 * X.real, 2 * X.imag, r0 - r1)):
+        err = np.abs(got[i] - want).max() / peak
+        print(f"    spinor component {i}: {err:.2e}", flush=True)
+        assert err < tol
+
+
+@pytest.mark.parametrize("g", [24, 36, 48, 60])
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("spinor", [False, True], ids=["plain", "spinor"])
+def test_collect_density_separate_boxes(gpu, g, dtype, spinor):
+    """The density on the separate passes (passes A and B of the apply, then
+    the x-pass accumulation k_xpass_density), forced on the boxes of the
+    batched workloads, against the oracle."""
+    spec = _cell_with_box(g)
+    print(f"[collect_density_separate] {g}^3 {dtype} spinor={spinor}", flush=True)
+    basis = Basis.from_spec(spec)
+    basis.set_local_path(1)
+    _density_check(basis, OBasis(spec), dtype, 4, g + 7, spinor)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("spinor", [False, True], ids=["plain", "spinor"])
+def test_collect_density_large_boxes(gpu, dtype, spinor):
+    """Boxes whose x-y planes do not fit the plane pass: the 144^3 box of
+    the 256-atom cell (both precisions) and, for the spinor cross term in
+    complex128, the 96^3 box of the 64-atom cell (two 96 x 97 planes exceed
+    LDS), against the oracle with three bands."""
+    spec = systems.make_cell(4 if (not spinor or dtype == C64) else 1)
+    print(f"[collect_density_large] {spec.shape} {dtype} spinor={spinor}", flush=True)
+    basis = Basis.from_spec(spec)
+    _density_check(basis, OBasis(spec), dtype, 3, 5, spinor)
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])

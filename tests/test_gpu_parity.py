@@ -236,19 +236,16 @@ def test_collect_density_spinor(gpu, g, dtype):
 
 
 def test_collect_density_spinor_large_plane(gpu):
-    """96^3 box (512-thread plane kernels): the complex64 cross term against
-    the oracle; complex128 needs two 96 x 97 planes of LDS (298 KB) and the
-    cross term is refused with an error, before any launch."""
+    """96^3 box (512-thread plane kernels): the complex64 cross term on the
+    plane pass and the complex128 one (two 96 x 97 planes exceed LDS) on the
+    separate passes, against the oracle."""
     spec = systems.make_cell(1)
     print("[collect_density spinor] 96^3", flush=True)
     ob = OBasis(spec)
     el = electrons_from_spec(spec)
     b = el.basis
     _spinor_density_check(b, ob, C64, 3, seed=11)
-    C = rand_coeff(ob, 2, seed=5)
-    coeff = torch.cat([to_wf(b, C, C128).coeff] * 2, dim=3)
-    with pytest.raises(RuntimeError, match="two x-y planes"):
-        b.collect_density(coeff, torch.ones(1, len(ob.k), 2, dtype=torch.float64), need_Mvec=True)
+    _spinor_density_check(b, ob, C128, 3, seed=12)
 
 
 def test_apply_local_blocking_inplace_accumulate(cfg0_el):
