@@ -711,6 +711,116 @@ class _GroupTail:
 
 
 # ---------------------------------------------------------------- solver
+class Scheduler:
+    """Host bookkeeping of the served solver, with no device calls (so that
+    CPU tests drive it with simulated jobs and buckets): the FIFO queue, the
+    HBM budget held by admitted jobs, the group of every bucket key and
+    cancellations. Jobs need ``job_id``, ``arrival``, ``_key`` (shape, band
+    class), ``K``, ``n_tot`` and ``hbm_bytes(nb, dtype, G)``."""
+
+    def __init__(self, hbm_budget: float, max_k_per_bucket: int, max_bucket_bytes: float,
+                 csize: int, dtype=None, n_groups: int = 1):
+        self.hbm_budget = hbm_budget
+        self.max_k_per_bucket = max_k_per_bucket
+        self.max_bucket_bytes = max_bucket_bytes
+        self.csize = csize
+        self.dtype = dtype
+        self.n_groups = max(1, int(n_groups))
+        self.queue: list = []
+        self.hbm_in_use = 0
+        self.held: dict[int, int] = {}
+        self.group_of: dict = {}  # bucket key -> group
+        self.cancelled: set = set()  # job ids withdrawn (never give a result)
+        self._cancel_running: set = set()  # running jobs to drop at their next tick
+
+    def submit(self, jobs) -> None:
+        self.queue.extend(jobs)
+
+    def cancel(self, job_id: int, running_ids) -> bool:
+        """Withdraw a job: from the queue at once, or (running) at its
+        group's next tick. False when the job is neither queued nor
+        running."""
+        for i, j in enumerate(self.queue):
+            if j.job_id == job_id:
+                del self.queue[i]
+                self.cancelled.add(job_id)
+                return True
+        if job_id in running_ids:
+            self._cancel_running.add(job_id)
+            return True
+        return False
+
+    def split_retiring(self, jobs, converged) -> tuple[list, list]:
+        """(jobs to finish, jobs to drop) of one bucket at a tick: converged
+        jobs finish unless cancelled; cancelled ones are dropped, result-less."""
+        drop = [j for j in jobs if j.job_id in self._cancel_running]
+        for j in drop:
+            self._cancel_running.discard(j.job_id)
+            self.cancelled.add(j.job_id)
+        dropped = {j.job_id for j in drop}
+        return [j for j in converged if j.job_id not in dropped], drop
+
+    def release(self, job_id: int) -> None:
+        """A job left its bucket: its HBM estimate is free for admission."""
+        self.hbm_in_use -= self.held.pop(job_id, 0)
+
+    def admit(self, now: Optional[float], group: int, buckets: dict, reserved: int,
+              loads) -> dict:
+        """Take queued jobs (arrived by `now`) that fit into buckets of
+        `group` (a key seen for the first time joins the group with the
+        least work, `loads()` giving each group's); returns key -> jobs.
+        `buckets` maps key -> (k-points, basis size) of the bucket's current
+        jobs. The budget holds against `reserved` (the bytes live device
+        tensors hold) plus the estimates admitted in this call, and against
+        the estimates of every admitted job; the first job is always taken.
+        The queue is FIFO: once the budget is spent no later job is taken,
+        and a full bucket (k-points or state bytes) takes no later job of
+        its key."""
+        taken: dict = {}
+        rest = []
+        load = None
+        pending = 0
+        full: set = set()
+        for pos, job in enumerate(self.queue):
+            if now is not None and job.arrival > now:
+                rest.extend(self.queue[pos:])  # arrivals are in order
+                break
+            key = job._key
+            if key in full:
+                rest.append(job)
+                continue
+            if key not in self.group_of:
+                if load is None:
+                    load = loads()
+                g = int(np.argmin(load))
+                self.group_of[key] = g
+                load[g] += float(job.K) * job.n_tot * key[1]
+            if self.group_of[key] != group:
+                rest.append(job)
+                continue
+            kb, gb = buckets.get(key, (0, 0))
+            joining = taken.get(key, [])
+            k_in = kb + sum(j.K for j in joining)
+            G = max([gb] + [j.n_tot for j in joining] + [job.n_tot])
+            need = job.hbm_bytes(key[1], self.dtype, G)
+            if max(self.hbm_in_use, reserved + pending) + need > self.hbm_budget and (
+                    self.hbm_in_use > 0):
+                rest.extend(self.queue[pos:])
+                break
+            in_bucket = 4 * (2 * key[1]) * G * self.csize * (k_in + job.K)  # X, HX, buffers
+            if k_in > 0 and (k_in + job.K > self.max_k_per_bucket
+                             or in_bucket > self.max_bucket_bytes):
+                full.add(key)
+                rest.append(job)
+                continue
+            pending += need
+            self.hbm_in_use += need
+            self.held[job.job_id] = need
+            taken.setdefault(key, []).append(job)
+        self.queue = rest
+        return taken
+
+
 class BatchSolver:
     """Runs band-solve jobs in buckets, admitting new jobs as others finish.
 
@@ -754,14 +864,12 @@ class BatchSolver:
         self.max_bucket_bytes = max_bucket_bytes
         self._csize = torch.empty((), dtype=dtype).element_size()
         self.buckets: dict = {}
-        self.queue: list[CellJob] = []
         self.results: dict[int, CellResult] = {}
         self.pending: list[_Pending] = []
-        self.hbm_in_use = 0
-        self._held: dict[int, int] = {}
         self._info: list[torch.Tensor] = []
         self.n_groups = max(1, int(groups))
-        self._group_of: dict = {}  # bucket key -> group
+        self.sched = Scheduler(hbm_budget, max_k_per_bucket, max_bucket_bytes, self._csize,
+                               dtype, self.n_groups)
         self._tail = [_GroupTail() for _ in range(self.n_groups)]
         self._turn = 0
         self.n_ticks = 0
@@ -774,6 +882,27 @@ class BatchSolver:
     def clock(self) -> float:
         """Seconds since the solver started (the time base of arrivals)."""
         return time.perf_counter() - self.t0
+
+    @property
+    def queue(self) -> list:
+        return self.sched.queue
+
+    @property
+    def hbm_in_use(self) -> int:
+        return self.sched.hbm_in_use
+
+    @property
+    def cancelled(self) -> set:
+        return self.sched.cancelled
+
+    def cancel(self, job_id: int) -> bool:
+        """Withdraw a request: a queued job leaves the queue; a running job
+        leaves its bucket at its group's next tick (its HBM and bucket rows
+        then free for admission). A cancelled job gives no result; the
+        other jobs' results are those they would have had without it.
+        False when the job is neither queued nor running."""
+        running = {j.job_id for b in self.buckets.values() for j in b.jobs}
+        return self.sched.cancel(job_id, running)
 
     def band_class(self, job: CellJob) -> int:
         """Smallest configured class that holds the job's n_bands +
@@ -792,7 +921,7 @@ class BatchSolver:
         """Queue jobs (FIFO); their bucket key is fixed here, once."""
         for job in jobs:
             job._key = (job.shape, self.band_class(job))
-        self.queue.extend(jobs)
+        self.sched.submit(jobs)
 
     def _bucket(self, key) -> Bucket:
         b = self.buckets.get(key)
@@ -807,62 +936,17 @@ class BatchSolver:
         load = [0.0] * self.n_groups
         for key, b in self.buckets.items():
             if b.jobs:
-                load[self._group_of[key]] += float(b.Kb) * b.Gb * b.nb
+                load[self.sched.group_of[key]] += float(b.Kb) * b.Gb * b.nb
         return load
 
     def _admit(self, now: Optional[float], group: int = 0) -> dict:
-        """Take queued jobs (arrived by `now`) that fit into buckets of
-        `group` (a bucket seen for the first time joins the group with the
-        least work); returns key -> jobs."""
-        taken: dict = {}
-        rest = []
-        load = None
-        # The budget holds against the bytes live tensors really hold (a
-        # host-side counter of the caching allocator; its cached free blocks
-        # are reused, so they do not count) plus the estimates of the jobs
-        # admitted in this call. The queue is FIFO: once the budget is spent
-        # no later job fits, and a full bucket takes no later job of its key.
-        reserved = torch.cuda.memory_allocated(rc.device)
-        pending = 0
-        full: set = set()
-        for pos, job in enumerate(self.queue):
-            if now is not None and job.arrival > now:
-                rest.extend(self.queue[pos:])  # arrivals are in order
-                break
-            key = job._key
-            if key in full:
-                rest.append(job)
-                continue
-            if key not in self._group_of:
-                if load is None:
-                    load = self._group_loads()
-                g = int(np.argmin(load))
-                self._group_of[key] = g
-                load[g] += float(job.K) * job.n_tot * key[1]
-            if self._group_of[key] != group:
-                rest.append(job)
-                continue
-            b = self.buckets.get(key)
-            joining = taken.get(key, [])
-            k_in = (b.Kb if b else 0) + sum(j.K for j in joining)
-            G = max([b.Gb if b else 0] + [j.n_tot for j in joining] + [job.n_tot])
-            need = job.hbm_bytes(key[1], self.dtype, G)
-            if max(self.hbm_in_use, reserved + pending) + need > self.hbm_budget and (
-                    self.hbm_in_use > 0):
-                rest.extend(self.queue[pos:])
-                break
-            in_bucket = 4 * (2 * key[1]) * G * self._csize * (k_in + job.K)  # X, HX, buffers
-            if k_in > 0 and (k_in + job.K > self.max_k_per_bucket
-                             or in_bucket > self.max_bucket_bytes):
-                full.add(key)
-                rest.append(job)
-                continue
-            pending += need
-            self.hbm_in_use += need
-            self._held[job.job_id] = need
-            taken.setdefault(key, []).append(job)
-        self.queue = rest
-        return taken
+        """Queued jobs (arrived by `now`) that fit into buckets of `group`
+        (Scheduler.admit against the bytes live tensors hold: a host-side
+        counter of the caching allocator, whose cached free blocks are
+        reused and so do not count); returns key -> jobs."""
+        state = {key: (b.Kb, b.Gb) for key, b in self.buckets.items() if b.jobs}
+        return self.sched.admit(now, group, state, torch.cuda.memory_allocated(rc.device),
+                                self._group_loads)
 
     def _collect(self, pending: list, host: list) -> None:
         for p, h in zip(pending, host):
@@ -905,16 +989,18 @@ class BatchSolver:
         host = {id(b): h.numpy() for b, h in tail.packs}
         # 2. converged jobs (their HBM is free for admission) and admissions
         #    (host decisions only).
-        retire = {id(b): b.converged_jobs(host[id(b)]) for b in active}
-        for done in retire.values():
-            for job in done:
-                self.hbm_in_use -= self._held.pop(job.job_id, 0)
+        retire, drop = {}, {}
+        for b in active:
+            retire[id(b)], drop[id(b)] = self.sched.split_retiring(
+                b.jobs, b.converged_jobs(host[id(b)]))
+            for job in retire[id(b)] + drop[id(b)]:
+                self.sched.release(job.job_id)
         newcomers = self._admit(now, g)
         # 3. the Davidson step of every bucket that keeps its jobs starts
         #    first (up to its first eigenproblem).
         steps, smats = [], []
         for b in active:
-            if not retire[id(b)] and (b.shape, b.nb) not in newcomers:
+            if not retire[id(b)] and not drop[id(b)] and (b.shape, b.nb) not in newcomers:
                 self._start(b.expand_steps(), steps, smats)
         # 4. retirement (each finish runs up to its eigenproblem before the
         #    restack), admission, restack and start of new jobs.
@@ -927,7 +1013,7 @@ class BatchSolver:
         for key in keys:
             b = self._bucket(key)
             new = newcomers.get(key, [])
-            gone = retire.get(id(b), [])
+            gone = retire.get(id(b), []) + drop.get(id(b), [])
             if new or gone:
                 b._restack([j for j in b.jobs if j not in gone], new)
                 for job in gone:
@@ -949,7 +1035,7 @@ class BatchSolver:
         #    the group's next tick reads.
         nxt = self._tail[g]
         for key, b in self.buckets.items():
-            if b.jobs and self._group_of.get(key) == g:
+            if b.jobs and self.sched.group_of.get(key) == g:
                 b.residuals()
                 nxt.packs.append((b, b.pack().to("cpu", non_blocking=True)))
         nxt.pending = [(p, p.data.to("cpu", non_blocking=True)) for p in self.pending]

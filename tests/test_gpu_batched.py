@@ -4,7 +4,8 @@ Several small cells of different boxes, band counts and k-meshes run
 together; every cell's eigenvalues must match LAPACK on its dense H within
 1e-8 Ha (plain, complex128) or 1e-6 Ha (mixed, complex64 + FP64
 Rayleigh-Ritz), and a cell's result must not depend on the company it runs
-in (alone vs in the mix, same tolerances)."""
+in (alone vs in the mix, same tolerances), on a budget that defers it, or
+on other requests cancelled while queued or running."""
 import numpy as np
 import pytest
 import torch
@@ -100,7 +101,7 @@ def test_bucket_groups_served_in_turn(gpu, dense_refs):
     while not solver.idle:
         solver.tick(now=now)
         now += 0.01
-    assert len(set(solver._group_of.values())) == 2  # both groups hold buckets
+    assert len(set(solver.sched.group_of.values())) == 2  # both groups hold buckets
     for i, ref in enumerate(dense_refs):
         r = solver.results[i]
         assert r.converged and np.abs(r.eig - ref).max() < TOL[C64]
@@ -118,3 +119,44 @@ def test_served_stream_matches_dense_and_alone(gpu, dense_refs):
         assert np.abs(rep.results[i].eig - ref).max() < TOL[C64]
     alone = serve([specs[2]], [0.0], _solver(C64)).results[0]
     assert np.abs(alone.eig - rep.results[2].eig).max() < TOL[C64]
+
+
+@pytest.mark.parametrize("dtype", [C128, C64], ids=["plain", "mixed"])
+def test_budget_and_cancellation(gpu, dense_refs, dtype):
+    """A budget that holds about two of the cells (the rest deferred until
+    others retire), one request cancelled while queued and one while
+    running: the cancelled ones give no result and every other cell's
+    eigenvalues are those of an unconstrained run (to the tolerance) and of
+    the dense reference."""
+    specs = _specs()
+    free_run = _solver(dtype)
+    free_run.submit([CellJob(s, job_id=i) for i, s in enumerate(specs)])
+    ref_res = free_run.run()
+    del free_run
+    torch.cuda.empty_cache()
+    solver = _solver(dtype)
+    jobs = [CellJob(s, job_id=i) for i, s in enumerate(specs)]
+    one = max(j.hbm_bytes(solver.band_class(j), dtype, j.n_tot) for j in jobs)
+    solver.sched.hbm_budget = torch.cuda.memory_allocated() + 1.8 * one
+    solver.submit(jobs)
+    print(f"[budget/cancel {dtype}] budget {solver.sched.hbm_budget / 1e9:.3f} GB, largest job"
+          f" {one / 1e6:.1f} MB", flush=True)
+    solver.tick()
+    queued = [j.job_id for j in solver.queue]
+    assert queued, "the budget should defer some cells"
+    assert solver.cancel(queued[-1])
+    running = [j.job_id for b in solver.buckets.values() for j in b.jobs]
+    assert solver.cancel(running[0])
+    assert not solver.cancel(999)
+    while not solver.idle:
+        solver.tick()
+    torch.cuda.synchronize()
+    gone = {queued[-1], running[0]}
+    assert solver.cancelled == gone
+    assert set(solver.results) == set(range(len(specs))) - gone
+    for i, r in solver.results.items():
+        err = np.abs(r.eig - ref_res[i].eig).max()
+        print(f"  cell {i}: |eig - unconstrained| {err:.2e}", flush=True)
+        assert err < TOL[dtype]
+        assert np.abs(r.eig - dense_refs[i]).max() < TOL[dtype]
+    assert solver.hbm_in_use == 0
