@@ -438,7 +438,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from qimpy_amd import gemm, rc, systems
+    from qimpy_amd import autotune, gemm, rc, systems
 
     rc.init()
     rc.require_gpu()
@@ -497,6 +497,7 @@ def main():
                       "n": wh["n_latency"], "note": "saturated server: queueing-dominated"},
         "step_ms_min_max": wh["step_ms_min_max"],
         "mean_iterations": round(wh["mean_iterations"], 1),
+        "tuned_launch_choices": autotune.load_default(),
     })
     if mixed is not None and plain is not None:
         out["plain_fp64"] = {"value": round(wp["value"], 3), "unit": "band solves/s",

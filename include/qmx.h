@@ -356,6 +356,28 @@ int64_t qmx_potrf_inv_work_bytes(int64_t n);
 int qmx_potrf_inv(const void* O, int64_t n, int64_t ldo, void* X, int* info, void* work,
                   int64_t work_bytes, int n_groups, void* stream);
 
+/* Tuned launch choices (no counterpart in the reference, which has no
+ * device kernels to tune): a process-wide table read by the launch code
+ * before its own cost model, filled by qimpy_amd.autotune from timings on
+ * the device (or its cache file). Keys and choices (int64):
+ * - QMX_TUNE_DOT, qmx_dot: key (dtype, qmx_tune_batch_bucket(batch), M, N,
+ *   qmx_tune_bucket(G)),
+ *   choice (kernel 0 = fitted blocks / 1 = 64 x 64 tiles, bm16, bn16 block
+ *   size in 16-row tiles (kernel 0), chunks of the basis axis);
+ * - QMX_TUNE_ROTATE, qmx_gemm: key (dtype, m_dtype, variant, m, K,
+ *   qmx_tune_bucket(G)), choice (16-row output tiles per block, MT);
+ * - QMX_TUNE_PLANE, the fused plane pass of qmx_apply_local_k: key (dtype,
+ *   nx, ny, nz), choice (0 the specialised kernel, 1 the general kernel on
+ *   256 threads, 2 on 512).
+ * A choice that does not fit the shape is ignored. Setting entries while
+ * launches run on other threads is safe; the launches then see either. */
+enum { QMX_TUNE_DOT = 1, QMX_TUNE_ROTATE = 2, QMX_TUNE_PLANE = 3 };
+int64_t qmx_tune_bucket(int64_t G);
+int64_t qmx_tune_batch_bucket(int64_t batch);
+int qmx_tune_set(int op, const int64_t* key, int n_key, const int64_t* choice, int n_choice);
+int qmx_tune_clear(void);
+int64_t qmx_tune_count(void);
+
 /* Fused steps of the batched Davidson iteration (no counterpart as kernels
  * in the reference, which runs them as tensor expressions):
  * - qmx_normalize_constrain: X[k][b] /= norm[k][b] (0 where norm <= 1e-30),

@@ -91,6 +91,11 @@ SIGNATURES = {
     "qmx_heevx_large": (i32, [vp, i64, i64, i64, vp, vp, i64, i64, vp, vp, i64, i32, vp]),
     "qmx_potrf_inv_work_bytes": (i64, [i64]),
     "qmx_potrf_inv": (i32, [vp, i64, i64, vp, vp, vp, i64, i32, vp]),
+    "qmx_tune_bucket": (i64, [i64]),
+    "qmx_tune_batch_bucket": (i64, [i64]),
+    "qmx_tune_set": (i32, [i32, vp, i32, vp, i32]),
+    "qmx_tune_clear": (i32, []),
+    "qmx_tune_count": (i64, []),
     "qmx_normalize_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
     "qmx_canonical_basis": (i32, [i32, vp, vp, i64, i64, ct.c_double, vp, vp]),
     "qmx_subspace_assemble": (i32, [vp, vp, vp, vp, i64, i64, ct.c_double, ct.c_double, vp, vp,

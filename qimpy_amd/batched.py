@@ -500,6 +500,9 @@ class Bucket:
         dt = Cin.dtype
         V = self.V64 if dt == C128 else self.V
         ke = self.ke if dt == C128 else self.ke_r
+        if gemm.recorder is not None:
+            key = ("plane", DTYPE_CODE[dt]) + tuple(int(x) for x in self.shape)
+            gemm.recorder[key] = gemm.recorder.get(key, 0) + 1
         _lib.call("qmx_apply_local_k", self._plan, k0, nk, DTYPE_CODE[dt], _c(Cin), Cin.stride(1),
                   Cin.stride(0), _c(out), out.stride(1), out.stride(0), _c(V), _c(ke[k0:]),
                   ke.stride(0), B, 0, _c(self.workspace), self.workspace.numel(), rc.stream_ptr())
@@ -847,6 +850,10 @@ class BatchSolver:
                  band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
                  max_k_per_bucket: int = 2048, max_bucket_bytes: float = 16e9, groups: int = 1):
         rc.require_gpu()
+        from . import autotune
+
+        #: tuned launch choices applied from this device's cache (once per process)
+        self.tuned = autotune.load_default()
         self.dtype = dtype
         #: mixed form: stop a job when every wanted band's Kato-Temple error
         #: estimate is below KT_SAFETY * eig_tolerance (Ha); None stops on

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "qmx_common.h"
+#include "qmx_tune.h"
 
 namespace qmx {
 void prof_begin(hipStream_t st, hipEvent_t* a);
@@ -401,8 +402,46 @@ constexpr int kTileTarget = 2048;  // workgroups the 64 x 64 kernel aims for
 // matrix cores (4 real MFMAs against the HBM share of one CU).
 inline int64_t tiling_cost(int64_t tiles16, int64_t rows16) { return 13 * tiles16 + rows16; }
 
-DotPlan dot_plan(int64_t batch, int64_t M, int64_t N, int64_t G) {
+// Chunks of the basis axis: nch requested, rounded to whole slabs.
+void dot_chunks(DotPlan& p, int64_t G, int64_t nch) {
+  nch = std::max<int64_t>(1, std::min<int64_t>(nch, 65535));
+  int64_t len = (G + nch - 1) / nch;
+  len = (len + kSlab - 1) / kSlab * kSlab;
+  len = std::max<int64_t>(len, kSlab);
+  p.chunk_len = len;
+  p.nchunks = (int)std::max<int64_t>(1, (G + len - 1) / len);
+}
+
+// A tuned choice (kernel, bm16, bn16, chunks) for this shape, if one is set
+// and fits it.
+bool dot_tuned(DotPlan& p, int dtype, int64_t batch, int64_t M, int64_t N, int64_t G) {
+  const int64_t key[5] = {dtype, tune_b_bucket(batch), M, N, tune_g_bucket(G)};
+  int64_t ch[4];
+  if (!tune_lookup(QMX_TUNE_DOT, key, 5, ch, 4)) return false;
+  const int64_t M16 = (M + 15) / 16, N16 = (N + 15) / 16;
+  if (ch[0] == 0) {
+    const int64_t bm = ch[1], bn = ch[2];
+    if (bm < 1 || bn < 1 || bm > M16 || bn > N16 || bm + bn > kBlkRows16 || bm * bn > kBlkTiles)
+      return false;
+    p.block = true;
+    p.bm16 = (int)bm;
+    p.bn16 = (int)bn;
+    p.mtiles = (int)((M16 + bm - 1) / bm);
+    p.ntiles = (int)((N16 + bn - 1) / bn);
+  } else if (ch[0] == 1) {
+    p.block = false;
+    p.mtiles = (int)((M + kTile - 1) / kTile);
+    p.ntiles = (int)((N + kTile - 1) / kTile);
+  } else {
+    return false;
+  }
+  dot_chunks(p, G, ch[3]);
+  return true;
+}
+
+DotPlan dot_plan(int dtype, int64_t batch, int64_t M, int64_t N, int64_t G) {
   DotPlan p{};
+  if (dot_tuned(p, dtype, batch, M, N, G)) return p;
   const int64_t M16 = (M + 15) / 16, N16 = (N + 15) / 16;
   int64_t best = -1;
   for (int bm = 1; bm <= std::min<int64_t>(M16, kBlkRows16 - 1); ++bm)
@@ -428,12 +467,7 @@ DotPlan dot_plan(int64_t batch, int64_t M, int64_t N, int64_t G) {
   const int64_t target = p.block ? kBlkTarget : kTileTarget;
   const int64_t want = std::max<int64_t>(1, (target + tiles - 1) / tiles);
   const int64_t most = std::max<int64_t>(1, (G + kMinChunk - 1) / kMinChunk);
-  int64_t nch = std::min<int64_t>(std::min<int64_t>(want, most), 65535);
-  int64_t len = (G + nch - 1) / nch;
-  len = (len + kSlab - 1) / kSlab * kSlab;
-  len = std::max<int64_t>(len, kSlab);
-  p.chunk_len = len;
-  p.nchunks = (int)std::max<int64_t>(1, (G + len - 1) / len);
+  dot_chunks(p, G, std::min<int64_t>(want, most));
   return p;
 }
 
@@ -465,7 +499,7 @@ int dot_impl(const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
              int64_t batch, int64_t M, int64_t N, int64_t G, void* out, int64_t ldc, int64_t sc,
              int accumulate, void* ws, int64_t ws_bytes, hipStream_t st, bool profile) {
   using C2 = typename Cplx<T>::T;
-  const DotPlan p = dot_plan(batch, M, N, G);
+  const DotPlan p = dot_plan(sizeof(T) == 4 ? QMX_C64 : QMX_C128, batch, M, N, G);
   const int64_t need = batch * p.nchunks * M * N * (int64_t)sizeof(C2);
   QMX_CHECK_ARG(ws && ws_bytes >= need, "qmx_dot: workspace too small");
   QMX_CHECK_ARG(batch * p.mtiles * p.ntiles < (1ll << 31), "qmx_dot: too many tiles");
@@ -509,7 +543,7 @@ extern "C" {
 
 int64_t qmx_dot_workspace_bytes(int dtype, int64_t batch, int64_t M, int64_t N, int64_t G) {
   if ((dtype != QMX_C64 && dtype != QMX_C128) || batch < 0 || M < 0 || N < 0 || G < 0) return -1;
-  const DotPlan p = dot_plan(batch, M, N, G);
+  const DotPlan p = dot_plan(dtype, batch, M, N, G);
   return std::max<int64_t>(1, batch * p.nchunks * M * N * (dtype == QMX_C64 ? 8 : 16));
 }
 

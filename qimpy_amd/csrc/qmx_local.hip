@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "qmx_common.h"
+#include "qmx_tune.h"
 #include <algorithm>
 
 #include "qmx_fft_core.h"
@@ -960,7 +961,21 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     Vt = vt;
   }
   const size_t shP = plane_lds_bytes(p, sizeof(C2));
-  const int ntP = plane ? plane_threads(p) : 0;
+  int ntP = plane ? plane_threads(p) : 0;
+  bool fixed_ok = p->local_path != 3;  // the specialised plane kernel where it exists
+  if (plane) {
+    // Tuned plane kernel: 0 specialised, 1 general on 256 threads, 2 on 512.
+    const int64_t key[4] = {sizeof(R) == 4 ? QMX_C64 : QMX_C128, nx, ny, nz};
+    int64_t ch[1];
+    if (tune_lookup(QMX_TUNE_PLANE, key, 4, ch, 1)) {
+      if (ch[0] == 2) {
+        ntP = kPlaneThreads;
+        fixed_ok = false;
+      } else if (ch[0] == 1 && ntP == kPlaneThreadsSmall) {
+        fixed_ok = false;
+      }
+    }
+  }
   if (plane) {
     const int rc = ntP == kPlaneThreadsSmall
                        ? set_plane_lds<C2, R, kPlaneThreadsSmall, kPlaneMaxESmall>(shP)
@@ -991,7 +1006,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
       const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
       const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
       const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
-      if (p->local_path != 3 && ntP == kPlaneThreadsSmall && plane_fixed_available(nx, ny, ncol)) {
+      if (fixed_ok && ntP == kPlaneThreadsSmall && plane_fixed_available(nx, ny, ncol)) {
         const int rc = launch_plane_fixed(sizeof(R) == 8, nx, gP, shP, st, T1, Vt, vslot,
                                           p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
                                           tw_of<C2>(p, 1), ncol, nxa, cur);

@@ -30,6 +30,7 @@
 #include <string>
 
 #include "qmx_common.h"
+#include "qmx_tune.h"
 
 namespace qmx {
 void prof_begin(hipStream_t st, hipEvent_t* a);
@@ -371,7 +372,17 @@ int rotate_dispatch(int variant, int transM, int64_t batch, int m, int K, int64_
   // FP64 accumulators take twice the registers: at most 5 tiles per wave.
   constexpr int kMaxFp = sizeof(T) == 8 ? 5 : 8;
   int blocks = 1;
-  const int mt = variant == kBf16x3 ? pick_mt<6>(m, &blocks) : pick_mt<kMaxFp>(m, &blocks);
+  int mt = variant == kBf16x3 ? pick_mt<6>(m, &blocks) : pick_mt<kMaxFp>(m, &blocks);
+  {
+    const int64_t key[6] = {sizeof(T) == 4 ? QMX_C64 : QMX_C128,
+                            sizeof(MC2) == 8 ? QMX_C64 : QMX_C128, variant, m, K, tune_g_bucket(G)};
+    int64_t ch[1];
+    const int mt_max = variant == kBf16x3 ? 6 : kMaxFp;
+    if (tune_lookup(QMX_TUNE_ROTATE, key, 6, ch, 1) && ch[0] >= 1 && ch[0] <= mt_max) {
+      mt = (int)ch[0];
+      blocks = (m + 16 * mt - 1) / (16 * mt);
+    }
+  }
   const int64_t gtiles = (G + kRotCols - 1) / kRotCols;
   QMX_CHECK_ARG(gtiles < (1ll << 31) && blocks < 65536 && batch < 65536,
                 "qmx_gemm: grid too large");

@@ -23,6 +23,10 @@ from . import _lib, rc
 _DTYPE = {torch.complex64: 0, torch.complex128: 1}
 
 
+#: Shapes launched while a tuning recorder is active (qimpy_amd.autotune).
+recorder: Optional[dict] = None
+
+
 def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
            accumulate: bool = False) -> torch.Tensor:
     """conj(A) @ B^T over the last axis, batched over the leading axes, as
@@ -47,6 +51,9 @@ def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
     if o3.data_ptr() != out.data_ptr() or o3.stride(-1) != 1:
         raise ValueError("dot128: out must be a contiguous (..., M, N) complex128 tensor")
     code = _DTYPE[A.dtype]
+    if recorder is not None:
+        recorder.setdefault(("dot", code, nb, M, N, G), 0)
+        recorder[("dot", code, nb, M, N, G)] += 1
     need = int(_lib.lib().qmx_dot_workspace_bytes(code, nb, M, N, G))
     ws = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
     _lib.call("qmx_dot", code, A3.data_ptr(), A3.stride(1), A3.stride(0), B3.data_ptr(),
@@ -165,6 +172,10 @@ def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
     if Mb is not M:  # a copy changed its strides
         trans, ldm = (1, Mb.stride(-2)) if Mb.stride(-1) == 1 else (0, Mb.stride(-1))
     variant = _variant_c64 if C.dtype == torch.complex64 else 0
+    if recorder is not None:
+        mcode = _DTYPE[Mb.dtype] if Mb.dtype in _DTYPE else 1
+        k = ("rotate", _DTYPE[C.dtype], mcode, variant, nb, Nout, Nin, G)
+        recorder[k] = recorder.get(k, 0) + 1
     _lib.call("qmx_gemm", _DTYPE[C.dtype], _DTYPE[Mb.dtype] if Mb.dtype in _DTYPE else 1, variant,
               trans, nb, Nout, Nin, G, float(alpha), Mb.data_ptr(), ldm, sm, Cb.data_ptr(),
               Cb.stride(-2), sw, float(beta), out.data_ptr(), out.stride(-2), so, rc.stream_ptr())
