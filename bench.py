@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--only", choices=("all", "mixed", "plain"), default="all",
                     help="headline streams to run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
+    ap.add_argument("--bf16x3", action="store_true",
+                    help="complex64 rotations as BF16 x 3 split-precision products")
     ap.add_argument("--no-extra", action="store_true", help="skip the side workloads")
     ap.add_argument("--side-only", action="store_true",
                     help="only the density and projector measurements (no served streams)")
@@ -442,6 +444,8 @@ def main():
 
     rc.init()
     rc.require_gpu()
+    if args.bf16x3:
+        gemm.set_bf16x3(True)
     if args.side_only:
         spec1 = systems.make_cell(1)
         print(json.dumps({"collect_density": density_measurement(spec1),

@@ -108,9 +108,12 @@ __global__ void __launch_bounds__(kRotThreads)
     im[t] = re[t];
   }
   // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk.
-  // Loads are clamped in bounds and masked where used; the next slab's are
-  // issued before this slab's barriers, so they overlap its MFMAs.
+  // Loads are clamped in bounds and masked where used. The next slab's W
+  // and op(M) values are issued before this slab's barriers, so their
+  // latency overlaps this slab's MFMAs; op(M) element u of a thread is
+  // (row i, k kk) with tid + kRotThreads u = kk ROWS + i.
   C2 wn[4];
+  T mrn[MT], min_[MT];
   auto load_w = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -118,7 +121,23 @@ __global__ void __launch_bounds__(kRotThreads)
       wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
     }
   };
+  auto load_m = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      const int e = tid + kRotThreads * u;  // < KS * ROWS
+      const int kk = e / ROWS, i = e - kk * ROWS;
+      const int gi = i0 + i, gk = k0 + kk;
+      mrn[u] = T(0);
+      min_[u] = T(0);
+      if (gi < m && gk < K) {
+        const MC2 v = op_m(Mb, ldm, transM, gi, gk);
+        mrn[u] = (T)v.x;
+        min_[u] = (T)v.y;
+      }
+    }
+  };
   load_w(0);
+  load_m(0);
   for (int k0 = 0; k0 < K; k0 += KS) {
     C2 w[4];
 #pragma unroll
@@ -126,21 +145,23 @@ __global__ void __launch_bounds__(kRotThreads)
       const bool ok = k0 + 4 * q + lk < K;
       w[q] = ok ? wn[q] : C2{T(0), T(0)};
     }
-    if (k0 + KS < K) load_w(k0 + KS);
+    T mr[MT], mi[MT];
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      mr[u] = mrn[u];
+      mi[u] = min_[u];
+    }
+    if (k0 + KS < K) {
+      load_w(k0 + KS);
+      load_m(k0 + KS);
+    }
     __syncthreads();  // the previous slab's LDS reads are done
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
-      const int e = tid + kRotThreads * u;  // < KS * ROWS
+      const int e = tid + kRotThreads * u;
       const int kk = e / ROWS, i = e - kk * ROWS;
-      const int gi = i0 + i, gk = k0 + kk;
-      T vr = T(0), vi = T(0);
-      if (gi < m && gk < K) {
-        const MC2 v = op_m(Mb, ldm, transM, gi, gk);
-        vr = (T)v.x;
-        vi = (T)v.y;
-      }
-      sMr[kk * LDI + i] = vr;
-      sMi[kk * LDI + i] = vi;
+      sMr[kk * LDI + i] = mr[u];
+      sMi[kk * LDI + i] = mi[u];
     }
     __syncthreads();
     // Each of the four real products runs over all tiles before the next,
