@@ -50,6 +50,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+#: Dense MFMA peaks of MI355X (TFLOP/s): FP32 for complex64 data, FP64 for complex128.
+MFMA_PEAK_TFLOPS = {"c64": 157.3, "c128": 78.6}
 RATE = 200.0  # requests per second
 EIG_TOLERANCE = 1e-6  # mixed form: per-band eigenvalue error target (Ha)
 PLAIN_THRESHOLD = 1e-8  # plain form: the reference's |deig| stop
@@ -325,7 +327,7 @@ def projectors_measurement(spec) -> dict:
     torch.cuda.synchronize()
     prof = _lib.profile_read()
     _lib.profile_enable(False)
-    ms, kby, cnt = prof[_lib.QMX_KERNEL_PROJ + _lib.QMX_KERNEL_C128]
+    ms, kby, cnt = prof[_lib.QMX_KERNEL_PROJ + _lib.QMX_KERNEL_C128][:3]
     tk = ms / cnt * 1e-3
     return {"workload": f"64-atom cell, {b.grid.shape[0]}^3 box, {b.nk} k, n_tot {b.n_tot}, "
                         f"{beta.shape[1]} projectors (l <= {max(l_rf)})",
@@ -366,37 +368,68 @@ def chefsi_measurement(spec) -> dict:
 
 
 def roofline(prof, wall_s: float) -> tuple[dict, dict]:
+    """The dominant kernel of the profiled stream (largest total time among
+    the library's kernels) against its roofline: MFMA-bound (TFLOP/s against
+    the dense FP32 or FP64 MFMA peak) when its algorithmic flops per byte
+    exceed the ridge peak_flops / 8 TB/s, else HBM-bound (GB/s of its
+    compulsory bytes). The dominant bandwidth-bound kernel is reported
+    beside it, and the whole local apply under the dense-transform
+    convention."""
     from qimpy_amd import _lib
 
     names = {i: _lib.KERNEL_NAMES.get(i, str(i)) for i in prof}
-    bw = {i: v for i, v in prof.items() if names[i] in BANDWIDTH_KERNELS}
-    kid = max(bw, key=lambda i: bw[i][0])
-    ms, by, cnt = prof[kid]
-    avg_s = ms / cnt * 1e-3
-    achieved = by / cnt / avg_s / 1e9
-    kernels = {
-        names[i]: {"launches": c, "avg_us": round(m / c * 1e3, 2),
-                   "GBps": round(b / c / (m / c * 1e-3) / 1e9, 1),
-                   "share_of_stream": round(m * 1e-3 / wall_s, 3)}
-        for i, (m, b, c) in prof.items() if not names[i].startswith("local_apply")
-    }
+
+    def entry(i):
+        ms, by, cnt, fl = prof[i]
+        avg_s = ms / cnt * 1e-3
+        prec = "c128" if names[i].endswith("<c128>") else "c64"
+        peak_tf = MFMA_PEAK_TFLOPS[prec]
+        ridge = peak_tf * 1e12 / (HBM_PEAK_GBS * 1e9)
+        gbs = by / cnt / avg_s / 1e9
+        e = {"kernel": names[i], "avg_launch_us": round(avg_s * 1e6, 2),
+             "bytes_per_launch": round(by / cnt), "launches": cnt}
+        if fl and fl / by > ridge:
+            tf = fl / cnt / avg_s / 1e12
+            e.update({"bound": "mfma", "achieved": round(tf, 2), "peak": peak_tf,
+                      "unit": "TFLOP/s", "frac": round(tf / peak_tf, 4),
+                      "flops_per_launch": round(fl / cnt), "flop_per_byte": round(fl / by, 1),
+                      "GBps": round(gbs, 1)})
+        else:
+            e.update({"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        e["traffic"] = None
+        traffic = measured_traffic(names[i])
+        if traffic is not None:
+            e["traffic"] = round(traffic["ratio"] * by / cnt)
+            e["traffic_over_compulsory"] = traffic["ratio"]
+            e["traffic_source"] = traffic["source"]
+        return e
+
+    lib_ids = [i for i in prof if not names[i].startswith("local_apply")]
+    kid = max(lib_ids, key=lambda i: prof[i][0])
+    out = entry(kid)
+    bw = [i for i in lib_ids if names[i] in BANDWIDTH_KERNELS and entry(i)["bound"] == "hbm"]
+    if bw:
+        bid = max(bw, key=lambda i: prof[i][0])
+        if bid != kid:
+            out["dominant_bandwidth_kernel"] = entry(bid)
+    kernels = {}
+    for i in lib_ids:
+        ms, by, cnt, fl = prof[i]
+        k = {"launches": cnt, "avg_us": round(ms / cnt * 1e3, 2),
+             "GBps": round(by / cnt / (ms / cnt * 1e-3) / 1e9, 1),
+             "share_of_stream": round(ms * 1e-3 / wall_s, 3)}
+        if fl:
+            k["TFLOPs"] = round(fl / (ms * 1e-3) / 1e12, 2)
+        kernels[names[i]] = k
     apply = [prof[i] for i in prof if names[i].startswith("local_apply")]
     apply_ms = sum(a[0] for a in apply)
-    out = {"kernel": names[kid], "bound": "hbm", "achieved": round(achieved, 1),
-           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-           "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
-           "bytes_per_launch": round(by / cnt)}
     if apply_ms:
         gbs = sum(a[1] for a in apply) / (apply_ms * 1e-3) / 1e9
         out["local_apply_dense_convention"] = {
             "bytes_per_band": "b(2G + 4N) + b_V N / B", "achieved": round(gbs, 1),
             "frac": round(gbs / HBM_PEAK_GBS, 4), "calls": sum(a[2] for a in apply),
             "avg_call_us": round(apply_ms / sum(a[2] for a in apply) * 1e3, 2)}
-    traffic = measured_traffic(names[kid])
-    if traffic is not None:
-        out["traffic"] = round(traffic["ratio"] * by / cnt)
-        out["traffic_over_compulsory"] = traffic["ratio"]
-        out["traffic_source"] = traffic["source"]
     return out, kernels
 
 

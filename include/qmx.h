@@ -494,6 +494,10 @@ enum {
  * count, then clears the records. */
 int qmx_profile_enable(int on);
 int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids);
+/* The same with the summed algorithmic flops per kernel id (8 real flops
+ * per complex multiply-add of qmx_dot and qmx_gemm; 0 for the other
+ * kernels); flops may be NULL. */
+int qmx_profile_read_flops(double* ms, double* bytes, double* flops, int64_t* count, int n_ids);
 
 const char* qmx_last_error(void);
 const char* qmx_version(void);

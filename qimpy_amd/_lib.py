@@ -113,6 +113,7 @@ SIGNATURES = {
                             ct.c_double, vp, i64, i64, i64, vp]),
     "qmx_profile_enable": (i32, [i32]),
     "qmx_profile_read": (i32, [vp, vp, vp, i32]),
+    "qmx_profile_read_flops": (i32, [vp, vp, vp, vp, i32]),
     "qmx_last_error": (ct.c_char_p, []),
     "qmx_version": (ct.c_char_p, []),
 }
@@ -225,9 +226,13 @@ def profile_enable(on: bool) -> None:
 
 
 def profile_read() -> dict:
-    """{kernel id: (total ms, total compulsory bytes, launches)} since the last read."""
+    """{kernel id: (total ms, total compulsory bytes, launches, total
+    algorithmic flops)} since the last read."""
     ms = np.zeros(N_KERNEL_IDS)
     by = np.zeros(N_KERNEL_IDS)
+    fl = np.zeros(N_KERNEL_IDS)
     cnt = np.zeros(N_KERNEL_IDS, dtype=np.int64)
-    call("qmx_profile_read", ms.ctypes.data, by.ctypes.data, cnt.ctypes.data, N_KERNEL_IDS)
-    return {i: (float(ms[i]), float(by[i]), int(cnt[i])) for i in range(N_KERNEL_IDS) if cnt[i]}
+    call("qmx_profile_read_flops", ms.ctypes.data, by.ctypes.data, fl.ctypes.data,
+         cnt.ctypes.data, N_KERNEL_IDS)
+    return {i: (float(ms[i]), float(by[i]), int(cnt[i]), float(fl[i]))
+            for i in range(N_KERNEL_IDS) if cnt[i]}

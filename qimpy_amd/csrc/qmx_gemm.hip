@@ -24,6 +24,7 @@
 namespace qmx {
 void prof_begin(hipStream_t st, hipEvent_t* a);
 void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes);
+void prof_end_flops(hipStream_t st, hipEvent_t a, int id, double bytes, double flops);
 }  // namespace qmx
 
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
@@ -533,7 +534,8 @@ int dot_impl(const void* A, int64_t lda, int64_t sa, const void* B, int64_t ldb,
                      static_cast<const C2*>(ws), batch, (int)M, (int)N, p.nchunks,
                      static_cast<double2*>(out), ldc, sc, accumulate);
   QMX_LAUNCHED();
-  prof_end(st, ev, QMX_KERNEL_GEMM + (sizeof(T) == 8 ? QMX_KERNEL_C128 : 0), (double)batch * (M + N) * G * sizeof(C2));
+  prof_end_flops(st, ev, QMX_KERNEL_GEMM + (sizeof(T) == 8 ? QMX_KERNEL_C128 : 0),
+                 (double)batch * (M + N) * G * sizeof(C2), 8.0 * (double)batch * M * N * G);
   return QMX_OK;
 }
 

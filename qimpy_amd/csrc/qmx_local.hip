@@ -39,7 +39,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 struct ProfRec {
   int id;
   hipEvent_t a, b;
-  double bytes;
+  double bytes, flops;
 };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
@@ -53,12 +53,16 @@ void prof_begin(hipStream_t st, hipEvent_t* a) {
   if (hipEventCreate(a) != hipSuccess || hipEventRecord(*a, st) != hipSuccess) *a = nullptr;
 }
 
-void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes) {
+void prof_end_flops(hipStream_t st, hipEvent_t a, int id, double bytes, double flops) {
   if (!a) return;
   hipEvent_t b = nullptr;
   if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, st) != hipSuccess) return;
   std::lock_guard<std::mutex> lock(g_prof_mu);
-  g_prof.push_back({id, a, b, bytes});
+  g_prof.push_back({id, a, b, bytes, flops});
+}
+
+void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes) {
+  prof_end_flops(st, a, id, bytes, 0.0);
 }
 
 bool factorize(int n, Radices* out) {
@@ -1909,11 +1913,12 @@ int qmx_profile_enable(int on) {
   return QMX_OK;
 }
 
-int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids) {
+int qmx_profile_read_flops(double* ms, double* bytes, double* flops, int64_t* count, int n_ids) {
   QMX_CHECK_ARG(ms && bytes && count && n_ids > 0, "qmx_profile_read: null argument");
   for (int i = 0; i < n_ids; ++i) {
     ms[i] = 0;
     bytes[i] = 0;
+    if (flops) flops[i] = 0;
     count[i] = 0;
   }
   std::vector<ProfRec> recs;
@@ -1935,12 +1940,17 @@ int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids) {
     if (r.id >= 0 && r.id < n_ids) {
       ms[r.id] += t;
       bytes[r.id] += r.bytes;
+      if (flops) flops[r.id] += r.flops;
       count[r.id] += 1;
     }
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
   return rc;
+}
+
+int qmx_profile_read(double* ms, double* bytes, int64_t* count, int n_ids) {
+  return qmx_profile_read_flops(ms, bytes, nullptr, count, n_ids);
 }
 
 }  // extern "C"

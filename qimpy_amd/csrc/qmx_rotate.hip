@@ -34,7 +34,7 @@
 
 namespace qmx {
 void prof_begin(hipStream_t st, hipEvent_t* a);
-void prof_end(hipStream_t st, hipEvent_t a, int id, double bytes);
+void prof_end_flops(hipStream_t st, hipEvent_t a, int id, double bytes, double flops);
 }  // namespace qmx
 
 using namespace qmx;
@@ -441,7 +441,8 @@ int rotate_dispatch(int variant, int transM, int64_t batch, int m, int K, int64_
                     (double)batch * m * K * sizeof(MC2);
   const int id = (variant == kBf16x3 ? QMX_KERNEL_ROTATE_BF16X3 : QMX_KERNEL_ROTATE) +
                  (sizeof(T) == 8 ? QMX_KERNEL_C128 : 0);
-  prof_end(st, ev, id, by);
+  // Algorithmic flops: one complex multiply-add (8 real flops) per (b, i, k, g).
+  prof_end_flops(st, ev, id, by, 8.0 * (double)batch * m * K * G);
   return QMX_OK;
 }
 
