@@ -417,9 +417,11 @@ def roofline(prof, wall_s: float) -> tuple[dict, dict]:
     for i in lib_ids:
         ms, by, cnt, fl = prof[i]
         k = {"launches": cnt, "avg_us": round(ms / cnt * 1e3, 2),
+             "bytes_per_launch": round(by / cnt),
              "GBps": round(by / cnt / (ms / cnt * 1e-3) / 1e9, 1),
              "share_of_stream": round(ms * 1e-3 / wall_s, 3)}
         if fl:
+            k["flops_per_launch"] = round(fl / cnt)
             k["TFLOPs"] = round(fl / (ms * 1e-3) / 1e12, 2)
         kernels[names[i]] = k
     apply = [prof[i] for i in prof if names[i].startswith("local_apply")]
