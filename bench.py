@@ -55,6 +55,8 @@ MFMA_PEAK_TFLOPS = {"c64": 157.3, "c128": 78.6}
 RATE = 200.0  # requests per second
 EIG_TOLERANCE = 1e-6  # mixed form: per-band eigenvalue error target (Ha)
 PLAIN_THRESHOLD = 1e-8  # plain form: the reference's |deig| stop
+#: Band classes of the batched solver (jobs pad to the smallest that holds them).
+BAND_CLASSES = (32, 72)
 CHEFSI_RESIDUAL = 3e-5
 #: Library kernels that are bandwidth (not latency) bound; the roofline
 #: reports the one with the largest total time.
@@ -80,6 +82,8 @@ def parse():
     ap.add_argument("--bf16x3", action="store_true",
                     help="complex64 rotations as BF16 x 3 split-precision products")
     ap.add_argument("--no-extra", action="store_true", help="skip the side workloads")
+    ap.add_argument("--band-classes", default=",".join(map(str, BAND_CLASSES)),
+                    help="comma-separated band classes of the batched solver")
     ap.add_argument("--side-only", action="store_true",
                     help="only the density and projector measurements (no served streams)")
     return ap.parse_args()
@@ -182,7 +186,7 @@ def served(specs, arrivals, dtype, staged=None, profile=False):
 
     jobs = None if staged is None else [j.fresh(a) for j, a in zip(staged, arrivals)]
     solver = BatchSolver(dtype, eig_tolerance=EIG_TOLERANCE, eig_threshold=PLAIN_THRESHOLD,
-                         max_iterations=200, hbm_budget=200e9)
+                         max_iterations=200, hbm_budget=200e9, band_classes=BAND_CLASSES)
     if profile:
         _lib.profile_enable(True)
         _lib.profile_read()
@@ -203,7 +207,7 @@ def batched(specs, dtype):
     from qimpy_amd.batched import BatchSolver, CellJob
 
     solver = BatchSolver(dtype, eig_tolerance=EIG_TOLERANCE, eig_threshold=PLAIN_THRESHOLD,
-                         max_iterations=200)
+                         max_iterations=200, band_classes=BAND_CLASSES)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.submit([CellJob(s, job_id=i) for i, s in enumerate(specs)])
@@ -477,6 +481,8 @@ def main():
 
     from qimpy_amd import autotune, gemm, rc, systems
 
+    global BAND_CLASSES
+    BAND_CLASSES = tuple(int(c) for c in args.band_classes.split(",") if c)
     rc.init()
     rc.require_gpu()
     if args.bf16x3:
@@ -530,6 +536,7 @@ def main():
                                 f"{args.window} completed requests"),
                    "requests": n_req, "rate_per_s": args.rate, "window": args.window,
                    "inputs": "host arrays" if args.host_inputs else "staged in HBM",
+                   "band_classes": list(BAND_CLASSES),
                    "mixed_eig_tolerance_Ha": EIG_TOLERANCE,
                    "plain_eig_threshold": PLAIN_THRESHOLD},
         "latency_s": {"p50": round(wh["p50"], 3), "p99": round(wh["p99"], 3),

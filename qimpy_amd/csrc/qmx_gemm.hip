@@ -8,8 +8,12 @@
 // batch over one chunk with 16x16x4 MFMAs (FP32 for complex64 inputs, FP64
 // for complex128), staging 16-wide slabs of A and B through LDS (double
 // buffered, coalesced 128-byte row segments, split into real and imaginary
-// planes). Complex products take four real MFMAs:
-//   Re += Ar Br + Ai Bi,   Im += Ar Bi - Ai Br.
+// planes). Complex products take three real MFMAs (Gauss's form of
+// conj(a) b):
+//   p1 += Ar Br,  p2 += Ai Bi,  p3 += (Ar - Ai)(Br + Bi);
+//   Re = p1 + p2,  Im = p3 - p1 + p2,
+// which adds one rounding of the operand sums and doubles the error bound
+// of the imaginary part against the four-product form.
 // Chunk partials go to the workspace and a second kernel sums them in FP64
 // into the complex128 result, so complex64 inputs are accumulated in FP32
 // only within a chunk.
@@ -110,13 +114,14 @@ __global__ void __launch_bounds__(kDotThreads)
       sBi[buf][srow * LD + sg + q] = rb[q].y;
     }
   };
-  typename MF::Acc re[2][2], im[2][2];
+  typename MF::Acc p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      re[i][j] = MF::zero();
-      im[i][j] = MF::zero();
+      p1[i][j] = MF::zero();
+      p2[i][j] = MF::zero();
+      p3[i][j] = MF::zero();
     }
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave sub-tile
   const int li = lane & 15, lk = lane >> 4;               // MFMA operand lane layout
@@ -144,14 +149,19 @@ __global__ void __launch_bounds__(kDotThreads)
         br[j] = sBr[buf][r];
         bi[j] = sBi[buf][r];
       }
+      T ad[2], bs[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ad[i] = ar[i] - ai[i];
+        bs[i] = br[i] + bi[i];
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          re[i][j] = MF::mac(ar[i], br[j], re[i][j]);
-          re[i][j] = MF::mac(ai[i], bi[j], re[i][j]);
-          im[i][j] = MF::mac(ar[i], bi[j], im[i][j]);
-          im[i][j] = MF::mac(-ai[i], br[j], im[i][j]);
+          p1[i][j] = MF::mac(ar[i], br[j], p1[i][j]);
+          p2[i][j] = MF::mac(ai[i], bi[j], p2[i][j]);
+          p3[i][j] = MF::mac(ad[i], bs[j], p3[i][j]);
         }
     }
     if (more) {
@@ -169,7 +179,9 @@ __global__ void __launch_bounds__(kDotThreads)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm + 16 * i + MF::row(lk, r), n = n0 + wn + 16 * j + li;
-        if (m < M && n < N) pb[(int64_t)m * N + n] = C2{re[i][j][r], im[i][j][r]};
+        if (m < M && n < N)
+          pb[(int64_t)m * N + n] = C2{p1[i][j][r] + p2[i][j][r],
+                                      p3[i][j][r] - p1[i][j][r] + p2[i][j][r]};
       }
 }
 
@@ -301,15 +313,16 @@ __global__ void __launch_bounds__(kBlkThreads)
   const int ntiles = bm16 * bn16;
   const int li = lane & 15, lk = lane >> 4;
   int offA[kMaxT], offB[kMaxT];
-  typename MF::Acc re[kMaxT], im[kMaxT];
+  typename MF::Acc p1[kMaxT], p2[kMaxT], p3[kMaxT];
 #pragma unroll
   for (int s = 0; s < kMaxT; ++s) {
     const int t = std::min(wave + kWaves * s, ntiles - 1);
     const int ti = t / bn16, tj = t - ti * bn16;
     offA[s] = (16 * ti + li) * LD + 4 * lk;
     offB[s] = (rowsA + 16 * tj + li) * LD + 4 * lk;
-    re[s] = MF::zero();
-    im[s] = MF::zero();
+    p1[s] = MF::zero();
+    p2[s] = MF::zero();
+    p3[s] = MF::zero();
   }
   const bool last_live = wave + kWaves * (kMaxT - 1) < ntiles;  // uniform per wave
   if (g_begin < g_end) load(g_begin);
@@ -331,13 +344,9 @@ __global__ void __launch_bounds__(kBlkThreads)
       if (s + 1 < kMaxT || last_live) {  // a repeated last slot skips its MFMAs
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          re[s] = MF::mac(ar.v[k], br.v[k], re[s]);
-          im[s] = MF::mac(ar.v[k], bi.v[k], im[s]);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          re[s] = MF::mac(ai.v[k], bi.v[k], re[s]);
-          im[s] = MF::mac(-ai.v[k], br.v[k], im[s]);
+          p1[s] = MF::mac(ar.v[k], br.v[k], p1[s]);
+          p2[s] = MF::mac(ai.v[k], bi.v[k], p2[s]);
+          p3[s] = MF::mac(ar.v[k] - ai.v[k], br.v[k] + bi.v[k], p3[s]);
         }
       }
       ar = ar1;
@@ -356,7 +365,8 @@ __global__ void __launch_bounds__(kBlkThreads)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + 16 * ti + MF::row(lk, r), n = n0 + 16 * tj + li;
-        if (m < M && n < N) pb[(int64_t)m * N + n] = C2{re[s][r], im[s][r]};
+        if (m < M && n < N)
+          pb[(int64_t)m * N + n] = C2{p1[s][r] + p2[s][r], p3[s][r] - p1[s][r] + p2[s][r]};
       }
     }
   }
@@ -399,9 +409,9 @@ constexpr int kBlkTarget = 1024;   // workgroups the block kernel aims for
 constexpr int kTileTarget = 2048;  // workgroups the 64 x 64 kernel aims for
 
 // Modelled cost of an output tiling, in units of one 16-row slab load per
-// basis point: each 16 x 16 complex MFMA tile costs about 13 of those on the
-// matrix cores (4 real MFMAs against the HBM share of one CU).
-inline int64_t tiling_cost(int64_t tiles16, int64_t rows16) { return 13 * tiles16 + rows16; }
+// basis point: each 16 x 16 complex MFMA tile costs about 10 of those on the
+// matrix cores (3 real MFMAs against the HBM share of one CU).
+inline int64_t tiling_cost(int64_t tiles16, int64_t rows16) { return 10 * tiles16 + rows16; }
 
 // Chunks of the basis axis: nch requested, rounded to whole slabs.
 void dot_chunks(DotPlan& p, int64_t G, int64_t nch) {

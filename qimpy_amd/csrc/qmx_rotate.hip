@@ -19,8 +19,12 @@
 // arithmetic type allows it.
 //
 // Arithmetic:
-// - k_rotate<float>: FP32 MFMA 16x16x4, four real products per complex one;
-// - k_rotate<double>: FP64 MFMA 16x16x4 (the plain FP64 form);
+// - k_rotate<float>: FP32 MFMA 16x16x4, three real products per complex
+//   one (Gauss: Mr Wr, Mi Wi and (Mr + Mi)(Wr + Wi)), which adds at most one
+//   rounding of the operand sums and doubles the error bound of the
+//   imaginary part against the four-product form;
+// - k_rotate<double>: FP64 MFMA 16x16x4 (the plain FP64 form), the same
+//   three products;
 // - k_rotate_bf16x3: complex64 data as BF16 pairs (hi = RNE(x),
 //   lo = RNE(x - hi)), three BF16 products per real product
 //   (hi hi + hi lo + lo hi) on MFMA 16x16x32 with FP32 accumulation. Each
@@ -101,11 +105,14 @@ __global__ void __launch_bounds__(kRotThreads)
   const MC2* Mb = M + b * sm;
   const C2* Wb = W + b * sw;
   const int64_t gc = g < G ? g : G - 1;  // clamped: loads stay in bounds
-  typename MF::Acc re[MT], im[MT];
+  // Three real products per complex one (Gauss): p1 = Mr Wr, p2 = Mi Wi,
+  // p3 = (Mr + Mi)(Wr + Wi); then Re = p1 - p2, Im = p3 - p1 - p2.
+  typename MF::Acc p1[MT], p2[MT], p3[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) {
-    re[t] = typename MF::Acc{T(0), T(0), T(0), T(0)};
-    im[t] = re[t];
+    p1[t] = typename MF::Acc{T(0), T(0), T(0), T(0)};
+    p2[t] = p1[t];
+    p3[t] = p1[t];
   }
   // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk.
   // Loads are clamped in bounds and masked where used. The next slab's W
@@ -164,7 +171,7 @@ __global__ void __launch_bounds__(kRotThreads)
       sMi[kk * LDI + i] = mi[u];
     }
     __syncthreads();
-    // Each of the four real products runs over all tiles before the next,
+    // Each of the three real products runs over all tiles before the next,
     // so consecutive MFMAs write different accumulators.
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -175,14 +182,13 @@ __global__ void __launch_bounds__(kRotThreads)
         ar[t] = sMr[kr + 16 * t];
         ai[t] = sMi[kr + 16 * t];
       }
+      const T ws = w[q].x + w[q].y;
 #pragma unroll
-      for (int t = 0; t < MT; ++t) re[t] = MF::mac(ar[t], w[q].x, re[t]);
+      for (int t = 0; t < MT; ++t) p1[t] = MF::mac(ar[t], w[q].x, p1[t]);
 #pragma unroll
-      for (int t = 0; t < MT; ++t) im[t] = MF::mac(ar[t], w[q].y, im[t]);
+      for (int t = 0; t < MT; ++t) p2[t] = MF::mac(ai[t], w[q].y, p2[t]);
 #pragma unroll
-      for (int t = 0; t < MT; ++t) re[t] = MF::mac(-ai[t], w[q].y, re[t]);
-#pragma unroll
-      for (int t = 0; t < MT; ++t) im[t] = MF::mac(ai[t], w[q].x, im[t]);
+      for (int t = 0; t < MT; ++t) p3[t] = MF::mac(ar[t] + ai[t], ws, p3[t]);
     }
   }
   if (g >= G) return;
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(kRotThreads)
       const int i = i0 + 16 * t + MF::row(lk, r);
       if (i < m) {
         C2* o = ob + (int64_t)i * ldo + g;
-        C2 v{alpha * re[t][r], alpha * im[t][r]};
+        C2 v{alpha * (p1[t][r] - p2[t][r]), alpha * (p3[t][r] - p1[t][r] - p2[t][r])};
         if (beta != T(0)) {
           const C2 p = *o;
           v.x += beta * p.x;
