@@ -200,7 +200,9 @@ class CellJob:
         if self._dev is None:
             dev = rc.device
             h = {name: t.to(dev, non_blocking=True) for name, t in self._host.items()}
-            Gk = (h["iG"].to(torch.float64) + h["k"][:, None, :]) @ h["GbT"]
+            x = h["iG"].to(torch.float64) + h["k"][:, None, :]
+            GbT = h["GbT"]
+            Gk = x[..., 0:1] * GbT[0] + x[..., 1:2] * GbT[1] + x[..., 2:3] * GbT[2]
             ke = 0.5 * Gk.square().sum(-1)
             ke[torch.arange(self.n_tot, device=dev)[None, :] >= h["n"][:, None]] = 0.0
             self._dev = dict(k=h["k"], iG=h["iG"], ke=ke, V=h["V"], n=h["n"], pos=h["pos"],
@@ -335,7 +337,8 @@ def _lowdin(O: torch.Tensor) -> torch.Tensor:
     no host synchronisation."""
     E = 0.5 * (O + dagger(O))
     E.diagonal(dim1=-2, dim2=-1).sub_(1.0)
-    S = (0.375 * E) @ E - 0.5 * E
+    S = E.clone()
+    gemm.matmul(E, E, out=S, alpha=0.375, beta=-0.5)
     S.diagonal(dim1=-2, dim2=-1).add_(1.0)
     return S
 
@@ -640,11 +643,12 @@ class Bucket:
         _lib.call("qmx_subspace_assemble", _c(XO), _c(XH), _c(self.eig), _c(lam), self.Kb, nb, tol,
                   self.DROP_SHIFT, _c(E), _c(H), rc.stream_ptr())
         # Loewdin factor of O = I + E: I - E/2 + 3 E^2 / 8 (error O(|E|^3)).
-        S = torch.baddbmm(E, E, E, beta=-0.5, alpha=0.375)
+        S = E.clone()
+        gemm.matmul(E, E, out=S, alpha=0.375, beta=-0.5)
         S.diagonal(dim1=-2, dim2=-1).add_(1.0)
         # The eigensolver reads the lower triangle of S H S.
-        E, W2 = yield S @ H @ S, nb  # only the nb lowest eigenvectors
-        V = S @ W2[:, :, :nb]  # (Kb, 2 nb, nb) complex128
+        E, W2 = yield gemm.matmul(gemm.matmul(S, H), S), nb  # only the nb lowest eigenvectors
+        V = gemm.matmul(S, W2[:, :, :nb])  # (Kb, 2 nb, nb) complex128
         gemm.rotate(V, self.X, out=self.X2[:, :nb])
         gemm.rotate(V, self.HX, out=self.HX2[:, :nb])
         self.X, self.X2 = self.X2, self.X
@@ -682,9 +686,9 @@ class Bucket:
         else:
             C, HC = self.X[rows, :nb], self.HX[rows, :nb]
         S = _lowdin(_dot(C, C))  # C is orthonormal to float32 rounding or better
-        Hh = S @ _dot(C, HC) @ S
+        Hh = gemm.matmul(gemm.matmul(S, _dot(C, HC)), S)
         E, W = yield 0.5 * (Hh + dagger(Hh))
-        Vs = S @ W
+        Vs = gemm.matmul(S, W)
         C = gemm.rotate(Vs, C)
         HC = gemm.rotate(Vs, HC)
         rnorm = torch.linalg.vector_norm(HC - C * E[:, :, None].to(C128), dim=-1)

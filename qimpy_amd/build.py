@@ -18,7 +18,8 @@ LIB = LIBDIR / "libqmx.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["qmx_local.hip", "qmx_ops.hip", "qmx_eig.hip", "qmx_heevx.hip", "qmx_subspace.hip", "qmx_gemm.hip",
-           "qmx_projectors.hip", "qmx_rotate.hip", "qmx_plane_fixed.hip", "qmx_tune.hip"]
+           "qmx_projectors.hip", "qmx_rotate.hip", "qmx_plane_fixed.hip", "qmx_cols_fixed.hip",
+           "qmx_tune.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

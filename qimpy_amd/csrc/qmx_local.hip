@@ -125,6 +125,9 @@ struct qmx_plan {
   int* d_n;           // [nk]
   int* d_vslot;       // [nk] potential box of each k
   int* d_col_ix;      // [ncol] x of each column
+  int* d_zrange;      // [nk][ncol] z interval of each column's points, packed (lo, hi)
+  bool col_interval;  // every column's points form one cyclic z interval
+  int col_run_max;    // most points of kColFixedThreads consecutive columns of one k
   int local_path;     // 0 automatic, 1 separate passes B, C, D
   int64_t* h_nsum;    // [nk + 1] prefix sums of n[k] (host)
   int n_vslots;       // 1 + largest slot
@@ -987,13 +990,45 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     if (rc) return rc;
   }
   const FastDiv fny = make_fastdiv(ny), fnxa = make_fastdiv(nxa);
+  // The column passes with the z length fixed at compile time, where they
+  // exist (path 3 forces the general kernels everywhere).
+  const bool cols_fixed = plane && p->col_interval && p->local_path != 3 &&
+                          cols_fixed_available(nz, sizeof(R) == 8);
+  ColFixedArgs fa{};
+  if (cols_fixed) {
+    fa.st = st;
+    fa.C = C;
+    fa.ldb = ldb;
+    fa.ldk = ldk;
+    fa.out = out;
+    fa.oldb = oldb;
+    fa.oldk = oldk;
+    fa.ke = ke;
+    fa.ke_ldk = ke_ldk;
+    fa.cstart = cstart;
+    fa.zrange = p->d_zrange + (int64_t)k0 * ncol;
+    fa.nvec = nvec;
+    fa.tw = tw_of<C2>(p, 2);
+    fa.ncol = ncol;
+    fa.n_tot = p->n_tot;
+    fa.run_max = p->col_run_max;
+    fa.scale = (double)scale;
+    fa.accumulate = accumulate;
+  }
   for (int64_t b0 = 0; b0 < nb; b0 += blk) {
     const int cur = (int)std::min<int64_t>(blk, nb - b0);
     C2* T1 = reinterpret_cast<C2*>(static_cast<char*>(ws) + fixed);
     C2* T2 = T1 + (int64_t)nk * cur * ncol * nz;
     dim3 gA((ncol + Mz - 1) / Mz, cur, nk);
+    fa.grid = dim3((ncol + kColFixedThreads - 1) / kColFixedThreads, cur, nk);
+    fa.T1 = T1;
+    fa.nb = cur;
+    fa.b0 = (int)b0;
     prof_begin(st, &ev);
-    if (plane)
+    if (cols_fixed) {
+      const int rc = launch_cols_fixed(sizeof(R) == 8, false, nz, fa);
+      if (rc) return rc;
+    } else if (plane)
       hipLaunchKernelGGL((k_sphere_to_cols<C2, R, true>), gA, dim3(kThreads), shA, st, C, ldb,
                          ldk, T1, t1idx, cstart, tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur,
                          p->n_tot, (int)b0);
@@ -1059,7 +1094,10 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
     prof_end(st, ev, idoff + QMX_KERNEL_PASS_D, cur * (t1_elems + t2_elems) * bc);
     }
     prof_begin(st, &ev);
-    if (plane)
+    if (cols_fixed) {
+      const int rc = launch_cols_fixed(sizeof(R) == 8, true, nz, fa);
+      if (rc) return rc;
+    } else if (plane)
       hipLaunchKernelGGL((k_cols_to_sphere<C2, R, true>), gA, dim3(kThreads), shA, st, T1, C,
                          ldb, ldk, out, oldb, oldk, ke, ke_ldk, t1idx, cstart, nvec,
                          tw_of<C2>(p, 2), p->rad[2], fz, ncol, cur, p->n_tot, (int)b0, scale,
@@ -1570,7 +1608,51 @@ namespace {
 // Host-side index tables of a basis (no device calls).
 struct GeomTables {
   std::vector<int> x_list, pcol_start, col_iy, cstart, t1idx;
+  std::vector<int> zrange;  // [nk][ncol] packed (lo, hi), see column_intervals
+  bool col_interval = true;
+  int col_run_max = 0;
 };
+
+// The z interval of each column's points, for the fixed column passes: a
+// sphere meets a line of the box in one interval of signed z, stored in
+// increasing z as z = lo..hi (lo >= 0) or, when it contains z = 0 and
+// negative z, as 0..hi then n + lo..n - 1 (lo < 0). Packed as
+// (lo & 0xffff) | hi << 16; an empty column is (1, 0). col_interval turns
+// false if any column's points are not one such interval.
+void column_intervals(int nk, int ncol, int n_tot, int n2, GeomTables* t) {
+  t->zrange.assign((size_t)nk * ncol, 1);
+  t->col_interval = true;
+  t->col_run_max = 0;
+  for (int k = 0; k < nk; ++k) {
+    const int* cs = t->cstart.data() + (size_t)k * (ncol + 1);
+    const int* t1 = t->t1idx.data() + (size_t)k * n_tot;
+    for (int c = 0; c < ncol; ++c) {
+      const int a = cs[c], b = cs[c + 1], cnt = b - a, base = c * n2;
+      int lo = 1, hi = 0;
+      if (cnt > 0) {
+        const int zf = t1[a] - base, zl = t1[b - 1] - base;
+        if (zl - zf + 1 == cnt) {
+          lo = zf;
+          hi = zl;
+        } else {
+          int h = 0;  // points 0..h, then l..n2 - 1
+          while (h + 1 < cnt && t1[a + h + 1] - base == h + 1) ++h;
+          const int l = h + 1 < cnt ? t1[a + h + 1] - base : n2;
+          if (zf == 0 && zl == n2 - 1 && cnt - (h + 1) == n2 - l) {
+            lo = l - n2;
+            hi = h;
+          } else {
+            t->col_interval = false;
+          }
+        }
+      }
+      t->zrange[(size_t)k * ncol + c] = (lo & 0xFFFF) | (hi << 16);
+    }
+    for (int c0 = 0; c0 < ncol; c0 += kColFixedThreads)
+      t->col_run_max =
+          std::max(t->col_run_max, cs[std::min(ncol, c0 + kColFixedThreads)] - cs[c0]);
+  }
+}
 
 int build_tables(const qmx_geom* geom, GeomTables* t) {
   QMX_CHECK_ARG(geom, "qmx_geom_tables: null geometry");
@@ -1651,6 +1733,7 @@ int build_tables(const qmx_geom* geom, GeomTables* t) {
     cs[0] = 0;
     for (int cc = 0; cc < ncol; ++cc) cs[cc + 1] = cs[cc] + counts[cc];
   }
+  column_intervals(nk, ncol, n_tot, n2, t);
   return QMX_OK;
 }
 
@@ -1717,10 +1800,12 @@ int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
   std::vector<int> col_ix(t.col_iy.size());
   for (size_t xa = 0; xa < t.x_list.size(); ++xa)
     for (int c = t.pcol_start[xa]; c < t.pcol_start[xa + 1]; ++c) col_ix[c] = t.x_list[xa];
+  p->col_interval = t.col_interval;
+  p->col_run_max = t.col_run_max;
   if ((rc = upload_tables(p, {&t.x_list, &t.pcol_start, &t.col_iy, &t.cstart, &t.t1idx, &nvec,
-                              &vslot, &col_ix},
+                              &vslot, &col_ix, &t.zrange},
                           {&p->d_x_list, &p->d_pcol_start, &p->d_col_iy, &p->d_cstart,
-                           &p->d_t1idx, &p->d_n, &p->d_vslot, &p->d_col_ix}))) {
+                           &p->d_t1idx, &p->d_n, &p->d_vslot, &p->d_col_ix, &p->d_zrange}))) {
     free_plan(p);
     return rc;
   }

@@ -180,3 +180,14 @@ def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
               trans, nb, Nout, Nin, G, float(alpha), Mb.data_ptr(), ldm, sm, Cb.data_ptr(),
               Cb.stride(-2), sw, float(beta), out.data_ptr(), out.stride(-2), so, rc.stream_ptr())
     return out
+
+
+def matmul(A: torch.Tensor, B: torch.Tensor, out: Optional[torch.Tensor] = None,
+           alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+    """out = alpha A @ B + beta out for small batched complex matrices (the
+    subspace products of a Davidson step: S H S, S W, Loewdin factors) on
+    the rotation kernel (qmx_gemm): A (..., m, k) of any strides with one
+    unit-stride axis, B (..., k, n) with unit stride along n, out (..., m, n)
+    of B's type. A @ B is rotate's (A^T)^T @ B."""
+    return rotate(A.transpose(-2, -1), B, out=out, alpha=alpha, beta=beta)
+

@@ -40,7 +40,9 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
         return torch.zeros((K, 0, G), dtype=torch.complex128, device=dev)
     iGk = iG.to(torch.float64) + k[:, None, :]
     Gb = torch.from_numpy(np.ascontiguousarray(np.asarray(Gbasis, dtype=np.float64).T))
-    Gk = iGk @ (Gb.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else Gb)
+    Gb = Gb.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else Gb
+    # Three-term sums rather than matmul: no library GEMM for a 3-long axis.
+    Gk = iGk[..., 0:1] * Gb[0] + iGk[..., 1:2] * Gb[1] + iGk[..., 2:3] * Gb[2]
     Gmag2 = Gk.square().sum(-1)
     Y = []
     for l in proj_l:
@@ -55,7 +57,9 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
                   * Gmag2[..., None])  # (K, G, n_rc)
     # exp(-2 pi i (iG + k) . tau) / sqrt(V), atoms last (_ions.py:304-311):
     phase = torch.polar(torch.full((), 1.0 / np.sqrt(volume), device=dev, dtype=torch.float64),
-                        (-2 * np.pi) * (iGk @ positions.T))  # (K, G, A)
+                        (-2 * np.pi) * (iGk[..., None, 0] * positions[:, 0]
+                                        + iGk[..., None, 1] * positions[:, 1]
+                                        + iGk[..., None, 2] * positions[:, 2]))  # (K, G, A)
     proj = phase[..., :, None, None] * f[..., None, :, None] * Y[..., None, None, :]
     proj = proj.reshape(K, G, -1).transpose(1, 2).contiguous()  # (K, P, G)
     pad = torch.arange(G, device=dev)[None, None, :] >= n.to(dev)[:, None, None]
