@@ -432,7 +432,11 @@ class Bucket:
         Kb = sum(j.K for j in jobs)
         Kkeep = sum(j.K for j in keep)
         old_X, old_HX = getattr(self, "X", None), getattr(self, "HX", None)
-        if not self._buffers(Kb, max(j.n_tot for j in jobs), max(j.P for j in jobs)):
+        # Rows of the projector, coupling and kinetic-energy buffers that a
+        # job already holds at its new offset (same buffers, same place).
+        old_at = {id(j): a for j, a in zip(getattr(self, "jobs", []), getattr(self, "k0", []))}
+        realloc = self._buffers(Kb, max(j.n_tot for j in jobs), max(j.P for j in jobs))
+        if not realloc:
             # same buffers: the new state goes into the spare pair, which
             # becomes current ([X, HX, X2, HX2] bases)
             f = self._full
@@ -456,18 +460,23 @@ class Bucket:
         ke = self._ke_full[:Kb]
         beta = self._beta_full[:Kb]
         D = self._D_full[:Kb]
-        ke.zero_()
-        beta.zero_()
-        D.zero_()
         V64 = torch.empty((len(jobs), self.N), dtype=torch.float64, device=dev)
         fi = np.zeros((Kb, Gb), dtype=np.int64)
         v_slot = np.zeros(Kb, dtype=np.int32)
         for i, (j, a) in enumerate(zip(jobs, k0)):
             t = j.device_tables()
-            ke[a : a + j.K, : j.n_tot] = t["ke"]
-            if j.P:
-                beta[a : a + j.K, : j.P, : j.n_tot] = j.projectors(j.n_tot)
-                D[a : a + j.K, : j.P] = t["D"].to(self.rdtype)
+            if realloc or old_at.get(id(j)) != a:
+                # The job's rows with their padding zeroed (entries past its
+                # basis and projectors), written only where they moved.
+                rows = slice(a, a + j.K)
+                ke[rows, : j.n_tot] = t["ke"]
+                ke[rows, j.n_tot :] = 0.0
+                if j.P:
+                    beta[rows, : j.P, : j.n_tot] = j.projectors(j.n_tot)
+                    beta[rows, : j.P, j.n_tot :] = 0.0
+                    D[rows, : j.P] = t["D"].to(self.rdtype)
+                beta[rows, j.P :] = 0.0
+                D[rows, j.P :] = 0.0
             V64[i] = t["V"]
             fi[a : a + j.K, : j.n_tot] = j.fft_index
             v_slot[a : a + j.K] = i

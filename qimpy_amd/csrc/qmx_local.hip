@@ -44,19 +44,60 @@ struct ProfRec {
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
+// Resolved events go back to a free list per device (creating an event is a
+// driver call far slower than recording one), so a profiled stream adds
+// little host time per launch.
+static std::vector<std::pair<int, hipEvent_t>> g_prof_free;
 
 bool prof_on() { return g_prof_on; }
+
+static hipEvent_t prof_event() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    for (size_t i = g_prof_free.size(); i-- > 0;) {
+      if (g_prof_free[i].first == dev) {
+        hipEvent_t e = g_prof_free[i].second;
+        g_prof_free[i] = g_prof_free.back();
+        g_prof_free.pop_back();
+        return e;
+      }
+    }
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+static void prof_release(hipEvent_t e) {
+  int dev = 0;
+  if (!e) return;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipEventDestroy(e);
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof_free.push_back({dev, e});
+}
 
 void prof_begin(hipStream_t st, hipEvent_t* a) {
   *a = nullptr;
   if (!g_prof_on) return;
-  if (hipEventCreate(a) != hipSuccess || hipEventRecord(*a, st) != hipSuccess) *a = nullptr;
+  *a = prof_event();
+  if (*a && hipEventRecord(*a, st) != hipSuccess) {
+    prof_release(*a);
+    *a = nullptr;
+  }
 }
 
 void prof_end_flops(hipStream_t st, hipEvent_t a, int id, double bytes, double flops) {
   if (!a) return;
-  hipEvent_t b = nullptr;
-  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, st) != hipSuccess) return;
+  hipEvent_t b = prof_event();
+  if (!b || hipEventRecord(b, st) != hipSuccess) {
+    prof_release(a);
+    prof_release(b);
+    return;
+  }
   std::lock_guard<std::mutex> lock(g_prof_mu);
   g_prof.push_back({id, a, b, bytes, flops});
 }
@@ -2028,8 +2069,8 @@ int qmx_profile_read_flops(double* ms, double* bytes, double* flops, int64_t* co
       if (flops) flops[r.id] += r.flops;
       count[r.id] += 1;
     }
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+    prof_release(r.a);
+    prof_release(r.b);
   }
   return rc;
 }

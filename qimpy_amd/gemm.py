@@ -35,7 +35,7 @@ def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
     flattens to a single batch axis)."""
     if A.dtype != B.dtype or A.dtype not in _DTYPE:
         raise TypeError(f"dot128: complex64 or complex128 operands of one type, not {A.dtype}/{B.dtype}")
-    batch = torch.broadcast_shapes(A.shape[:-2], B.shape[:-2])
+    batch = _bshape(A.shape[:-2], B.shape[:-2])
     M, G = A.shape[-2:]
     N = B.shape[-2]
     if B.shape[-1] != G:
@@ -60,6 +60,14 @@ def dot128(A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None,
               B3.stride(1), B3.stride(0), nb, M, N, G, o3.data_ptr(), o3.stride(1), o3.stride(0),
               int(accumulate), ws.data_ptr(), need, rc.stream_ptr())
     return out
+
+
+def _bshape(a: torch.Size, b: torch.Size) -> torch.Size:
+    """Broadcast of two leading shapes (the common equal case without
+    torch.broadcast_shapes, which costs tens of microseconds per call)."""
+    if a == b:
+        return a
+    return torch.broadcast_shapes(a, b)
 
 
 def _resolved(X: torch.Tensor) -> torch.Tensor:
@@ -144,7 +152,7 @@ def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
     if C.shape[-2] != Nin:
         raise ValueError(f"rotate: M {tuple(M.shape)} does not match C {tuple(C.shape)}")
     G = C.shape[-1]
-    batch = torch.broadcast_shapes(M.shape[:-2], C.shape[:-2])
+    batch = _bshape(M.shape[:-2], C.shape[:-2])
     if out is None:
         out = torch.empty(batch + (Nout, G), dtype=C.dtype, device=C.device)
         beta = 0.0

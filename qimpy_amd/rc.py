@@ -52,9 +52,19 @@ def require_gpu() -> torch.device:
     return device
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_get_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream_ptr(stream: Optional[torch.cuda.Stream] = None) -> int:
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return int(s.cuda_stream)
+    """The hipStream_t of `stream`, by default torch's current stream on the
+    current device (read without building a Stream object: every library
+    call asks for it)."""
+    if stream is not None:
+        return int(stream.cuda_stream)
+    if _raw_stream is not None and _get_device is not None:
+        return int(_raw_stream(_get_device()))
+    return int(torch.cuda.current_stream().cuda_stream)
 
 
 def compute_stream_wait_current() -> None:
