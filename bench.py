@@ -57,6 +57,8 @@ EIG_TOLERANCE = 1e-6  # mixed form: per-band eigenvalue error target (Ha)
 PLAIN_THRESHOLD = 1e-8  # plain form: the reference's |deig| stop
 #: Band classes of the batched solver (jobs pad to the smallest that holds them).
 BAND_CLASSES = (32, 72)
+#: Bucket groups the served solver deals its buckets to (served in turn).
+GROUPS = 2
 CHEFSI_RESIDUAL = 3e-5
 #: Library kernels that are bandwidth (not latency) bound; the roofline
 #: reports the one with the largest total time.
@@ -84,6 +86,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the side workloads")
     ap.add_argument("--band-classes", default=",".join(map(str, BAND_CLASSES)),
                     help="comma-separated band classes of the batched solver")
+    ap.add_argument("--groups", type=int, default=GROUPS,
+                    help="bucket groups of the served solver (served in turn)")
     ap.add_argument("--side-only", action="store_true",
                     help="only the density and projector measurements (no served streams)")
     return ap.parse_args()
@@ -186,7 +190,8 @@ def served(specs, arrivals, dtype, staged=None, profile=False):
 
     jobs = None if staged is None else [j.fresh(a) for j, a in zip(staged, arrivals)]
     solver = BatchSolver(dtype, eig_tolerance=EIG_TOLERANCE, eig_threshold=PLAIN_THRESHOLD,
-                         max_iterations=200, hbm_budget=200e9, band_classes=BAND_CLASSES)
+                         max_iterations=200, hbm_budget=200e9, band_classes=BAND_CLASSES,
+                         groups=GROUPS)
     if profile:
         _lib.profile_enable(True)
         _lib.profile_read()
@@ -481,8 +486,9 @@ def main():
 
     from qimpy_amd import autotune, gemm, rc, systems
 
-    global BAND_CLASSES
+    global BAND_CLASSES, GROUPS
     BAND_CLASSES = tuple(int(c) for c in args.band_classes.split(",") if c)
+    GROUPS = args.groups
     rc.init()
     rc.require_gpu()
     if args.bf16x3:
@@ -536,7 +542,7 @@ def main():
                                 f"{args.window} completed requests"),
                    "requests": n_req, "rate_per_s": args.rate, "window": args.window,
                    "inputs": "host arrays" if args.host_inputs else "staged in HBM",
-                   "band_classes": list(BAND_CLASSES),
+                   "band_classes": list(BAND_CLASSES), "groups": GROUPS,
                    "mixed_eig_tolerance_Ha": EIG_TOLERANCE,
                    "plain_eig_threshold": PLAIN_THRESHOLD},
         "latency_s": {"p50": round(wh["p50"], 3), "p99": round(wh["p99"], 3),

@@ -42,10 +42,11 @@ def _solver(dtype, **kw):
     return BatchSolver(dtype, **args)
 
 
+@pytest.mark.parametrize("groups", [1, 2])
 @pytest.mark.parametrize("dtype", [C128, C64], ids=["plain", "mixed"])
-def test_batched_cells_match_dense(gpu, dense_refs, dtype):
+def test_batched_cells_match_dense(gpu, dense_refs, dtype, groups):
     specs = _specs()
-    solver = _solver(dtype)
+    solver = _solver(dtype, groups=groups)
     solver.submit([CellJob(s, job_id=i) for i, s in enumerate(specs)])
     results = solver.run()
     assert sorted(results) == list(range(len(specs)))
