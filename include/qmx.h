@@ -441,7 +441,11 @@ int qmx_dot_noalloc(int dtype, const void* A, int64_t lda, int64_t sa, const voi
  * FP64 MFMA for complex128; variant 1 (complex64 only): BF16 x 3 split
  * precision (each FP32 operand as a BF16 hi + lo pair, three BF16 products
  * per real product, FP32 accumulation), relative error ~2^-16 per product.
- * out must not overlap W or M. Replaces _matmul (the C @ M of
+ * out must not overlap M; it may be W itself (out == W with ldo == ldw and
+ * so == sw: an in-place rotation), which takes one block of output rows, so
+ * m <= 128 (complex64, variant 0), 96 (variant 1) or 80 (complex128); it must
+ * not overlap W otherwise.
+ * Replaces _matmul (the C @ M of
  * Wavefunction, dft/electrons/_wavefunction_dot.py:138-191), the nonlocal
  * back-projection beta (D beta^H C) (_hamiltonian.py:21-24) and the
  * Davidson rotations (_davidson.py:212-215). */

@@ -409,7 +409,8 @@ class Bucket:
         self._cap = (kcap, g_need, p_need)
         self._full = [torch.zeros((kcap, 2 * nb, g_need), dtype=dt, device=dev) for _ in range(4)]
         self._beta_full = torch.zeros((kcap, p_need, g_need), dtype=dt, device=dev)
-        self._D_full = torch.zeros((kcap, p_need), dtype=self.rdtype, device=dev)
+        # FP64 couplings: they scale the complex128 projections directly.
+        self._D_full = torch.zeros((kcap, p_need), dtype=torch.float64, device=dev)
         self._ke_full = torch.zeros((kcap, g_need), dtype=torch.float64, device=dev)
         return True
 
@@ -446,13 +447,22 @@ class Bucket:
         eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
         deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
         if keep:
-            where = {id(j): a for j, a in zip(self.jobs, self.k0)}
-            idx = torch.cat([torch.arange(where[id(j)], where[id(j)] + j.K, device=dev) for j in keep])
+            # Kept jobs in runs of consecutive old rows, one copy per run.
+            runs = []  # (new row, old row, rows)
+            a_new = 0
+            for j in keep:
+                a_old = old_at[id(j)]
+                if runs and runs[-1][1] + runs[-1][2] == a_old:
+                    runs[-1][2] += j.K
+                else:
+                    runs.append([a_new, a_old, j.K])
+                a_new += j.K
             g = old_X.shape[-1]
-            Xn[:Kkeep, :nb, :g] = old_X[idx, :nb]
-            HXn[:Kkeep, :nb, :g] = old_HX[idx, :nb]
-            eig[:Kkeep] = self.eig.index_select(0, idx)
-            deig[:Kkeep] = self.deig.index_select(0, idx)
+            for a, o, n_rows in runs:
+                Xn[a : a + n_rows, :nb, :g] = old_X[o : o + n_rows, :nb]
+                HXn[a : a + n_rows, :nb, :g] = old_HX[o : o + n_rows, :nb]
+                eig[a : a + n_rows] = self.eig[o : o + n_rows]
+                deig[a : a + n_rows] = self.deig[o : o + n_rows]
         k0, acc = [], 0
         for j in jobs:
             k0.append(acc)
@@ -474,7 +484,7 @@ class Bucket:
                 if j.P:
                     beta[rows, : j.P, : j.n_tot] = j.projectors(j.n_tot)
                     beta[rows, : j.P, j.n_tot :] = 0.0
-                    D[rows, : j.P] = t["D"].to(self.rdtype)
+                    D[rows, : j.P] = t["D"]
                 beta[rows, j.P :] = 0.0
                 D[rows, j.P :] = 0.0
             V64[i] = t["V"]
@@ -522,7 +532,7 @@ class Bucket:
             if beta is None:
                 beta, D = self.beta[k0 : k0 + nk], self.D[k0 : k0 + nk]
             proj = _dot(beta, Cin)  # (nk, P, B), complex128
-            proj *= D[:, :, None].to(proj.real.dtype)
+            proj *= D[:, :, None] if D.dtype == torch.float64 else D[:, :, None].double()
             gemm.rotate(proj, beta, out=out, beta=1.0)  # out += beta^T (D P)
 
     # ------------------------------------------------------------ job start
@@ -583,27 +593,32 @@ class Bucket:
         self.rnorm = rnorm2.sqrt()
 
     def pack(self) -> torch.Tensor:
-        """Convergence data for the host read: [rnorm, deig, number of
-        unconverged QL eigensolves since the last read]."""
-        fail = torch.zeros(2, dtype=torch.float64, device=rc.device)
-        for info in self._info:
-            fail[0] += (info != 0).sum()
-        self._info = []
-        fail[1] = (~torch.isfinite(self.eig)).sum() + (~torch.isfinite(self.rnorm)).sum()
+        """Convergence data for the host read: [rnorm, deig, eig, number of
+        unconverged QL eigensolves since the last read] in one launch (the
+        host checks the values for non-finite entries itself)."""
+        if self._info:
+            fail = torch.stack([(i != 0).sum() for i in self._info]).sum().reshape(1).to(
+                torch.float64)
+            self._info = []
+        else:
+            fail = getattr(self, "_zero1", None)
+            if fail is None or fail.device != self.eig.device:
+                fail = self._zero1 = torch.zeros(1, dtype=torch.float64, device=self.eig.device)
         return torch.cat([self.rnorm.flatten(), self.deig.flatten(), self.eig.flatten(), fail])
 
     def converged_jobs(self, packed: np.ndarray) -> list[CellJob]:
         nb = self.nb
-        if packed[-2] != 0:
-            raise RuntimeError(f"batched QL eigensolver did not converge ({int(packed[-2])} matrices)"
-                               f" in bucket {self.shape}, {nb} bands")
         if packed[-1] != 0:
-            raise RuntimeError(f"non-finite eigenvalues or residuals ({int(packed[-1])}) in bucket"
-                               f" {self.shape}, {nb} bands")
+            raise RuntimeError(f"batched QL eigensolver did not converge ({int(packed[-1])} matrices)"
+                               f" in bucket {self.shape}, {nb} bands")
         n = self.Kb * nb
         rnorm = packed[:n].reshape(self.Kb, nb)
         deig = packed[n : 2 * n].reshape(self.Kb, nb)
         eig = packed[2 * n : 3 * n].reshape(self.Kb, nb)
+        bad = int((~np.isfinite(eig)).sum() + (~np.isfinite(rnorm)).sum())
+        if bad:
+            raise RuntimeError(f"non-finite eigenvalues or residuals ({bad}) in bucket"
+                               f" {self.shape}, {nb} bands")
         done = []
         for j, a in zip(self.jobs, self.k0):
             rows = slice(a, a + j.K)
@@ -642,7 +657,10 @@ class Bucket:
         Ut = torch.empty((self.Kb, nb, nb), dtype=dt, device=rc.device)
         _lib.call("qmx_canonical_basis", DTYPE_CODE[dt], _c(W), _c(lam), self.Kb, nb, tol, _c(Ut),
                   rc.stream_ptr())
-        R.copy_(gemm.rotate(Ut.transpose(-2, -1), R))  # Ut @ R
+        if nb <= gemm.inplace_rows_max(dt):
+            gemm.rotate(Ut.transpose(-2, -1), R, out=R)  # Ut @ R in place
+        else:
+            R.copy_(gemm.rotate(Ut.transpose(-2, -1), R))
         self.hamiltonian(R, HR, 0, self.Kb)
         XO = _dot(self.X, R)  # (Kb, 2 nb, nb): [C^H R; R^H R]
         XH = _dot(self.X, HR)

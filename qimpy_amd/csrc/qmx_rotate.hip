@@ -391,6 +391,9 @@ void launch_bf(dim3 grid, hipStream_t st, const void* M, int64_t ldm, int64_t sm
                      (float)beta);
 }
 
+// In place (out == W): a workgroup reads every row of W at its columns
+// before it writes any output row there, and no other workgroup touches
+// those columns, provided the output rows form one block (grid.y == 1).
 template <typename T, typename MC2>
 int rotate_dispatch(int variant, int transM, int64_t batch, int m, int K, int64_t G,
                     double alpha, const void* M, int64_t ldm, int64_t sm, const void* W,
@@ -398,9 +401,13 @@ int rotate_dispatch(int variant, int transM, int64_t batch, int m, int K, int64_
                     hipStream_t st) {
   // FP64 accumulators take twice the registers: at most 5 tiles per wave.
   constexpr int kMaxFp = sizeof(T) == 8 ? 5 : 8;
+  const bool inplace = out == W;
   int blocks = 1;
   int mt = variant == kBf16x3 ? pick_mt<6>(m, &blocks) : pick_mt<kMaxFp>(m, &blocks);
-  {
+  if (inplace) {
+    QMX_CHECK_ARG(ldo == ldw && so == sw, "qmx_gemm: in place needs out and W with one layout");
+    QMX_CHECK_ARG(blocks == 1, "qmx_gemm: in place needs m <= 16 x the tile limit");
+  } else {
     const int64_t key[6] = {sizeof(T) == 4 ? QMX_C64 : QMX_C128,
                             sizeof(MC2) == 8 ? QMX_C64 : QMX_C128, variant, m, K, tune_g_bucket(G)};
     int64_t ch[1];

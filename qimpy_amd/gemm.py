@@ -135,6 +135,14 @@ def _batch_view(X: torch.Tensor, batch: tuple) -> tuple[torch.Tensor, int]:
     return X, step
 
 
+def inplace_rows_max(dtype: torch.dtype) -> int:
+    """Most output rows of an in-place rotation (out is C): one block of the
+    rotation kernel's output rows."""
+    if dtype == torch.complex128:
+        return 80
+    return 96 if _variant_c64 else 128
+
+
 def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
            alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
     """out = alpha M^T @ C + beta out over the last two axes, batched over the
@@ -142,7 +150,8 @@ def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
     precision, any strides with one unit-stride axis); C: (..., Nin, G)
     complex64 or complex128 with unit stride along G; out: (..., Nout, G) of
     C's type with unit stride along G (allocated when None; beta is then 0).
-    out must not overlap M or C."""
+    out must not overlap M; it may be C itself (a rotation in place, for
+    Nout = Nin <= inplace_rows_max) and must not overlap C otherwise."""
     if C.dtype not in _DTYPE:
         raise TypeError(f"rotate: complex64 or complex128 operand, not {C.dtype}")
     if not M.is_complex():
@@ -158,6 +167,11 @@ def rotate(M: torch.Tensor, C: torch.Tensor, out: Optional[torch.Tensor] = None,
         beta = 0.0
     elif out.shape != batch + (Nout, G) or out.dtype != C.dtype:
         raise ValueError(f"rotate: out {tuple(out.shape)} {out.dtype} != {batch + (Nout, G)}")
+    inplace = out is not None and out.data_ptr() == C.data_ptr()
+    if inplace and (out.stride() != C.stride() or out.shape != C.shape or C.stride(-1) != 1
+                    or Nout > inplace_rows_max(C.dtype)):
+        raise ValueError("rotate: in place needs out = C (same view, unit stride along G, "
+                         f"at most {inplace_rows_max(C.dtype)} rows)")
     if C.stride(-1) != 1:
         C = C.contiguous()
     if M.stride(-1) == 1:

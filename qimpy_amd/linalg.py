@@ -150,27 +150,33 @@ def eigh_grouped(mats: list, n_want: Optional[Sequence[int]] = None,
                                   else _lib.lib().qmx_heev_small_work_bytes(n))
         total += work_of[(n, w)] * H.shape[0]
     ws = torch.empty(total, dtype=torch.uint8, device=dev)
-    groups = {True: [], False: []}
+    # One descriptor per matrix, built per batch with array arithmetic.
+    parts = {True: [], False: []}
     off, i_info = 0, 0
+    ws0, info0 = ws.data_ptr(), info.data_ptr()
     for H, (E, V), w in zip(flat, outs, want):
         B, n = H.shape[0], H.shape[-1]
-        for b in range(B):
-            groups[bisect or n <= 128].append((H.data_ptr() + b * n * n * 16, E.data_ptr() + b * n * 8,
-                                               V.data_ptr() + b * n * n * 16,
-                                               info.data_ptr() + 4 * i_info, ws.data_ptr() + off,
-                                               n, n, n, w))
-            off += work_of[(n, w)]
-            i_info += 1
+        b = np.arange(B, dtype=np.uint64)
+        d = np.empty(B, dtype=_HEEV_DESC)
+        d["A"] = np.uint64(H.data_ptr()) + b * np.uint64(n * n * 16)
+        d["E"] = np.uint64(E.data_ptr()) + b * np.uint64(n * 8)
+        d["V"] = np.uint64(V.data_ptr()) + b * np.uint64(n * n * 16)
+        d["info"] = np.uint64(info0 + 4 * i_info) + b * np.uint64(4)
+        d["work"] = np.uint64(ws0 + off) + b * np.uint64(work_of[(n, w)])
+        d["lda"], d["ldv"], d["n"], d["n_want"] = n, n, n, w
+        parts[bisect or n <= 128].append(d)
+        off += work_of[(n, w)] * B
+        i_info += B
     keep = []
-    for small, rows in groups.items():
+    for small, rows in parts.items():
         if not rows:
             continue
-        arr = np.array(rows, dtype=_HEEV_DESC)
+        arr = np.concatenate(rows)
         d = torch.from_numpy(arr.view(np.uint8)).pin_memory().to(dev, non_blocking=True)
         keep.append(d)
         n_max = int(arr["n"].max())
         _lib.call("qmx_heevx_grouped" if bisect else "qmx_heev_small_grouped", d.data_ptr(),
-                  len(rows), n_max, rc.stream_ptr())
+                  len(arr), n_max, rc.stream_ptr())
     return [(E.reshape(H.shape[:-1]), V.reshape(H.shape)) for H, (E, V) in zip(mats, outs)], info
 
 

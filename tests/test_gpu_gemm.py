@@ -116,3 +116,32 @@ def test_blas_entry_points(gpu):
         _lib.call("qmx_gemm_c64", 1, 1, m, n, k, ct.c_double(1.0), At.data_ptr(), m, k * m,
                   Bt.data_ptr(), n, k * n, ct.c_double(0.0), Ct.data_ptr(), n, m * n, batch,
                   rc.stream_ptr())
+
+
+@pytest.mark.parametrize("n", [32, 72, 80, 128])
+@pytest.mark.parametrize("kind", ["c128", "c64", "bf16x3"])
+def test_rotate_in_place(gpu, n, kind):
+    """out = C (in place, one block of output rows) equals the out-of-place
+    rotation bit for bit, with and without accumulation; beyond the block
+    limit the library refuses it."""
+    dt = C128 if kind == "c128" else C64
+    gemm.set_bf16x3(kind == "bf16x3")
+    try:
+        limit = gemm.inplace_rows_max(dt)
+        print(f"[rotate in place] {kind} n {n} (limit {limit})", flush=True)
+        rng = np.random.default_rng(n)
+        M = torch.tensor(crandn(rng, (6, n, n)) / np.sqrt(n), device="cuda")
+        W = torch.tensor(crandn(rng, (6, n, 1411)), device="cuda").to(dt)
+        if n > limit:
+            with pytest.raises(ValueError):
+                gemm.rotate(M, W, out=W)
+            return
+        ref = gemm.rotate(M, W)
+        ref2 = gemm.rotate(M, W, out=W.clone(), alpha=0.5, beta=-1.0)
+        X = W.clone()
+        gemm.rotate(M, X, out=X)
+        X2 = W.clone()
+        gemm.rotate(M, X2, out=X2, alpha=0.5, beta=-1.0)
+        assert torch.equal(X, ref) and torch.equal(X2, ref2)
+    finally:
+        gemm.set_bf16x3(False)
