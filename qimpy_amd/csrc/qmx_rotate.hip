@@ -51,6 +51,11 @@ typedef __bf16 rot_bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kRotThreads = 256;  // four waves
 constexpr int kRotCols = 64;      // plane waves per workgroup (16 per wave)
+// k per slab (a multiple of 16). 32 was measured slower for complex64
+// (62 against 82 TFLOP/s on 128 x (144 -> 72) over 3456 plane waves): its
+// prefetched operands take the registers (121 against 56 VGPRs) that keep
+// eight waves per SIMD resident, and those waves hide the load latency.
+constexpr int kRotSlab = 16;
 
 template <typename T>
 struct RotMfma;
@@ -85,7 +90,7 @@ __device__ __forceinline__ MC2 op_m(const MC2* Mb, int64_t ldm, int transM, int 
   return transM ? Mb[(int64_t)k * ldm + i] : Mb[(int64_t)i * ldm + k];
 }
 
-template <typename T, typename MC2, int MT>
+template <typename T, typename MC2, int MT, int KSL = kRotSlab>
 __global__ void __launch_bounds__(kRotThreads)
     k_rotate(const MC2* __restrict__ M, int64_t ldm, int64_t sm, int transM,
              const typename Cplx<T>::T* __restrict__ W, int64_t ldw, int64_t sw,
@@ -95,7 +100,10 @@ __global__ void __launch_bounds__(kRotThreads)
   using MF = RotMfma<T>;
   constexpr int LDI = rot_ldi<MT>();
   constexpr int ROWS = 16 * MT;
-  constexpr int KS = 16;  // k per slab
+  constexpr int KS = KSL;  // k per slab
+  constexpr int NQ = KS / 4;        // k rows of four per slab
+  constexpr int NU = MT * KS / 16;  // op(M) values a thread stages per slab
+  static_assert(KS % 16 == 0, "slab of whole 16-row groups");
   __shared__ T sMr[KS * LDI], sMi[KS * LDI];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -119,18 +127,18 @@ __global__ void __launch_bounds__(kRotThreads)
   // and op(M) values are issued before this slab's barriers, so their
   // latency overlaps this slab's MFMAs; op(M) element u of a thread is
   // (row i, k kk) with tid + kRotThreads u = kk ROWS + i.
-  C2 wn[4];
-  T mrn[MT], min_[MT];
+  C2 wn[NQ];
+  T mrn[NU], min_[NU];
   auto load_w = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int k = k0 + 4 * q + lk;
       wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
     }
   };
   auto load_m = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < MT; ++u) {
+    for (int u = 0; u < NU; ++u) {
       const int e = tid + kRotThreads * u;  // < KS * ROWS
       const int kk = e / ROWS, i = e - kk * ROWS;
       const int gi = i0 + i, gk = k0 + kk;
@@ -146,15 +154,15 @@ __global__ void __launch_bounds__(kRotThreads)
   load_w(0);
   load_m(0);
   for (int k0 = 0; k0 < K; k0 += KS) {
-    C2 w[4];
+    C2 w[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const bool ok = k0 + 4 * q + lk < K;
       w[q] = ok ? wn[q] : C2{T(0), T(0)};
     }
-    T mr[MT], mi[MT];
+    T mr[NU], mi[NU];
 #pragma unroll
-    for (int u = 0; u < MT; ++u) {
+    for (int u = 0; u < NU; ++u) {
       mr[u] = mrn[u];
       mi[u] = min_[u];
     }
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(kRotThreads)
     }
     __syncthreads();  // the previous slab's LDS reads are done
 #pragma unroll
-    for (int u = 0; u < MT; ++u) {
+    for (int u = 0; u < NU; ++u) {
       const int e = tid + kRotThreads * u;
       const int kk = e / ROWS, i = e - kk * ROWS;
       sMr[kk * LDI + i] = mr[u];
@@ -174,7 +182,7 @@ __global__ void __launch_bounds__(kRotThreads)
     // Each of the three real products runs over all tiles before the next,
     // so consecutive MFMAs write different accumulators.
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int kr = (4 * q + lk) * LDI + li;
       T ar[MT], ai[MT];
 #pragma unroll
