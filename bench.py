@@ -412,6 +412,11 @@ def roofline(prof, wall_s: float) -> tuple[dict, dict]:
             e["traffic"] = round(traffic["ratio"] * by / cnt)
             e["traffic_over_compulsory"] = traffic["ratio"]
             e["traffic_source"] = traffic["source"]
+            if "valu_busy" in traffic:
+                # HBM is not this kernel's limit: the counters put it on the
+                # VALU (issue-cycle fraction, packed FP32 disabled).
+                e["limiter"] = {"unit": "VALU issue fraction", "achieved": traffic["valu_busy"],
+                                "source": traffic["source"]}
         return e
 
     lib_ids = [i for i in prof if not names[i].startswith("local_apply")]
@@ -457,8 +462,11 @@ def measured_traffic(kernel: str):
         return None
     for e in table.get("entries", []):
         if e.get("kernel") == kernel:
-            return {"ratio": e["traffic_over_compulsory"],
-                    "source": f"{e['profile']} via profiles/pmc_traffic.json"}
+            out = {"ratio": e["traffic_over_compulsory"],
+                   "source": f"{e['profile']} via profiles/pmc_traffic.json"}
+            if "valu_busy" in e:
+                out["valu_busy"] = e["valu_busy"]
+            return out
     return None
 
 
