@@ -229,6 +229,21 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb,
                          int64_t out_ldk, int64_t nk, int64_t nb, int64_t n_tot,
                          double* norm2, double* rnorm2, void* stream);
 
+/* The same residual, then each row scaled to unit norm (zero where its norm
+ * is below 1e-30) and zeroed past n[k] (int32 [nk] on the device), in one
+ * launch: the preconditioned, normalised and constrained expansion of a
+ * Davidson step (_davidson.py:182-187 with Wavefunction.constrain,
+ * _wavefunction.py:264-275). norm2 and rnorm2 are those before the
+ * scaling. With complex64 data the preconditioner factor
+ * 1 / (x + exp(-x)) is evaluated in FP32 (the residual and its norms stay
+ * FP64). */
+int qmx_precond_residual_normalize(int dtype, const void* C, const void* HC, int64_t ldb,
+                                   int64_t ldk, const double* eig, const double* KEref,
+                                   const double* ke, int64_t ke_ldk, void* out,
+                                   int64_t out_ldb, int64_t out_ldk, int64_t nk, int64_t nb,
+                                   int64_t n_tot, double* norm2, double* rnorm2,
+                                   const int32_t* n, void* stream);
+
 /* Linear combination of three wavefunction blocks of the same layout:
  *   out(k, b, g) = a[k, b] X(k, b, g) + b[k, b] Y(k, b, g) + c[k, b] Z(k, b, g)
  * with the coefficients device doubles at [k * cf_ldk + b * cf_ldb]

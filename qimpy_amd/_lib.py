@@ -59,6 +59,10 @@ SIGNATURES = {
         i32,
         [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp, vp],
     ),
+    "qmx_precond_residual_normalize": (
+        i32,
+        [i32, vp, vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, i64, i64, i64, vp, vp, vp, vp],
+    ),
     "qmx_scale_constrain": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, vp, vp]),
     "qmx_axpby": (i32, [i32, i64, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, vp, i64, i64, vp]),
     "qmx_apply_gradient": (i32, [i32, vp, i64, i64, i64, i64, i64, vp, i32, ct.c_double, vp, i64,

@@ -321,12 +321,6 @@ def _scale_constrain(X: torch.Tensor, scale: Optional[torch.Tensor], n32: torch.
               G, None if s is None else _c(s), _c(n32), rc.stream_ptr())
 
 
-def _normalize_constrain(X: torch.Tensor, norm: torch.Tensor, n32: torch.Tensor) -> None:
-    K, B, G = X.shape
-    _lib.call("qmx_normalize_constrain", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K,
-              B, G, _c(norm), _c(n32), rc.stream_ptr())
-
-
 def _dot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """conj(A) @ B^T over the basis axis, batched over k, complex128."""
     return gemm.dot128(A, B)
@@ -514,7 +508,7 @@ class Bucket:
         self.beta, self.D = beta, D
         self.V64 = V64
         self.V = V64 if dt == C128 else V64.to(self.rdtype)
-        self.rnorm = self.norm = None
+        self.rnorm = None
         need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dt], 2 * nb))
         one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], 1))
         size = max(one, min(need, WORKSPACE_CAP))
@@ -587,19 +581,19 @@ class Bucket:
 
     # ------------------------------------------------------------ iteration
     def residuals(self) -> None:
-        """R = precondition(HC - C eig) into X[:, nb:] with norms
-        (_davidson.py:182-184)."""
+        """R = precondition(HC - C eig) into X[:, nb:], normalised and
+        constrained, with the unpreconditioned residual norms for the
+        convergence read (_davidson.py:182-187), in one launch."""
         nb = self.nb
         C, HC, R = self.X[:, :nb], self.HX[:, :nb], self.X[:, nb:]
         KEref = _band_ke(C, self.ke)
         norm2 = torch.empty((self.Kb, nb), dtype=torch.float64, device=rc.device)
         rnorm2 = torch.empty_like(norm2)
         eig = self.eig.contiguous()
-        _lib.call("qmx_precond_residual", DTYPE_CODE[self.dtype], _c(C), _c(HC), C.stride(1),
-                  C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0), _c(R),
-                  R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
-                  rc.stream_ptr())
-        self.norm = norm2.sqrt()
+        _lib.call("qmx_precond_residual_normalize", DTYPE_CODE[self.dtype], _c(C), _c(HC),
+                  C.stride(1), C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0),
+                  _c(R), R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
+                  _c(self.n32), rc.stream_ptr())
         self.rnorm = rnorm2.sqrt()
 
     def pack(self) -> torch.Tensor:
@@ -656,9 +650,9 @@ class Bucket:
         nb, dt = self.nb, self.dtype
         C, R = self.X[:, :nb], self.X[:, nb:]
         HR = self.HX[:, nb:]
-        # Normalised residuals; a null one (below 1e-30, where 1/norm would
-        # overflow float32) is zeroed and then dropped with the null space:
-        _normalize_constrain(R, self.norm, self.n32)
+        # R holds the normalised residuals (residuals()); a null one (below
+        # 1e-30, where 1/norm would overflow float32) is zero and is then
+        # dropped with the null space.
         for _ in range(self.GS_PASSES):  # R <- (1 - C C^H) R
             gemm.rotate(_dot(C, R), C, out=R, alpha=-1.0, beta=1.0)
         # Canonical orthonormalisation of R (drops near-dependent directions):

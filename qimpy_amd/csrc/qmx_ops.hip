@@ -59,13 +59,33 @@ __global__ void __launch_bounds__(kRowThreads)
   }
 }
 
-template <typename C2>
+// The preconditioner factor 1 / (x + exp(-x)), x = KE / KEref, in FP64, or
+// in FP32 (LOW) where the residual itself is stored in float32. x is formed
+// in FP64 first: 1 / KEref of a (near-)null band can exceed the float range,
+// and KE = 0 times an infinite float would give NaN where FP64 gives 0; a
+// float x that overflows to inf gives the factor 0, as FP64 does.
+template <bool LOW>
+__device__ __forceinline__ double precond_factor(double ke, double inv_ref) {
+  if constexpr (LOW) {
+    const float x = (float)(ke * inv_ref);
+    return (double)(1.0f / (x + __expf(-x)));
+  } else {
+    const double x = ke * inv_ref;
+    return 1.0 / (x + exp(-x));
+  }
+}
+
+// NORM: the row is then scaled to unit norm (zero where its norm is below
+// 1e-30) and its padding g >= nvec[k] zeroed, in the same launch (the row
+// is still in cache when it is re-read).
+template <typename C2, bool LOW, bool NORM>
 __global__ void __launch_bounds__(kRowThreads)
     k_precond_residual(const C2* __restrict__ C, const C2* __restrict__ HC, int64_t ldb,
                        int64_t ldk, const double* __restrict__ eig,
                        const double* __restrict__ KEref, const double* __restrict__ ke,
                        int64_t ke_ldk, C2* __restrict__ out, int64_t oldb, int64_t oldk, int nb,
-                       int n_tot, double* __restrict__ norm2, double* __restrict__ rnorm2) {
+                       int n_tot, double* __restrict__ norm2, double* __restrict__ rnorm2,
+                       const int* __restrict__ nvec) {
   using R = decltype(C2{}.x);
   const int b = blockIdx.x, k = blockIdx.y;
   const int64_t ir = (int64_t)k * nb + b;
@@ -77,9 +97,7 @@ __global__ void __launch_bounds__(kRowThreads)
   const double inv_ref = 1.0 / KEref[ir];
   double s = 0, sr = 0;
   for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
-    double x = kk[g] * inv_ref;
-    x += exp(-x);
-    const double inv_x = 1.0 / x;
+    const double inv_x = precond_factor<LOW>(kk[g], inv_ref);
     const C2 cc = c[g], hh = h[g];
     const double ex = (double)hh.x - e * (double)cc.x;
     const double ey = (double)hh.y - e * (double)cc.y;
@@ -92,6 +110,21 @@ __global__ void __launch_bounds__(kRowThreads)
   if (threadIdx.x == 0) {
     norm2[ir] = s;
     if (rnorm2) rnorm2[ir] = sr;
+  }
+  if constexpr (NORM) {
+    __shared__ double scale;
+    if (threadIdx.x == 0) scale = s > 1e-60 ? 1.0 / sqrt(s) : 0.0;
+    __syncthreads();
+    const R f = (R)scale;
+    const int nk_valid = nvec[k];
+    for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+      if (g >= nk_valid || f == R(0)) {  // a null (or non-finite) row becomes zero
+        o[g] = C2{R(0), R(0)};
+      } else {
+        const C2 v = o[g];
+        o[g] = C2{v.x * f, v.y * f};
+      }
+    }
   }
 }
 
@@ -259,14 +292,39 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, 
   hipStream_t st = static_cast<hipStream_t>(stream);
   dim3 grid((unsigned)nb, (unsigned)nk);
   QMX_DISPATCH(dtype,
-               hipLaunchKernelGGL(k_precond_residual<float2>, grid, dim3(kRowThreads), 0, st,
-                                  (const float2*)C, (const float2*)HC, ldb, ldk, eig, KEref, ke,
-                                  ke_ldk, (float2*)out, out_ldb, out_ldk, (int)nb, (int)n_tot,
-                                  norm2, rnorm2),
-               hipLaunchKernelGGL(k_precond_residual<double2>, grid, dim3(kRowThreads), 0, st,
-                                  (const double2*)C, (const double2*)HC, ldb, ldk, eig, KEref,
-                                  ke, ke_ldk, (double2*)out, out_ldb, out_ldk, (int)nb,
-                                  (int)n_tot, norm2, rnorm2));
+               hipLaunchKernelGGL((k_precond_residual<float2, false, false>), grid,
+                                  dim3(kRowThreads), 0, st, (const float2*)C, (const float2*)HC,
+                                  ldb, ldk, eig, KEref, ke, ke_ldk, (float2*)out, out_ldb,
+                                  out_ldk, (int)nb, (int)n_tot, norm2, rnorm2, nullptr),
+               hipLaunchKernelGGL((k_precond_residual<double2, false, false>), grid,
+                                  dim3(kRowThreads), 0, st, (const double2*)C, (const double2*)HC,
+                                  ldb, ldk, eig, KEref, ke, ke_ldk, (double2*)out, out_ldb,
+                                  out_ldk, (int)nb, (int)n_tot, norm2, rnorm2, nullptr));
+  return QMX_OK;
+}
+
+int qmx_precond_residual_normalize(int dtype, const void* C, const void* HC, int64_t ldb,
+                                   int64_t ldk, const double* eig, const double* KEref,
+                                   const double* ke, int64_t ke_ldk, void* out, int64_t out_ldb,
+                                   int64_t out_ldk, int64_t nk, int64_t nb, int64_t n_tot,
+                                   double* norm2, double* rnorm2, const int32_t* n,
+                                   void* stream) {
+  QMX_CHECK_ARG(C && HC && eig && KEref && ke && out && norm2 && n,
+                "qmx_precond_residual_normalize: null argument");
+  QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
+                "qmx_precond_residual_normalize: bad sizes");
+  if (nk == 0 || nb == 0) return QMX_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  dim3 grid((unsigned)nb, (unsigned)nk);
+  QMX_DISPATCH(dtype,
+               hipLaunchKernelGGL((k_precond_residual<float2, true, true>), grid,
+                                  dim3(kRowThreads), 0, st, (const float2*)C, (const float2*)HC,
+                                  ldb, ldk, eig, KEref, ke, ke_ldk, (float2*)out, out_ldb,
+                                  out_ldk, (int)nb, (int)n_tot, norm2, rnorm2, n),
+               hipLaunchKernelGGL((k_precond_residual<double2, false, true>), grid,
+                                  dim3(kRowThreads), 0, st, (const double2*)C, (const double2*)HC,
+                                  ldb, ldk, eig, KEref, ke, ke_ldk, (double2*)out, out_ldb,
+                                  out_ldk, (int)nb, (int)n_tot, norm2, rnorm2, n));
   return QMX_OK;
 }
 

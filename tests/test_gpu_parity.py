@@ -310,6 +310,47 @@ def test_band_reductions_and_precond(cfg0_el):
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
+def test_precond_residual_normalize(cfg0_el, dtype):
+    """qmx_precond_residual_normalize (the batched step's residual pass):
+    the preconditioned residual of the oracle, scaled to unit norm per band
+    and zero past each k's basis, with the norms before scaling. Complex64
+    evaluates the preconditioner factor in FP32 (tolerance 1e-5)."""
+    from oracle.davidson import precondition as o_precondition
+    from qimpy_amd import _lib, rc
+    from qimpy_amd.basis import DTYPE_CODE
+
+    spec, ob, el = cfg0_el
+    K, B, G = len(ob.k), 5, ob.n_tot
+    C = rand_coeff(ob, B, seed=8)
+    HC = rand_coeff(ob, B, seed=9)
+    eig = np.random.default_rng(10).standard_normal((K, B))
+    KEref = np.einsum("kbg,kg->kb", np.abs(C) ** 2, ob.KE)
+    ref = o_precondition(ob, HC - C * eig[..., None], KEref)
+    nref = np.sqrt((np.abs(ref) ** 2).sum(-1))
+    dev = "cuda"
+    Ct = torch.tensor(C, device=dev).to(dtype).contiguous()
+    HCt = torch.tensor(HC, device=dev).to(dtype).contiguous()
+    out = torch.full_like(Ct, 7.0)  # padding must come back zero
+    norm2 = torch.empty((K, B), dtype=torch.float64, device=dev)
+    rnorm2 = torch.empty_like(norm2)
+    eig_t = torch.tensor(eig, device=dev)
+    KEref_t = torch.tensor(KEref, device=dev)
+    ke_t = torch.tensor(ob.KE, device=dev).contiguous()
+    n32 = torch.tensor(ob.n, dtype=torch.int32, device=dev)
+    _lib.call("qmx_precond_residual_normalize", DTYPE_CODE[dtype], Ct.data_ptr(), HCt.data_ptr(),
+              G, B * G, eig_t.data_ptr(), KEref_t.data_ptr(), ke_t.data_ptr(), G,
+              out.data_ptr(), G, B * G, K, B, G, norm2.data_ptr(), rnorm2.data_ptr(),
+              n32.data_ptr(), rc.stream_ptr())
+    tol = 1e-13 if dtype == C128 else 1e-5
+    got = out.cpu().numpy()
+    assert rel(got, ref / nref[..., None]) < tol
+    assert rel(norm2.sqrt().cpu().numpy(), nref) < tol
+    rn = np.sqrt((np.abs(HC - C * eig[..., None]) ** 2).sum(-1))
+    assert rel(rnorm2.sqrt().cpu().numpy(), rn) < (1e-14 if dtype == C128 else 1e-6)
+    assert not np.any(got[ob.pad_mask[:, None, :].repeat(B, axis=1)])
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
 def test_hamiltonian(dtype, small_el, cfg0_el):
     for spec, ob, el in (small_el, cfg0_el):
         C = rand_coeff(ob, 6, seed=8)
