@@ -427,7 +427,7 @@ class Bucket:
         jobs = keep + new
         if not jobs:
             self.jobs, self.k0, self.Kb = [], [], 0
-            for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "norm", "beta", "D",
+            for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "pnorm2", "beta", "D",
                          "ke", "ke_r", "n32", "V", "V64", "workspace", "_full", "_beta_full",
                          "_D_full", "_ke_full"):
                 setattr(self, name, None)
@@ -508,7 +508,7 @@ class Bucket:
         self.beta, self.D = beta, D
         self.V64 = V64
         self.V = V64 if dt == C128 else V64.to(self.rdtype)
-        self.rnorm = None
+        self.rnorm = self.pnorm2 = None
         need = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[dt], 2 * nb))
         one = int(_lib.lib().qmx_plan_workspace_bytes(self._plan, DTYPE_CODE[C128], 1))
         size = max(one, min(need, WORKSPACE_CAP))
@@ -581,20 +581,23 @@ class Bucket:
 
     # ------------------------------------------------------------ iteration
     def residuals(self) -> None:
-        """R = precondition(HC - C eig) into X[:, nb:], normalised and
-        constrained, with the unpreconditioned residual norms for the
-        convergence read (_davidson.py:182-187), in one launch."""
+        """R = precondition(HC - C eig) into X[:, nb:], with the
+        unpreconditioned residual norms for the convergence read
+        (_davidson.py:182-184). Zero where C and HC are (the padding). Not
+        normalised: the step folds the normalisation of _davidson.py:186-187
+        into its small matrices."""
         nb = self.nb
         C, HC, R = self.X[:, :nb], self.HX[:, :nb], self.X[:, nb:]
         KEref = _band_ke(C, self.ke)
         norm2 = torch.empty((self.Kb, nb), dtype=torch.float64, device=rc.device)
         rnorm2 = torch.empty_like(norm2)
         eig = self.eig.contiguous()
-        _lib.call("qmx_precond_residual_normalize", DTYPE_CODE[self.dtype], _c(C), _c(HC),
-                  C.stride(1), C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0),
-                  _c(R), R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
-                  _c(self.n32), rc.stream_ptr())
+        _lib.call("qmx_precond_residual", DTYPE_CODE[self.dtype], _c(C), _c(HC), C.stride(1),
+                  C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0), _c(R),
+                  R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
+                  rc.stream_ptr())
         self.rnorm = rnorm2.sqrt()
+        self.pnorm2 = norm2
 
     def pack(self) -> torch.Tensor:
         """Convergence data for the host read: [rnorm, deig, eig, number of
@@ -650,17 +653,22 @@ class Bucket:
         nb, dt = self.nb, self.dtype
         C, R = self.X[:, :nb], self.X[:, nb:]
         HR = self.HX[:, nb:]
-        # R holds the normalised residuals (residuals()); a null one (below
-        # 1e-30, where 1/norm would overflow float32) is zero and is then
-        # dropped with the null space.
         for _ in range(self.GS_PASSES):  # R <- (1 - C C^H) R
             gemm.rotate(_dot(C, R), C, out=R, alpha=-1.0, beta=1.0)
-        # Canonical orthonormalisation of R (drops near-dependent directions):
-        lam, W = yield _dot(R, R)
+        # Canonical orthonormalisation of the normalised residuals (drops
+        # near-dependent directions). The normalisation (_davidson.py:186-187)
+        # acts on the small matrices only: s = 1 / |R_i| before the
+        # projection above (0 for a null row, below 1e-30) scales the Gram
+        # matrix by s_i s_j and the canonical basis along its input axis, so
+        # no pass over R is needed (row scaling commutes with the projection).
+        d = self.pnorm2
+        sc = torch.where(d > 1e-60, d.clamp_min(1e-300).rsqrt(), torch.zeros_like(d))
+        lam, W = yield _dot(R, R) * (sc[:, :, None] * sc[:, None, :])
         tol = self.DROP_RTOL_MIXED if self.mixed else self.DROP_RTOL
         Ut = torch.empty((self.Kb, nb, nb), dtype=dt, device=rc.device)
         _lib.call("qmx_canonical_basis", DTYPE_CODE[dt], _c(W), _c(lam), self.Kb, nb, tol, _c(Ut),
                   rc.stream_ptr())
+        Ut *= sc[:, None, :].to(Ut.real.dtype)  # Ut[j][i] s_i: R' = Ut diag(s) R
         if nb <= gemm.inplace_rows_max(dt):
             gemm.rotate(Ut.transpose(-2, -1), R, out=R)  # Ut @ R in place
         else:
