@@ -45,6 +45,8 @@ typedef struct qmx_geom {
   const int32_t* n;          /* [nk] in-sphere counts */
   const int64_t* fft_index;  /* [nk * n_tot] linear box indices */
   const int32_t* v_slot;     /* [nk] potential box used by each k, or NULL (all 0) */
+  const int64_t* const* fft_rows; /* [nk] start of each k's n[k] indices, or NULL (rows
+                                     of fft_index; with fft_rows, fft_index may be NULL) */
 } qmx_geom;
 
 typedef struct qmx_plan qmx_plan;
@@ -264,6 +266,27 @@ int qmx_axpby(int dtype, int64_t nk, int64_t nb, int64_t n_tot, const double* a,
 int qmx_scale_constrain(int dtype, void* C, int64_t ldb, int64_t ldk, int64_t nk,
                         int64_t nb, int64_t n_tot, const double* scale,
                         const int32_t* n, void* stream);
+
+/* Row blocks between device buffers in one launch (a batched strided copy
+ * with zero padding). No counterpart in the reference: it moves the stacked
+ * state of the batched engine when a bucket's jobs change (kept jobs'
+ * wavefunctions, the rows of jobs that moved, their projectors and
+ * potentials). blocks: device array of n qmx_copy_block; block i writes
+ *   dst[i0 * dst_s0 + i1 * dst_s1 + c], i0 < n0, i1 < n1, c < dst_cols,
+ * from src at the same (i0, i1) (src strides) for c < src_cols and zero
+ * beyond it, or zero throughout when src is NULL. mode 0 copies 8-byte
+ * words (counts and strides in words); mode 1 converts doubles to floats
+ * (counts and strides in reals of each side). Workgroups are dealt out in
+ * block order: block i takes ceil(n0 n1 / rows_per_wg) of them from
+ * first_wg on, and n_wg is their total. Blocks must not overlap each other
+ * or their sources. */
+typedef struct qmx_copy_block {
+  const void* src;
+  void* dst;
+  int64_t n0, n1, src_cols, dst_cols, src_s0, src_s1, dst_s0, dst_s1;
+  int64_t rows_per_wg, first_wg, mode, reserved;
+} qmx_copy_block;
+int qmx_copy_blocks(const void* blocks, int64_t n, int64_t n_wg, void* stream);
 
 /* Bandwidth-limited xorshift randomisation of bands [b_start, b_stop)
  * (Wavefunction.randomize, _wavefunction_init.py:27-125): state

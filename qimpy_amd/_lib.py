@@ -33,11 +33,13 @@ class QmxGeom(ct.Structure):
         ("n", c_i32p),
         ("fft_index", c_i64p),
         ("v_slot", c_i32p),
+        ("fft_rows", ct.POINTER(c_i64p)),
     ]
 
 
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "qmx_copy_blocks": (i32, [vp, i64, i64, vp]),
     "qmx_geom_tables": (i32, [ct.POINTER(QmxGeom), c_i64p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p]),
     "qmx_plan_create": (i32, [ct.POINTER(QmxGeom), ct.POINTER(vp)]),
     "qmx_plan_destroy": (i32, [vp]),
@@ -167,13 +169,50 @@ def make_geom(shape, n: np.ndarray, fft_index: np.ndarray,
     g.n_tot = n_tot
     g.n = n32.ctypes.data_as(c_i32p)
     g.fft_index = fi.ctypes.data_as(c_i64p)
-    vs = None
-    if v_slot is not None:
-        vs = np.ascontiguousarray(v_slot, dtype=np.int32)
-        if vs.shape != (nk,):
-            raise ValueError("v_slot needs one entry per k-point")
-        g.v_slot = vs.ctypes.data_as(c_i32p)
+    vs = _set_v_slot(g, nk, v_slot)
     return g, (n32, fi, vs)
+
+
+def _set_v_slot(g: QmxGeom, nk: int, v_slot) -> np.ndarray | None:
+    if v_slot is None:
+        return None
+    vs = np.ascontiguousarray(v_slot, dtype=np.int32)
+    if vs.shape != (nk,):
+        raise ValueError("v_slot needs one entry per k-point")
+    g.v_slot = vs.ctypes.data_as(c_i32p)
+    return vs
+
+
+def make_geom_rows(shape, n: np.ndarray, blocks: list, n_tot: int,
+                   v_slot: np.ndarray | None = None) -> tuple[QmxGeom, tuple]:
+    """Geometry struct whose k-points are the rows of `blocks` (int64 arrays
+    (K_i, n_tot_i), C-contiguous, in order) through per-k row pointers, so
+    that the index rows of many cells need no concatenated copy. The blocks
+    must stay alive while the struct is used (the returned tuple holds
+    them)."""
+    n32 = np.ascontiguousarray(n, dtype=np.int32)
+    rows = []
+    for b in blocks:
+        if b.dtype != np.int64 or not b.flags.c_contiguous or b.ndim != 2:
+            raise ValueError("index blocks must be C-contiguous int64 matrices")
+        if b.shape[1] > n_tot:
+            raise ValueError("index block wider than n_tot")
+        rows.append(b.ctypes.data + b.strides[0] * np.arange(b.shape[0], dtype=np.uint64))
+    ptrs = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros(0, np.uint64))
+    nk = len(ptrs)
+    if n32.shape != (nk,):
+        raise ValueError("n needs one entry per k-point")
+    if np.any(n32 > np.concatenate([np.full(b.shape[0], b.shape[1]) for b in blocks])):
+        raise ValueError("n[k] exceeds its index row")
+    g = QmxGeom()
+    for i in range(3):
+        g.shape[i] = int(shape[i])
+    g.nk = nk
+    g.n_tot = int(n_tot)
+    g.n = n32.ctypes.data_as(c_i32p)
+    g.fft_rows = ptrs.ctypes.data_as(ct.POINTER(c_i64p))
+    vs = _set_v_slot(g, nk, v_slot)
+    return g, (n32, ptrs, vs, blocks)
 
 
 def create_plan(geom: QmxGeom) -> ct.c_void_p:

@@ -276,6 +276,68 @@ def _to_device(a, dtype=None) -> torch.Tensor:
     return t.pin_memory().to(rc.device, non_blocking=True)
 
 
+class _BlockCopies:
+    """Row-block copies between device tensors, issued together as one
+    qmx_copy_blocks launch (a restack moves many small and a few large
+    blocks; one torch copy each costs more host time than the device
+    needs)."""
+
+    WG_BYTES = 32 << 10  # bytes written per workgroup
+    _WORDS = {torch.float64: 1, C64: 1, C128: 2}
+    _NARROW = {(C128, C64): 2, (torch.float64, torch.float32): 1}  # reals per element
+
+    def __init__(self):
+        self.rows: list = []
+
+    def add(self, dst: torch.Tensor, src: Optional[torch.Tensor] = None) -> None:
+        """dst <- src along the last axis, zero past src's length (zero
+        throughout without src). 1-3 dimensional views with unit stride
+        along the last axis; src broadcasts over dst's leading axes. Same
+        dtype (float64, complex64, complex128), or complex128 into
+        complex64 / float64 into float32 (rounded)."""
+        if dst.numel() == 0:
+            return
+        if dst.dim() > 3 or dst.stride(-1) != 1:
+            raise ValueError("copy destination needs 1-3 axes and unit stride along the last")
+        lead = tuple(dst.shape[:-1])
+        if src is not None:
+            src = src.expand(*lead, src.shape[-1])
+            if src.stride(-1) != 1 and src.shape[-1] > 1:
+                raise ValueError("copy source needs unit stride along its last axis")
+            if src.shape[-1] > dst.shape[-1]:
+                raise ValueError("copy source longer than its destination")
+        if src is None or src.dtype == dst.dtype:
+            mode, us, ud = 0, self._WORDS[dst.dtype], self._WORDS[dst.dtype]
+        else:
+            mode = 1
+            us = ud = self._NARROW[(src.dtype, dst.dtype)]
+
+        def dims3(t, u):
+            st = [x * u for x in t.stride()[:-1]]
+            return [0] * (2 - len(st)) + st  # (the padded leading axes have length 1)
+
+        shape = [1] * (3 - dst.dim()) + list(dst.shape)
+        d_s = dims3(dst, ud)
+        s_s = dims3(src, us) if src is not None else [0, 0]
+        dst_cols = dst.shape[-1] * ud
+        src_cols = src.shape[-1] * us if src is not None else 0
+        unit_bytes = 4 if mode == 1 else 8
+        rpw = max(1, self.WG_BYTES // (dst_cols * unit_bytes))
+        self.rows.append([src.data_ptr() if src is not None else 0, dst.data_ptr(), shape[0],
+                          shape[1], src_cols, dst_cols, s_s[0], s_s[1], d_s[0], d_s[1], rpw, 0,
+                          mode, 0])
+
+    def run(self) -> None:
+        if not self.rows:
+            return
+        a = np.array(self.rows, dtype=np.int64)
+        wg = -(-(a[:, 2] * a[:, 3]) // a[:, 10])
+        a[:, 11] = np.cumsum(wg) - wg
+        d = _to_device(a)
+        _lib.call("qmx_copy_blocks", _c(d), len(a), int(wg.sum()), rc.stream_ptr())
+        self.rows = []
+
+
 #: Plans of restacked buckets, destroyed at the next host synchronisation
 #: (freeing device memory waits for the whole device).
 _retired_plans: list = []  # (stamp, plan handle)
@@ -450,6 +512,7 @@ class Bucket:
         Xn, HXn, X2n, HX2n = (b[:Kb] for b in self._full)
         eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
         deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
+        cp = _BlockCopies()
         if keep:
             # Kept jobs in runs of consecutive old rows, one copy per run.
             runs = []  # (new row, old row, rows)
@@ -461,12 +524,11 @@ class Bucket:
                 else:
                     runs.append([a_new, a_old, j.K])
                 a_new += j.K
-            g = old_X.shape[-1]
-            for a, o, n_rows in runs:
-                Xn[a : a + n_rows, :nb, :g] = old_X[o : o + n_rows, :nb]
-                HXn[a : a + n_rows, :nb, :g] = old_HX[o : o + n_rows, :nb]
-                eig[a : a + n_rows] = self.eig[o : o + n_rows]
-                deig[a : a + n_rows] = self.deig[o : o + n_rows]
+            for a, o, n_rows in runs:  # (zero past the old basis length)
+                cp.add(Xn[a : a + n_rows, :nb], old_X[o : o + n_rows, :nb])
+                cp.add(HXn[a : a + n_rows, :nb], old_HX[o : o + n_rows, :nb])
+                cp.add(eig[a : a + n_rows], self.eig[o : o + n_rows])
+                cp.add(deig[a : a + n_rows], self.deig[o : o + n_rows])
         k0, acc = [], 0
         for j in jobs:
             k0.append(acc)
@@ -475,7 +537,6 @@ class Bucket:
         beta = self._beta_full[:Kb]
         D = self._D_full[:Kb]
         V64 = torch.empty((len(jobs), self.N), dtype=torch.float64, device=dev)
-        fi = np.zeros((Kb, Gb), dtype=np.int64)
         v_slot = np.zeros(Kb, dtype=np.int32)
         for i, (j, a) in enumerate(zip(jobs, k0)):
             t = j.device_tables()
@@ -483,20 +544,20 @@ class Bucket:
                 # The job's rows with their padding zeroed (entries past its
                 # basis and projectors), written only where they moved.
                 rows = slice(a, a + j.K)
-                ke[rows, : j.n_tot] = t["ke"]
-                ke[rows, j.n_tot :] = 0.0
+                cp.add(ke[rows], t["ke"])
                 if j.P:
-                    beta[rows, : j.P, : j.n_tot] = j.projectors(j.n_tot)
-                    beta[rows, : j.P, j.n_tot :] = 0.0
-                    D[rows, : j.P] = t["D"]
-                beta[rows, j.P :] = 0.0
-                D[rows, j.P :] = 0.0
-            V64[i] = t["V"]
-            fi[a : a + j.K, : j.n_tot] = j.fft_index
+                    cp.add(beta[rows, : j.P], j.projectors(j.n_tot))
+                    cp.add(D[rows], t["D"])
+                else:
+                    cp.add(D[rows])
+                cp.add(beta[rows, j.P :])
+            cp.add(V64[i], t["V"])
             v_slot[a : a + j.K] = i
+        cp.run()
         n = np.concatenate([j.n for j in jobs])
         self._destroy_plan()
-        geom, keepalive = _lib.make_geom(self.shape, n, fi, v_slot)
+        geom, keepalive = _lib.make_geom_rows(self.shape, n, [j.fft_index for j in jobs], Gb,
+                                              v_slot)
         self._plan = _lib.create_plan(geom)
         del keepalive
         self.jobs, self.k0, self.Kb, self.Gb, self.Pb = jobs, k0, Kb, Gb, Pb

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "qmx_common.h"
@@ -1570,15 +1571,21 @@ static void pool_release(const PoolBlock& blk) {
 // first waits for the event recorded after that plan's last call. Every
 // entry point that reads a plan waits for `uploaded` on its own stream
 // (plan_enter).
-static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>& src,
-                         const std::vector<int**>& dst) {
-  std::vector<size_t> off(src.size());
+struct TableSpan {
+  const int* src;  // copied into the block; null: written by the fill callback
+  size_t n;        // ints
+  int** dst;       // set to the table's device address
+};
+
+template <typename Fill>
+static int upload_tables(qmx_plan* p, const std::vector<TableSpan>& spans, Fill&& fill) {
+  std::vector<size_t> off(spans.size());
   size_t total = 0;
-  for (size_t i = 0; i < src.size(); ++i) {
+  for (size_t i = 0; i < spans.size(); ++i) {
     off[i] = total;
-    total += (src[i]->size() * sizeof(int) + 255) / 256 * 256;
+    total += (spans[i].n * sizeof(int) + 255) / 256 * 256;
+    *spans[i].dst = nullptr;
   }
-  for (int** d : dst) *d = nullptr;
   if (total == 0) return QMX_OK;
   PoolBlock blk;
   const int rc = pool_acquire(total, &blk);
@@ -1590,11 +1597,14 @@ static int upload_tables(qmx_plan* p, const std::vector<const std::vector<int>*>
   p->last_use = blk.last_use;
   auto* h = static_cast<char*>(p->h_stage);
   auto* d = static_cast<char*>(p->d_block);
-  for (size_t i = 0; i < src.size(); ++i) {
-    if (src[i]->empty()) continue;
-    std::memcpy(h + off[i], src[i]->data(), src[i]->size() * sizeof(int));
-    *dst[i] = reinterpret_cast<int*>(d + off[i]);
+  std::vector<int*> host(spans.size());
+  for (size_t i = 0; i < spans.size(); ++i) {
+    host[i] = reinterpret_cast<int*>(h + off[i]);
+    if (spans[i].n == 0) continue;
+    if (spans[i].src) std::memcpy(host[i], spans[i].src, spans[i].n * sizeof(int));
+    *spans[i].dst = reinterpret_cast<int*>(d + off[i]);
   }
+  fill(host);
   QMX_HIP(hipStreamWaitEvent(nullptr, p->last_use, 0));
   QMX_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, nullptr));
   QMX_HIP(hipEventRecord(p->uploaded, nullptr));
@@ -1646,63 +1656,67 @@ const char* qmx_version(void) { return "qmx 0.1 gfx950"; }
 
 namespace {
 
-// Host-side index tables of a basis (no device calls).
-struct GeomTables {
-  std::vector<int> x_list, pcol_start, col_iy, cstart, t1idx;
-  std::vector<int> zrange;  // [nk][ncol] packed (lo, hi), see column_intervals
-  bool col_interval = true;
-  int col_run_max = 0;
+// Host-side index tables of a basis (no device calls), built in two phases:
+// the column structure (scan_columns), then the per-k tables into arrays the
+// caller provides (fill_k_tables), so that a plan writes them straight into
+// its pinned staging block. Both phases split the k-points over threads
+// when a geometry has enough sphere points to pay for them.
+struct GeomColumns {
+  std::vector<int> x_list, pcol_start, col_iy, col_ix;
+  std::vector<int> col_of;  // [n0 * n1] column index of each box column, -1 if empty
 };
 
-// The z interval of each column's points, for the fixed column passes: a
-// sphere meets a line of the box in one interval of signed z, stored in
-// increasing z as z = lo..hi (lo >= 0) or, when it contains z = 0 and
-// negative z, as 0..hi then n + lo..n - 1 (lo < 0). Packed as
-// (lo & 0xffff) | hi << 16; an empty column is (1, 0). col_interval turns
-// false if any column's points are not one such interval.
-void column_intervals(int nk, int ncol, int n_tot, int n2, GeomTables* t) {
-  t->zrange.assign((size_t)nk * ncol, 1);
-  t->col_interval = true;
-  t->col_run_max = 0;
-  for (int k = 0; k < nk; ++k) {
-    const int* cs = t->cstart.data() + (size_t)k * (ncol + 1);
-    const int* t1 = t->t1idx.data() + (size_t)k * n_tot;
-    for (int c = 0; c < ncol; ++c) {
-      const int a = cs[c], b = cs[c + 1], cnt = b - a, base = c * n2;
-      int lo = 1, hi = 0;
-      if (cnt > 0) {
-        const int zf = t1[a] - base, zl = t1[b - 1] - base;
-        if (zl - zf + 1 == cnt) {
-          lo = zf;
-          hi = zl;
-        } else {
-          int h = 0;  // points 0..h, then l..n2 - 1
-          while (h + 1 < cnt && t1[a + h + 1] - base == h + 1) ++h;
-          const int l = h + 1 < cnt ? t1[a + h + 1] - base : n2;
-          if (zf == 0 && zl == n2 - 1 && cnt - (h + 1) == n2 - l) {
-            lo = l - n2;
-            hi = h;
-          } else {
-            t->col_interval = false;
-          }
-        }
-      }
-      t->zrange[(size_t)k * ncol + c] = (lo & 0xFFFF) | (hi << 16);
-    }
-    for (int c0 = 0; c0 < ncol; c0 += kColFixedThreads)
-      t->col_run_max =
-          std::max(t->col_run_max, cs[std::min(ncol, c0 + kColFixedThreads)] - cs[c0]);
-  }
+struct KTables {
+  int* cstart;        // [nk][ncol + 1] first sphere point of each column
+  int* t1idx;         // [nk][n_tot] col * n2 + z of each sphere point, 0 past n[k]
+  int* zrange;        // [nk][ncol] packed z interval of each column (below)
+  bool col_interval;  // every column of every k is one such interval
+  int col_run_max;    // most points of kColFixedThreads consecutive columns of one k
+};
+
+constexpr int64_t kPointsPerThread = int64_t(1) << 18;
+constexpr int kMaxTableThreads = 16;
+
+int table_threads(int nk, int64_t points) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int64_t t = std::min<int64_t>(std::min(hw, kMaxTableThreads), nk);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(t, points / kPointsPerThread + 1));
 }
 
-int build_tables(const qmx_geom* geom, GeomTables* t) {
+// f(slot, k_begin, k_end) over T consecutive ranges of [0, nk), range 0 on
+// the calling thread (and any range whose thread cannot be started).
+template <typename F>
+void run_k_ranges(int T, int nk, F&& f) {
+  auto lo = [&](int t) { return (int)((int64_t)nk * t / T); };
+  std::vector<std::thread> th;
+  th.reserve(T > 1 ? T - 1 : 0);
+  for (int t = 1; t < T; ++t) {
+    try {
+      th.emplace_back([&f, t, a = lo(t), b = lo(t + 1)] { f(t, a, b); });
+    } catch (...) {
+      f(t, lo(t), lo(t + 1));
+    }
+  }
+  f(0, 0, lo(1));
+  for (auto& x : th) x.join();
+}
+
+const int64_t* fft_row(const qmx_geom* geom, int k) {
+  return geom->fft_rows ? geom->fft_rows[k] : geom->fft_index + (int64_t)k * geom->n_tot;
+}
+
+int scan_columns(const qmx_geom* geom, GeomColumns* c) {
   QMX_CHECK_ARG(geom, "qmx_geom_tables: null geometry");
   const int n0 = geom->shape[0], n1 = geom->shape[1], n2 = geom->shape[2];
   const int nk = geom->nk, n_tot = geom->n_tot;
   QMX_CHECK_ARG(n0 > 0 && n1 > 0 && n2 > 0 && n0 <= 512 && n1 <= 512 && n2 <= 512,
                 "qmx_geom_tables: box sizes must be in 1..512");
   QMX_CHECK_ARG(nk >= 1 && nk < 65536 && n_tot >= 1, "qmx_geom_tables: bad nk / n_tot");
-  QMX_CHECK_ARG(geom->n && geom->fft_index, "qmx_geom_tables: null n / fft_index");
+  QMX_CHECK_ARG(geom->n && (geom->fft_index || geom->fft_rows),
+                "qmx_geom_tables: null n / fft_index");
+  if (geom->fft_rows)
+    for (int k = 0; k < nk; ++k)
+      QMX_CHECK_ARG(geom->fft_rows[k] || geom->n[k] == 0, "qmx_geom_tables: null fft_rows[k]");
   const int64_t N = (int64_t)n0 * n1 * n2;
   QMX_CHECK_ARG(N < (1ll << 31), "qmx_geom_tables: box too large");
   for (int a = 0; a < 3; ++a) {
@@ -1710,72 +1724,137 @@ int build_tables(const qmx_geom* geom, GeomTables* t) {
     QMX_CHECK_ARG(factorize(geom->shape[a], &rad),
                   "qmx_geom_tables: box sizes must be products of 2, 3 and 5");
   }
-  // Columns and planes touched by any k. Sphere points come in increasing
-  // box order, so one column's points are consecutive: the column index is
-  // recomputed only where a point leaves the current column.
-  std::vector<char> col_seen((size_t)n0 * n1, 0), x_seen(n0, 0);
-  for (int k = 0; k < nk; ++k) {
-    const int nkk = geom->n[k];
-    QMX_CHECK_ARG(nkk >= 0 && nkk <= n_tot, "qmx_geom_tables: n[k] outside 0..n_tot");
-    const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
-    int64_t prev = -1, col_end = -1;
-    for (int g = 0; g < nkk; ++g) {
-      const int64_t idx = fi[g];
-      QMX_CHECK_ARG(idx > prev && idx < N,
-                    "qmx_geom_tables: fft_index must be increasing box indices within each k");
-      prev = idx;
-      if (idx >= col_end) {
-        const int col = (int)(idx / n2);
-        col_end = (int64_t)(col + 1) * n2;
-        col_seen[col] = 1;
-        x_seen[col / n1] = 1;
+  for (int k = 0; k < nk; ++k)
+    QMX_CHECK_ARG(geom->n[k] >= 0 && geom->n[k] <= n_tot,
+                  "qmx_geom_tables: n[k] outside 0..n_tot");
+  // Columns touched by any k (a bitmap per thread, merged below). Sphere
+  // points come in increasing box order, so one column's points are
+  // consecutive: the column index is recomputed only where a point leaves
+  // the current column.
+  const int T = table_threads(nk, (int64_t)nk * n_tot);
+  const size_t ncols_box = (size_t)n0 * n1;
+  std::vector<std::vector<char>> seen(T);
+  std::vector<char> bad(T, 0);
+  run_k_ranges(T, nk, [&](int t, int k0, int k1) {
+    std::vector<char>& s = seen[t];
+    s.assign(ncols_box, 0);
+    for (int k = k0; k < k1; ++k) {
+      const int64_t* fi = fft_row(geom, k);
+      int64_t prev = -1, col_end = -1;
+      for (int g = 0; g < geom->n[k]; ++g) {
+        const int64_t idx = fi[g];
+        if (idx <= prev || idx >= N) {
+          bad[t] = 1;
+          return;
+        }
+        prev = idx;
+        if (idx >= col_end) {
+          const int64_t col = idx / n2;
+          col_end = (col + 1) * n2;
+          s[col] = 1;
+        }
       }
     }
-  }
-  std::vector<int> col_of((size_t)n0 * n1, -1);
-  t->x_list.clear();
-  t->pcol_start.clear();
-  t->col_iy.clear();
+  });
+  for (int t = 0; t < T; ++t)
+    QMX_CHECK_ARG(!bad[t], "qmx_geom_tables: fft_index must be increasing box indices within each k");
+  for (int t = 1; t < T; ++t)
+    for (size_t i = 0; i < ncols_box; ++i) seen[0][i] |= seen[t][i];
+  const std::vector<char>& col_seen = seen[0];
+  c->col_of.assign(ncols_box, -1);
+  c->x_list.clear();
+  c->pcol_start.clear();
+  c->col_iy.clear();
+  c->col_ix.clear();
   for (int x = 0; x < n0; ++x) {
-    if (!x_seen[x]) continue;
-    t->x_list.push_back(x);
-    t->pcol_start.push_back((int)t->col_iy.size());
+    const char* row = col_seen.data() + (size_t)x * n1;
+    if (std::find(row, row + n1, 1) == row + n1) continue;
+    c->x_list.push_back(x);
+    c->pcol_start.push_back((int)c->col_iy.size());
     for (int y = 0; y < n1; ++y) {
-      const int64_t c = (int64_t)x * n1 + y;
-      if (col_seen[c]) {
-        col_of[c] = (int)t->col_iy.size();
-        t->col_iy.push_back(y);
-      }
+      if (!row[y]) continue;
+      c->col_of[(size_t)x * n1 + y] = (int)c->col_iy.size();
+      c->col_iy.push_back(y);
+      c->col_ix.push_back(x);
     }
   }
-  t->pcol_start.push_back((int)t->col_iy.size());
-  const int ncol = (int)t->col_iy.size();
-  t->cstart.assign((size_t)nk * (ncol + 1), 0);
-  t->t1idx.assign((size_t)nk * n_tot, 0);
-  std::vector<int> counts(ncol);
-  for (int k = 0; k < nk; ++k) {
-    const int64_t* fi = geom->fft_index + (int64_t)k * n_tot;
-    int* cs = t->cstart.data() + (size_t)k * (ncol + 1);
-    int* t1 = t->t1idx.data() + (size_t)k * n_tot;
-    std::fill(counts.begin(), counts.end(), 0);
-    int64_t col_begin = 0, col_end = -1;
-    int c = 0;
-    for (int g = 0; g < geom->n[k]; ++g) {
-      const int64_t idx = fi[g];
-      if (idx >= col_end) {
-        const int64_t col = idx / n2;
-        col_begin = col * n2;
-        col_end = col_begin + n2;
-        c = col_of[col];
-      }
-      counts[c]++;
-      t1[g] = c * n2 + (int)(idx - col_begin);
-    }
-    cs[0] = 0;
-    for (int cc = 0; cc < ncol; ++cc) cs[cc + 1] = cs[cc] + counts[cc];
-  }
-  column_intervals(nk, ncol, n_tot, n2, t);
+  c->pcol_start.push_back((int)c->col_iy.size());
   return QMX_OK;
+}
+
+// The per-k tables. The z interval of each column's points (zrange, for the
+// fixed column passes): a sphere meets a line of the box in one interval of
+// signed z, stored in increasing z as z = lo..hi (lo >= 0) or, when it
+// contains z = 0 and negative z, as 0..hi then n + lo..n - 1 (lo < 0).
+// Packed as (lo & 0xffff) | hi << 16; an empty column is (1, 0).
+// col_interval turns false if any column's points are not one such interval.
+void fill_k_tables(const qmx_geom* geom, const GeomColumns& c, KTables* out) {
+  const int n2 = geom->shape[2], nk = geom->nk, n_tot = geom->n_tot;
+  const int ncol = (int)c.col_iy.size();
+  const int T = table_threads(nk, (int64_t)nk * n_tot);
+  std::vector<char> interval(T, 1);
+  std::vector<int> run_max(T, 0);
+  run_k_ranges(T, nk, [&](int t, int k0, int k1) {
+    std::vector<int> counts(ncol);
+    bool ok = true;
+    int rmax = 0;
+    for (int k = k0; k < k1; ++k) {
+      const int64_t* fi = fft_row(geom, k);
+      int* cs = out->cstart + (size_t)k * (ncol + 1);
+      int* t1 = out->t1idx + (size_t)k * n_tot;
+      int* zr = out->zrange + (size_t)k * ncol;
+      std::fill(counts.begin(), counts.end(), 0);
+      int64_t col_begin = 0, col_end = -1;
+      int cc = 0;
+      const int nkk = geom->n[k];
+      for (int g = 0; g < nkk; ++g) {
+        const int64_t idx = fi[g];
+        if (idx >= col_end) {
+          const int64_t col = idx / n2;
+          col_begin = col * n2;
+          col_end = col_begin + n2;
+          cc = c.col_of[col];
+        }
+        counts[cc]++;
+        t1[g] = cc * n2 + (int)(idx - col_begin);
+      }
+      std::fill(t1 + nkk, t1 + n_tot, 0);
+      cs[0] = 0;
+      for (int j = 0; j < ncol; ++j) cs[j + 1] = cs[j] + counts[j];
+      for (int j = 0; j < ncol; ++j) {
+        const int a = cs[j], b = cs[j + 1], cnt = b - a, base = j * n2;
+        int lo = 1, hi = 0;
+        if (cnt > 0) {
+          const int zf = t1[a] - base, zl = t1[b - 1] - base;
+          if (zl - zf + 1 == cnt) {
+            lo = zf;
+            hi = zl;
+          } else {
+            int h = 0;  // points 0..h, then l..n2 - 1
+            while (h + 1 < cnt && t1[a + h + 1] - base == h + 1) ++h;
+            const int l = h + 1 < cnt ? t1[a + h + 1] - base : n2;
+            if (zf == 0 && zl == n2 - 1 && cnt - (h + 1) == n2 - l) {
+              lo = l - n2;
+              hi = h;
+            } else {
+              ok = false;
+            }
+          }
+        }
+        zr[j] = (lo & 0xFFFF) | (hi << 16);
+      }
+      for (int c0 = 0; c0 < ncol; c0 += kColFixedThreads)
+        rmax = std::max(rmax, cs[std::min(ncol, c0 + kColFixedThreads)] - cs[c0]);
+    }
+    interval[t] = ok;
+    run_max[t] = rmax;
+  });
+  out->col_interval = true;
+  out->col_run_max = 0;
+  for (int t = 0; t < T; ++t) {
+    out->col_interval = out->col_interval && interval[t];
+    out->col_run_max = std::max(out->col_run_max, run_max[t]);
+  }
 }
 
 template <typename T>
@@ -1790,24 +1869,30 @@ extern "C" {
 int qmx_geom_tables(const qmx_geom* geom, int64_t* sizes, int32_t* x_list,
                     int32_t* pcol_start, int32_t* col_iy, int32_t* cstart, int32_t* t1idx) {
   QMX_CHECK_ARG(sizes, "qmx_geom_tables: null sizes");
-  GeomTables t;
-  int rc = build_tables(geom, &t);
+  GeomColumns c;
+  int rc = scan_columns(geom, &c);
   if (rc) return rc;
-  sizes[0] = (int64_t)t.x_list.size();
-  sizes[1] = (int64_t)t.col_iy.size();
-  copy_out(x_list, t.x_list);
-  copy_out(pcol_start, t.pcol_start);
-  copy_out(col_iy, t.col_iy);
-  copy_out(cstart, t.cstart);
-  copy_out(t1idx, t.t1idx);
+  sizes[0] = (int64_t)c.x_list.size();
+  sizes[1] = (int64_t)c.col_iy.size();
+  copy_out(x_list, c.x_list);
+  copy_out(pcol_start, c.pcol_start);
+  copy_out(col_iy, c.col_iy);
+  if (cstart || t1idx) {
+    const size_t nk = geom->nk, ncol = c.col_iy.size();
+    std::vector<int> cs(nk * (ncol + 1)), t1(nk * geom->n_tot), zr(nk * ncol);
+    KTables kt{cs.data(), t1.data(), zr.data(), true, 0};
+    fill_k_tables(geom, c, &kt);
+    copy_out(cstart, cs);
+    copy_out(t1idx, t1);
+  }
   return QMX_OK;
 }
 
 int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
   QMX_CHECK_ARG(geom && plan_out, "qmx_plan_create: null argument");
   *plan_out = nullptr;
-  GeomTables t;
-  int rc = build_tables(geom, &t);
+  GeomColumns t;
+  int rc = scan_columns(geom, &t);
   if (rc) return rc;
   auto* p = new qmx_plan();
   std::memset(p, 0, sizeof(qmx_plan));
@@ -1838,18 +1923,30 @@ int qmx_plan_create(const qmx_geom* geom, qmx_plan** plan_out) {
       p->n_vslots = std::max(p->n_vslots, vslot[k] + 1);
     }
   }
-  std::vector<int> col_ix(t.col_iy.size());
-  for (size_t xa = 0; xa < t.x_list.size(); ++xa)
-    for (int c = t.pcol_start[xa]; c < t.pcol_start[xa + 1]; ++c) col_ix[c] = t.x_list[xa];
-  p->col_interval = t.col_interval;
-  p->col_run_max = t.col_run_max;
-  if ((rc = upload_tables(p, {&t.x_list, &t.pcol_start, &t.col_iy, &t.cstart, &t.t1idx, &nvec,
-                              &vslot, &col_ix, &t.zrange},
-                          {&p->d_x_list, &p->d_pcol_start, &p->d_col_iy, &p->d_cstart,
-                           &p->d_t1idx, &p->d_n, &p->d_vslot, &p->d_col_ix, &p->d_zrange}))) {
+  const size_t nk = geom->nk, ncol = t.col_iy.size();
+  KTables kt{nullptr, nullptr, nullptr, true, 0};
+  rc = upload_tables(p,
+                     {{t.x_list.data(), t.x_list.size(), &p->d_x_list},
+                      {t.pcol_start.data(), t.pcol_start.size(), &p->d_pcol_start},
+                      {t.col_iy.data(), ncol, &p->d_col_iy},
+                      {nullptr, nk * (ncol + 1), &p->d_cstart},
+                      {nullptr, nk * geom->n_tot, &p->d_t1idx},
+                      {nvec.data(), nk, &p->d_n},
+                      {vslot.data(), nk, &p->d_vslot},
+                      {t.col_ix.data(), ncol, &p->d_col_ix},
+                      {nullptr, nk * ncol, &p->d_zrange}},
+                     [&](const std::vector<int*>& h) {
+                       kt.cstart = h[3];
+                       kt.t1idx = h[4];
+                       kt.zrange = h[8];
+                       fill_k_tables(geom, t, &kt);
+                     });
+  if (rc) {
     free_plan(p);
     return rc;
   }
+  p->col_interval = kt.col_interval;
+  p->col_run_max = kt.col_run_max;
   hipError_t e = hipGetDevice(&p->device);
   if (e != hipSuccess) {
     free_plan(p);

@@ -2,6 +2,8 @@
 //
 // All are HBM-bound streams over (k, band, g) rows: one workgroup per row,
 // accumulating in double with a wave64 shuffle reduction.
+#include <type_traits>
+
 #include "qmx_common.h"
 
 using namespace qmx;
@@ -242,6 +244,76 @@ __global__ void __launch_bounds__(kRowThreads)
   }
 }
 
+// Row blocks between device buffers in one launch: block i writes
+// dst[i0 * dst_s0 + i1 * dst_s1 + c] for i0 < n0, i1 < n1, c < dst_cols,
+// from src at the same (i0, i1) with src strides for c < src_cols and zero
+// beyond (or everywhere when src is null). mode 0 copies 8-byte words
+// (16-byte ones when every pointer, stride and width allows); mode 1
+// converts doubles to floats (counts and strides in reals of each side).
+// Workgroups are dealt to blocks in order: block i owns first_wg onwards,
+// rows_per_wg rows of its n0 x n1 rows each.
+struct CopyBlock {
+  const void* src;
+  void* dst;
+  int64_t n0, n1, src_cols, dst_cols, src_s0, src_s1, dst_s0, dst_s1, rows_per_wg, first_wg,
+      mode, pad;
+};
+static_assert(sizeof(CopyBlock) == 14 * 8, "CopyBlock is 14 int64 fields");
+
+template <typename S, typename D>
+__device__ __forceinline__ D convert_unit(S v) {
+  if constexpr (std::is_same<S, D>::value)
+    return v;
+  else
+    return (D)v;
+}
+
+template <typename S, typename D>
+__device__ __forceinline__ void copy_rows(const CopyBlock& b, int64_t r0, int64_t r1, int u) {
+  const int64_t sc = b.src_cols / u, dc = b.dst_cols / u;
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t i0 = r / b.n1, i1 = r - i0 * b.n1;
+    const S* s = b.src ? static_cast<const S*>(b.src) + (i0 * b.src_s0 + i1 * b.src_s1) / u
+                       : nullptr;
+    D* o = static_cast<D*>(b.dst) + (i0 * b.dst_s0 + i1 * b.dst_s1) / u;
+    for (int64_t c = threadIdx.x; c < dc; c += kRowThreads) {
+      D v;
+      if (s && c < sc)
+        v = convert_unit<S, D>(s[c]);
+      else
+        v = D{};
+      o[c] = v;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kRowThreads)
+    k_copy_blocks(const CopyBlock* __restrict__ blocks, int n) {
+  const int64_t wg = blockIdx.x;
+  int lo = 0, hi = n - 1;  // the last block with first_wg <= wg
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blocks[mid].first_wg <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const CopyBlock b = blocks[lo];
+  const int64_t rows = b.n0 * b.n1;
+  const int64_t r0 = (wg - b.first_wg) * b.rows_per_wg;
+  const int64_t r1 = r0 + b.rows_per_wg < rows ? r0 + b.rows_per_wg : rows;
+  if (b.mode == 1) {
+    copy_rows<double, float>(b, r0, r1, 1);
+    return;
+  }
+  const bool wide = ((b.src_cols | b.dst_cols | b.src_s0 | b.src_s1 | b.dst_s0 | b.dst_s1) & 1) == 0 &&
+                    ((reinterpret_cast<uintptr_t>(b.src) | reinterpret_cast<uintptr_t>(b.dst)) & 15) == 0;
+  if (wide)
+    copy_rows<ulonglong2, ulonglong2>(b, r0, r1, 2);
+  else
+    copy_rows<unsigned long long, unsigned long long>(b, r0, r1, 1);
+}
+
 }  // namespace
 
 #define QMX_DISPATCH(dtype, KERNEL_CALL_F32, KERNEL_CALL_F64)          \
@@ -405,6 +477,16 @@ int qmx_axpby(int dtype, int64_t nk, int64_t nb, int64_t n_tot, const double* a,
                                   cf_ldk, cf_ldb, static_cast<const double2*>(X),
                                   static_cast<const double2*>(Y), static_cast<const double2*>(Z),
                                   static_cast<double2*>(out), ldb, ldk, (int)n_tot));
+  return QMX_OK;
+}
+
+int qmx_copy_blocks(const void* blocks, int64_t n, int64_t n_wg, void* stream) {
+  QMX_CHECK_ARG(blocks && n >= 1 && n < (1ll << 31), "qmx_copy_blocks: bad block list");
+  QMX_CHECK_ARG(n_wg >= 1 && n_wg < (1ll << 31), "qmx_copy_blocks: bad workgroup count");
+  hipLaunchKernelGGL(k_copy_blocks, dim3((unsigned)n_wg), dim3(kRowThreads), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const CopyBlock*>(blocks),
+                     (int)n);
+  QMX_LAUNCHED();
   return QMX_OK;
 }
 

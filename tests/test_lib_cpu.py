@@ -140,3 +140,47 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_geom_tables_many_cells():
+    """The index tables of a plan over the k-points of many cells of one box
+    (large enough that the build splits the k-points over threads), given
+    as per-k row pointers or as one padded matrix: identical, and each
+    sphere point's column and z reproduce its box index."""
+    from qimpy_amd.batched import CellJob
+
+    jobs = []
+    i = 0
+    while len(jobs) < 12:
+        spec = systems.make_cfg2_cell(i, cfg=3)
+        i += 1
+        if tuple(spec.shape) == (48, 48, 48):
+            jobs.append(CellJob(spec, job_id=i))
+    shape = jobs[0].shape
+    Gb = max(j.n_tot for j in jobs)
+    n = np.concatenate([j.n for j in jobs])
+    fi = np.zeros((len(n), Gb), dtype=np.int64)
+    a = 0
+    for j in jobs:
+        fi[a : a + j.K, : j.n_tot] = j.fft_index
+        a += j.K
+    assert len(n) * Gb > 2 << 18  # several threads
+    T = _lib.geom_tables(shape, n, fi)
+    g, keep = _lib.make_geom_rows(shape, n, [j.fft_index for j in jobs], Gb)
+    sizes = np.zeros(2, dtype=np.int64)
+    out = {k: np.zeros_like(T[k]) for k in ("x_list", "pcol_start", "col_iy", "cstart", "t1idx")}
+    p = lambda x: x.ctypes.data_as(_lib.c_i32p)
+    _lib.call("qmx_geom_tables", _lib.ct.byref(g), sizes.ctypes.data_as(_lib.c_i64p),
+              p(out["x_list"]), p(out["pcol_start"]), p(out["col_iy"]), p(out["cstart"]),
+              p(out["t1idx"]))
+    assert (sizes[0], sizes[1]) == (T["nxa"], T["ncol"])
+    for k in out:
+        assert np.array_equal(out[k], T[k]), k
+    nx, ny, nz = shape
+    col_ix = np.repeat(T["x_list"], np.diff(T["pcol_start"]))
+    for k in range(len(n)):
+        t = T["t1idx"][k]
+        col, z = t[: n[k]] // nz, t[: n[k]] % nz
+        assert np.array_equal((col_ix[col] * ny + T["col_iy"][col]) * nz + z, fi[k, : n[k]])
+        assert np.all(t[n[k] :] == 0)
+        assert np.array_equal(np.diff(T["cstart"][k]), np.bincount(col, minlength=T["ncol"]))
