@@ -179,6 +179,16 @@ int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k
                    double dG, double G_max, double scale, void* out, int64_t ldp, int64_t ldk,
                    void* stream);
 
+/* The same projectors with Gaussian radial functions
+ * R_i(G) = exp(-rc_i^2 G^2 / 2) (rc: host double [n_rf]) in place of the
+ * splines: the analytic radials of the synthetic cells, built for each job
+ * the batched engine admits. */
+int qmx_gaussian_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
+                            const int32_t* n, const double* Gbasis, int64_t n_atoms,
+                            const double* positions, int64_t n_proj, const int32_t* l,
+                            const int32_t* m, const int32_t* i_rf, int64_t n_rf, const double* rc,
+                            double scale, void* out, int64_t ldp, int64_t ldk, void* stream);
+
 /* Gradient operator along Cartesian direction dir (Basis.apply_gradient,
  * dft/electrons/_basis_ops.py:15-26, with get_gradient = i (G + k),
  * _basis.py:240-254), scaled by alpha:

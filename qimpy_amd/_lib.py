@@ -86,6 +86,8 @@ SIGNATURES = {
     "qmx_density_cross_workspace_bytes": (i64, [vp, i32, i64]),
     "qmx_projectors": (i32, [i64, i64, vp, vp, vp, vp, i64, vp, i64, vp, vp, vp, i64, i64, vp,
                              ct.c_double, ct.c_double, ct.c_double, vp, i64, i64, vp]),
+    "qmx_gaussian_projectors": (i32, [i64, i64, vp, vp, vp, vp, i64, vp, i64, vp, vp, vp, i64, vp,
+                                      ct.c_double, vp, i64, i64, vp]),
     "qmx_radial_transform": (i32, [i64, i64, vp, vp, vp, vp, i64, ct.c_double, vp, vp]),
     "qmx_collect_density_cross": (i32, [vp, i64, i64, i32, vp, vp, i64, i64, i64, vp, i64, vp, i32,
                                         vp, i64, vp]),

@@ -175,7 +175,7 @@ class CellJob:
             raise ValueError(f"n_bands + n_bands_extra = {self.n_bands_max} exceeds"
                              f" min(n_basis)/2 = {int(self.n.min()) // 2} in Davidson")
         self.fft_index = np.zeros((self.K, self.n_tot), dtype=np.int64)
-        self.iG_np = np.zeros((self.K, self.n_tot, 3), dtype=np.int64)
+        self.iG_np = np.zeros((self.K, self.n_tot, 3), dtype=np.int32)
         for i, (lin, sel) in enumerate(tables):
             self.fft_index[i, : len(lin)] = lin
             self.iG_np[i, : len(lin)] = sel
@@ -188,7 +188,7 @@ class CellJob:
         # Host copies of the device tables, pinned here (by the thread that
         # builds the job) so that admission copies them without waiting.
         host = dict(k=np.asarray(spec.k, dtype=np.float64).reshape(-1, 3), iG=self.iG_np,
-                    GbT=np.asarray(spec.Gbasis, dtype=np.float64).T, n=self.n,
+                    GbT=np.asarray(spec.Gbasis, dtype=np.float64).T, n=self.n.astype(np.int32),
                     V=np.asarray(spec.V, dtype=np.float64).reshape(-1),
                     pos=np.asarray(spec.positions, dtype=np.float64),
                     D=np.asarray(spec.D, dtype=np.float64))

@@ -50,6 +50,7 @@ struct ProjDesc {
   double norm[kProjMax];  // N_lm, sqrt 2 included for m != 0
   double w[6][6];         // w_j(f) = sum_p w[j][p] f^p
   double G[9];            // Cartesian (G + k)_a = sum_b G[3 a + b] (iG + k)_b
+  double gauss[kProjMax];  // Gaussian radials: R(G) = exp(gauss[p] G^2)
 };
 
 __device__ __forceinline__ double solid_harmonic(int l, int m, double x, double y, double z,
@@ -72,6 +73,7 @@ __device__ __forceinline__ double solid_harmonic(int l, int m, double x, double 
   return q1 * (m > 0 ? cr : (m < 0 ? ci : 1.0));
 }
 
+template <bool GAUSS>
 __global__ void __launch_bounds__(kProjThreads)
     k_projectors(const int* __restrict__ iG, const double* __restrict__ kvec,
                  const int* __restrict__ n, const double* __restrict__ pos, int n_atoms, int apg,
@@ -96,27 +98,33 @@ __global__ void __launch_bounds__(kProjThreads)
   double Gc[3];
   for (int a = 0; a < 3; ++a) Gc[a] = d.G[3 * a] * f[0] + d.G[3 * a + 1] * f[1] + d.G[3 * a + 2] * f[2];
   const double r2 = Gc[0] * Gc[0] + Gc[1] * Gc[1] + Gc[2] * Gc[2];
-  const double t = sqrt(r2) * inv_dG;
-  int i = (int)floor(t);
-  double fr = t - i;
-  if (i > n_coeff - 6) {  // beyond the table (the host checks G_max): clamp, stay in bounds
-    i = n_coeff - 6;
-    fr = 1.0;
-  }
-  double w[6];
+  if constexpr (GAUSS) {
+    for (int p = 0; p < np; ++p)
+      val[p][tid] = scale * d.norm[p] * exp(d.gauss[p] * r2) *
+                    solid_harmonic(d.l[p], d.m[p], Gc[0], Gc[1], Gc[2], r2);
+  } else {
+    const double t = sqrt(r2) * inv_dG;
+    int i = (int)floor(t);
+    double fr = t - i;
+    if (i > n_coeff - 6) {  // beyond the table (the host checks G_max): clamp, stay in bounds
+      i = n_coeff - 6;
+      fr = 1.0;
+    }
+    double w[6];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    double s = d.w[j][5];
+    for (int j = 0; j < 6; ++j) {
+      double s = d.w[j][5];
 #pragma unroll
-    for (int q = 4; q >= 0; --q) s = s * fr + d.w[j][q];
-    w[j] = s;
-  }
-  for (int p = 0; p < np; ++p) {
-    const double* cf = coeff + (int64_t)i * n_rf + d.i_rf[p];
-    double R = 0.0;
+      for (int q = 4; q >= 0; --q) s = s * fr + d.w[j][q];
+      w[j] = s;
+    }
+    for (int p = 0; p < np; ++p) {
+      const double* cf = coeff + (int64_t)i * n_rf + d.i_rf[p];
+      double R = 0.0;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) R += w[j] * cf[(int64_t)j * n_rf];
-    val[p][tid] = scale * d.norm[p] * R * solid_harmonic(d.l[p], d.m[p], Gc[0], Gc[1], Gc[2], r2);
+      for (int j = 0; j < 6; ++j) R += w[j] * cf[(int64_t)j * n_rf];
+      val[p][tid] = scale * d.norm[p] * R * solid_harmonic(d.l[p], d.m[p], Gc[0], Gc[1], Gc[2], r2);
+    }
   }
   for (int a = a0; a < a1; ++a) {
     const double dot = f[0] * pos[3 * a] + f[1] * pos[3 * a + 1] + f[2] * pos[3 * a + 2];
@@ -223,14 +231,17 @@ void blip_polynomials(double w[6][6]) {
 
 }  // namespace
 
-extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
-                              const int32_t* n, const double* Gbasis, int64_t n_atoms,
-                              const double* positions, int64_t n_proj, const int32_t* l,
-                              const int32_t* m, const int32_t* i_rf, int64_t n_rf,
-                              int64_t n_coeff, const double* coeff, double dG, double G_max,
-                              double scale, void* out, int64_t ldp, int64_t ldk, void* stream) {
+// Both projector entry points: spline radials (rc null) or Gaussian ones
+// R_i(G) = exp(-rc_i^2 G^2 / 2) (coeff null).
+static int projectors_impl(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
+                           const int32_t* n, const double* Gbasis, int64_t n_atoms,
+                           const double* positions, int64_t n_proj, const int32_t* l,
+                           const int32_t* m, const int32_t* i_rf, int64_t n_rf, int64_t n_coeff,
+                           const double* coeff, double dG, double G_max, const double* rc,
+                           double scale, void* out, int64_t ldp, int64_t ldk, void* stream) {
   using qmx::set_error;
-  if (!iG || !k || !n || !Gbasis || !l || !m || !i_rf || !coeff || !out ||
+  const bool gauss = rc != nullptr;
+  if (!iG || !k || !n || !Gbasis || !l || !m || !i_rf || (!gauss && !coeff) || !out ||
       (n_atoms > 0 && !positions)) {
     set_error("qmx_projectors: null argument");
     return QMX_ERR_ARG;
@@ -244,7 +255,8 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
     set_error("qmx_projectors: more than 32 projectors per atom");
     return QMX_ERR_ARG;
   }
-  if (!(dG > 0.0) || !(G_max >= 0.0) || n_coeff < 6 || std::floor(G_max / dG) + 5 >= n_coeff) {
+  if (!gauss &&
+      (!(dG > 0.0) || !(G_max >= 0.0) || n_coeff < 6 || std::floor(G_max / dG) + 5 >= n_coeff)) {
     set_error("qmx_projectors: the radial table does not reach G_max");
     return QMX_ERR_ARG;
   }
@@ -266,6 +278,14 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
     d.i_rf[p] = (signed char)i_rf[p];
     d.norm[p] = std::sqrt((2 * lp + 1) / (4.0 * M_PI) * factorial(lp - am) / factorial(lp + am)) *
                 (mp != 0 ? std::sqrt(2.0) : 1.0);
+    if (gauss) {
+      const double r = rc[i_rf[p]];
+      if (!(r >= 0.0) || !std::isfinite(r)) {
+        set_error("qmx_gaussian_projectors: bad Gaussian width");
+        return QMX_ERR_ARG;
+      }
+      d.gauss[p] = -0.5 * r * r;
+    }
   }
   blip_polynomials(d.w);
   for (int i = 0; i < 9; ++i) d.G[i] = Gbasis[i];
@@ -278,13 +298,46 @@ extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, cons
   dim3 grid((unsigned)gx, (unsigned)nk, (unsigned)gz);
   hipEvent_t ev;
   qmx::prof_begin(st, &ev);
-  hipLaunchKernelGGL(k_projectors, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
-                     (int)n_atoms, (int)apg, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
-                     static_cast<double2*>(out), ldp, ldk, (int)n_tot);
+  if (gauss)
+    hipLaunchKernelGGL(k_projectors<true>, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
+                       (int)n_atoms, (int)apg, nullptr, 0, (int)n_rf, 0.0, scale, d,
+                       static_cast<double2*>(out), ldp, ldk, (int)n_tot);
+  else
+    hipLaunchKernelGGL(k_projectors<false>, grid, dim3(kProjThreads), 0, st, iG, k, n, positions,
+                       (int)n_atoms, (int)apg, coeff, (int)n_coeff, (int)n_rf, 1.0 / dG, scale, d,
+                       static_cast<double2*>(out), ldp, ldk, (int)n_tot);
   QMX_LAUNCHED();
   // compulsory bytes: the complex128 output
   qmx::prof_end(st, ev, QMX_KERNEL_PROJ + QMX_KERNEL_C128, 16.0 * nk * n_atoms * n_proj * n_tot);
   return QMX_OK;
+}
+
+extern "C" int qmx_projectors(int64_t nk, int64_t n_tot, const int32_t* iG, const double* k,
+                              const int32_t* n, const double* Gbasis, int64_t n_atoms,
+                              const double* positions, int64_t n_proj, const int32_t* l,
+                              const int32_t* m, const int32_t* i_rf, int64_t n_rf,
+                              int64_t n_coeff, const double* coeff, double dG, double G_max,
+                              double scale, void* out, int64_t ldp, int64_t ldk, void* stream) {
+  if (!coeff) {
+    qmx::set_error("qmx_projectors: null argument");
+    return QMX_ERR_ARG;
+  }
+  return projectors_impl(nk, n_tot, iG, k, n, Gbasis, n_atoms, positions, n_proj, l, m, i_rf,
+                         n_rf, n_coeff, coeff, dG, G_max, nullptr, scale, out, ldp, ldk, stream);
+}
+
+extern "C" int qmx_gaussian_projectors(int64_t nk, int64_t n_tot, const int32_t* iG,
+                                       const double* k, const int32_t* n, const double* Gbasis,
+                                       int64_t n_atoms, const double* positions, int64_t n_proj,
+                                       const int32_t* l, const int32_t* m, const int32_t* i_rf,
+                                       int64_t n_rf, const double* rc, double scale, void* out,
+                                       int64_t ldp, int64_t ldk, void* stream) {
+  if (!rc) {
+    qmx::set_error("qmx_gaussian_projectors: null argument");
+    return QMX_ERR_ARG;
+  }
+  return projectors_impl(nk, n_tot, iG, k, n, Gbasis, n_atoms, positions, n_proj, l, m, i_rf,
+                         n_rf, 0, nullptr, 0.0, 0.0, rc, scale, out, ldp, ldk, stream);
 }
 
 extern "C" int qmx_radial_transform(int64_t n_f, int64_t n_r, const double* r, const double* w,

@@ -38,6 +38,8 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
     n_atoms = positions.shape[0]
     if n_atoms == 0 or not proj_rc or not proj_l:
         return torch.zeros((K, 0, G), dtype=torch.complex128, device=dev)
+    if dev.type == "cuda":
+        return _gaussian_projectors(positions, proj_rc, proj_l, iG, k, Gbasis, volume, n)
     iGk = iG.to(torch.float64) + k[:, None, :]
     Gb = torch.from_numpy(np.ascontiguousarray(np.asarray(Gbasis, dtype=np.float64).T))
     Gb = Gb.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else Gb
@@ -64,6 +66,37 @@ def build_projectors(positions: torch.Tensor, proj_rc, proj_l, iG: torch.Tensor,
     proj = proj.reshape(K, G, -1).transpose(1, 2).contiguous()  # (K, P, G)
     pad = torch.arange(G, device=dev)[None, None, :] >= n.to(dev)[:, None, None]
     return proj.masked_fill_(pad, 0.0)
+
+
+def _gaussian_projectors(positions, proj_rc, proj_l, iG, k, Gbasis, volume, n) -> torch.Tensor:
+    """build_projectors on the device in one launch (qmx_gaussian_projectors):
+    the same projectors, atom -> radial -> (l, m) with m = -1, 0, 1 the
+    (y, z, x) harmonics of l = 1."""
+    K, G = iG.shape[0], iG.shape[1]
+    A = int(positions.shape[0])
+    l_p, m_p, r_p = [], [], []
+    for r in range(len(proj_rc)):
+        for l in proj_l:
+            if l > 1:
+                raise NotImplementedError("Gaussian projectors with l > 1 are not on this path")
+            for m in range(-l, l + 1):
+                l_p.append(l)
+                m_p.append(m)
+                r_p.append(r)
+    l_p, m_p, r_p = (np.array(x, dtype=np.int32) for x in (l_p, m_p, r_p))
+    rc_h = np.array(proj_rc, dtype=np.float64)
+    Gb = np.ascontiguousarray(np.asarray(Gbasis, dtype=np.float64))
+    iG32 = iG if iG.dtype == torch.int32 and iG.is_contiguous() else iG.to(torch.int32).contiguous()
+    n32 = n if n.dtype == torch.int32 else n.to(torch.int32)
+    kk = k.to(torch.float64).contiguous()
+    pos = positions.to(device=iG.device, dtype=torch.float64).contiguous()
+    out = torch.empty((K, A * len(l_p), G), dtype=torch.complex128, device=iG.device)
+    c_i32 = lambda a: a.ctypes.data_as(_lib.c_i32p)
+    _lib.call("qmx_gaussian_projectors", K, G, iG32.data_ptr(), kk.data_ptr(), n32.data_ptr(),
+              Gb.ctypes.data_as(_lib.c_f64p), A, pos.data_ptr(), len(l_p), c_i32(l_p), c_i32(m_p),
+              c_i32(r_p), len(rc_h), rc_h.ctypes.data_as(_lib.c_f64p), 1.0 / np.sqrt(volume),
+              out.data_ptr(), out.stride(1), out.stride(0), rc.stream_ptr())
+    return out
 
 
 DG_DEFAULT = 0.02  #: node spacing of the radial splines (math/_radial_function.py:22)

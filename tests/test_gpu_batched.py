@@ -161,3 +161,21 @@ def test_budget_and_cancellation(gpu, dense_refs, dtype):
         assert err < TOL[dtype]
         assert np.abs(r.eig - dense_refs[i]).max() < TOL[dtype]
     assert solver.hbm_in_use == 0
+
+
+@pytest.mark.parametrize("seed,n_atoms", [(21, 3), (22, 1), (23, 5)])
+def test_projectors_kernel_match_oracle(gpu, seed, n_atoms):
+    """qmx_gaussian_projectors (build_projectors on device tensors) against
+    the oracle's projectors, which pin the torch formula on the CPU."""
+    from oracle.hamiltonian import projectors as o_projectors
+    from qimpy_amd.ions import build_projectors
+
+    spec = systems.small_cell(seed=seed, n_atoms=n_atoms)
+    ob = OBasis(spec)
+    dev = torch.device("cuda")
+    got = build_projectors(torch.tensor(spec.positions, device=dev), spec.proj_rc, spec.proj_l,
+                           torch.tensor(ob.iG, device=dev), torch.tensor(ob.k, device=dev),
+                           spec.Gbasis, spec.volume, torch.tensor(ob.n, device=dev)).cpu().numpy()
+    ref = o_projectors(ob)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
