@@ -111,24 +111,62 @@ __device__ __forceinline__ void dft_comp(C2 (&v)[R1 * R2], const C2* __restrict_
   }
 }
 
-// Radices 2..5 directly, composite radices 6, 8, 9, 10, 12, 15, 16 through
-// dft_comp with the twiddle table.
+// Coprime R1, R2 (in 2..5): the prime-factor (Good-Thomas) form, no
+// twiddles. Input n = (R2 n1 + R1 n2) mod R, output
+// k = (R2 (R2^-1 mod R1) k1 + R1 (R1^-1 mod R2) k2) mod R, so that
+// w_R^(n k) = w_R1^(n1 k1) w_R2^(n2 k2): R2 DFTs of size R1, then R1 of
+// size R2, all index maps compile-time constants.
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+template <typename C2, int R1, int R2, int S>
+__device__ __forceinline__ void dft_pfa(C2 (&v)[R1 * R2]) {
+  constexpr int R = R1 * R2;
+  static_assert(gcd_int(R1, R2) == 1, "prime-factor DFT needs coprime factors");
+  constexpr int E1 = (R2 * inv_mod(R2, R1)) % R, E2 = (R1 * inv_mod(R1, R2)) % R;
+  C2 y[R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    C2 a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[(R2 * n1 + R1 * n2) % R];
+    dft<C2, R1, S>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[k1 * R2 + n2] = a[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    C2 b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[k1 * R2 + n2];
+    dft<C2, R2, S>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[(E1 * k1 + E2 * k2) % R] = b[k2];
+  }
+}
+
+// Radices 2..5 directly; composite radices 6, 10, 12, 15 (coprime factors)
+// in prime-factor form; 8, 9, 16 through dft_comp with the twiddle table.
 template <typename C2, int R, int S>
 __device__ __forceinline__ void dft_any(C2 (&v)[R], const C2* __restrict__ tw, int n) {
   if constexpr (R <= 5) {
     dft<C2, R, S>(v);
   } else if constexpr (R == 6) {
-    dft_comp<C2, 2, 3, S>(v, tw, n / R);
+    dft_pfa<C2, 2, 3, S>(v);
   } else if constexpr (R == 8) {
     dft_comp<C2, 4, 2, S>(v, tw, n / R);
   } else if constexpr (R == 9) {
     dft_comp<C2, 3, 3, S>(v, tw, n / R);
   } else if constexpr (R == 10) {
-    dft_comp<C2, 2, 5, S>(v, tw, n / R);
+    dft_pfa<C2, 2, 5, S>(v);
   } else if constexpr (R == 12) {
-    dft_comp<C2, 4, 3, S>(v, tw, n / R);
+    dft_pfa<C2, 4, 3, S>(v);
   } else if constexpr (R == 15) {
-    dft_comp<C2, 3, 5, S>(v, tw, n / R);
+    dft_pfa<C2, 3, 5, S>(v);
   } else if constexpr (R == 16) {
     dft_comp<C2, 4, 4, S>(v, tw, n / R);
   }

@@ -691,11 +691,33 @@ __global__ void __launch_bounds__(256) k_fill_zero(double* __restrict__ x, int64
     x[i] = 0.0;
 }
 
+// The order of x and y within a plane that a fixed plane kernel asks of its
+// potential (plane_fixed_space_order): the rho order n' = B ((n ia) mod A)
+// + (n ib) mod B for a prime-factor size n = A B (ia = B^-1 mod A,
+// ib = A^-1 mod B); A = 0 for the natural order.
+struct PlaneOrder {
+  int n, A, B, ia, ib;
+  __device__ int rho(int v) const { return B * ((v * ia) % A) + (v * ib) % B; }
+};
+
+PlaneOrder plane_order(bool fixed, int n) {
+  PlaneOrder po{n, 0, 1, 0, 0};
+  int A, B;
+  if (fixed && plane_fixed_space_order(n, &A, &B)) {
+    po.A = A;
+    po.B = B;
+    po.ia = inv_mod(B, A);
+    po.ib = inv_mod(A, B);
+  }
+  return po;
+}
+
 // Vt[s][z][xy] = V[s][xy][z]: each potential box to z-major order (32 x 32
-// tiles through LDS, both sides coalesced).
+// tiles through LDS, both sides coalesced), x and y in the order po asks.
 template <typename R>
 __global__ void __launch_bounds__(256)
-    k_potential_zmajor(const R* __restrict__ V, R* __restrict__ Vt, int nxy, int nz) {
+    k_potential_zmajor(const R* __restrict__ V, R* __restrict__ Vt, int nxy, int nz,
+                       PlaneOrder po) {
   __shared__ R tile[32][33];
   const int s = blockIdx.z, xy0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -706,9 +728,15 @@ __global__ void __launch_bounds__(256)
     if (xy < nxy && zz < nz) tile[r][tx] = Vs[(int64_t)xy * nz + zz];
   }
   __syncthreads();
+  const int xy = xy0 + tx;
+  int dst = xy;  // (x, y) in the rho order of the plane kernel where it asks for it
+  if (po.A > 0 && xy < nxy) {
+    const int x = xy / po.n, y = xy - x * po.n;
+    dst = po.rho(x) * po.n + po.rho(y);
+  }
   for (int r = ty; r < 32; r += 8) {
-    const int zz = z0 + r, xy = xy0 + tx;
-    if (xy < nxy && zz < nz) Vts[(int64_t)zz * nxy + xy] = tile[tx][r];
+    const int zz = z0 + r;
+    if (xy < nxy && zz < nz) Vts[(int64_t)zz * nxy + dst] = tile[tx][r];
   }
 }
 
@@ -997,18 +1025,6 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
   const double t1_elems = (double)nk * ncol * nz, t2_elems = (double)nk * nxa * ny * nz;
   hipEvent_t ev = nullptr, ev_apply = nullptr;
   prof_begin(st, &ev_apply);
-  const R* Vt = nullptr;
-  if (plane) {
-    // z-major copy of every potential box at the start of the workspace
-    R* vt = static_cast<R*>(ws);
-    const int nxy = nx * ny;
-    dim3 gT((nxy + 31) / 32, (nz + 31) / 32, p->n_vslots);
-    prof_begin(st, &ev);
-    hipLaunchKernelGGL((k_potential_zmajor<R>), gT, dim3(256), 0, st, V, vt, nxy, nz);
-    QMX_LAUNCHED();
-    prof_end(st, ev, idoff + QMX_KERNEL_VZMAJOR, 2.0 * p->n_vslots * (double)p->N * br);
-    Vt = vt;
-  }
   const size_t shP = plane_lds_bytes(p, sizeof(C2));
   int ntP = plane ? plane_threads(p) : 0;
   bool fixed_ok = p->local_path != 3;  // the specialised plane kernel where it exists
@@ -1024,6 +1040,21 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
         fixed_ok = false;
       }
     }
+  }
+  const bool plane_fixed =
+      plane && fixed_ok && ntP == kPlaneThreadsSmall && plane_fixed_available(nx, ny, ncol);
+  const R* Vt = nullptr;
+  if (plane) {
+    // z-major copy of every potential box at the start of the workspace
+    R* vt = static_cast<R*>(ws);
+    const int nxy = nx * ny;
+    dim3 gT((nxy + 31) / 32, (nz + 31) / 32, p->n_vslots);
+    prof_begin(st, &ev);
+    hipLaunchKernelGGL((k_potential_zmajor<R>), gT, dim3(256), 0, st, V, vt, nxy, nz,
+                       plane_order(plane_fixed, nx));
+    QMX_LAUNCHED();
+    prof_end(st, ev, idoff + QMX_KERNEL_VZMAJOR, 2.0 * p->n_vslots * (double)p->N * br);
+    Vt = vt;
   }
   if (plane) {
     const int rc = ntP == kPlaneThreadsSmall
@@ -1087,7 +1118,7 @@ int apply_local_impl(const qmx_plan* p, int k0, int nk, const void* Cv, int64_t 
       const int maxeP = ntP == kPlaneThreadsSmall ? kPlaneMaxESmall : kPlaneMaxE;
       const Radices rxc = combine_radices(p->rad[0], ny, ntP, maxeP);
       const Radices ryc = combine_radices(p->rad[1], nxa, ntP, maxeP);
-      if (fixed_ok && ntP == kPlaneThreadsSmall && plane_fixed_available(nx, ny, ncol)) {
+      if (plane_fixed) {
         const int rc = launch_plane_fixed(sizeof(R) == 8, nx, gP, shP, st, T1, Vt, vslot,
                                           p->d_col_ix, p->d_col_iy, p->d_x_list, tw_of<C2>(p, 0),
                                           tw_of<C2>(p, 1), ncol, nxa, cur);

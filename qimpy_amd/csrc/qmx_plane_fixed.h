@@ -17,6 +17,12 @@ int launch_plane_fixed(bool c128, int n, dim3 grid, size_t lds, hipStream_t st, 
                        const int* x_list, const void* twx, const void* twy, int ncol, int nxa,
                        int nb);
 
+// For a plane size whose fixed kernel runs its transforms in prime-factor
+// form (n = A B, gcd(A, B) = 1): A and B, true. The kernel then wants each
+// potential box with x and y in the rho order: value (x, y) at
+// x' ny + y', n' = B ((n (B^-1 mod A)) mod A) + (n (A^-1 mod B)) mod B.
+bool plane_fixed_space_order(int n, int* A, int* B);
+
 // The column passes with the z length fixed at compile time
 // (qmx_cols_fixed.hip): one thread per active column, kColFixedThreads
 // columns per workgroup, grid (column blocks, bands, k-points).

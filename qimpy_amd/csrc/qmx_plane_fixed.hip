@@ -24,13 +24,13 @@ constexpr int kColPer = 4;   // active columns staged per thread (ncol <= 1024)
 constexpr int kVPer = 16;    // potential values per thread (planes <= 4096 points)
 
 // First and second radix of a length-N transform (composite radices run in
-// registers, dft_any).
+// registers, dft_any); coprime pairs take the prime-factor passes.
 template <int N> struct Split;
 template <> struct Split<24> { static constexpr int A = 8, B = 3; };
 template <> struct Split<30> { static constexpr int A = 10, B = 3; };
 template <> struct Split<32> { static constexpr int A = 8, B = 4; };
 template <> struct Split<36> { static constexpr int A = 9, B = 4; };
-template <> struct Split<40> { static constexpr int A = 10, B = 4; };
+template <> struct Split<40> { static constexpr int A = 8, B = 5; };
 template <> struct Split<48> { static constexpr int A = 16, B = 3; };
 template <> struct Split<60> { static constexpr int A = 15, B = 4; };
 template <> struct Split<64> { static constexpr int A = 16, B = 4; };
@@ -91,12 +91,70 @@ __device__ __forceinline__ void fixed_pass(C2* buf, const C2* __restrict__ tw, i
   __syncthreads();
 }
 
+// Prime-factor (Good-Thomas) form of a length N = A B transform with
+// gcd(A, B) = 1, which needs no twiddles between its passes. The line is
+// held permuted: space index n = (B n1 + A n2) mod N at slot B n1 + n2 (the
+// rho order), frequency index k at slot B (k mod A) + (k mod B) (the gamma
+// order). Either direction is then pass 1, DFT_A over slots j + B r
+// (butterfly j < B), and pass 2, DFT_B over slots B j + r (butterfly
+// j < A), each butterfly in place on its own slots (no barrier between a
+// pass's reads and writes): gamma -> rho for the inverse transform,
+// rho -> gamma for the forward one. Lines and threads as in fixed_pass.
+template <int N>
+struct Pfa {
+  static constexpr int A = Split<N>::A, B = Split<N>::B;
+  // (24 = 8 x 3 keeps the twiddled passes: its radix-8 butterflies keep
+  // their internal twiddles, and the gamma index maps cost more VALU than
+  // the prime-factor passes save.)
+  static constexpr bool on = gcd_int(A, B) == 1 && N != 24;
+  __device__ static int gamma(int k) { return (k % A) * B + k % B; }
+};
+
+template <typename C2, int S, int N, bool FIRST, int NLMAX, bool ROWS, int LD>
+__device__ __forceinline__ void fixed_pass_pfa(C2* buf, const C2* __restrict__ tw, int nl,
+                                               const int* __restrict__ rows) {
+  constexpr int A = Pfa<N>::A, B = Pfa<N>::B;
+  constexpr int R = FIRST ? A : B, NBF = N / R;  // radix, butterflies per line
+  constexpr int STEP = FIRST ? B : 1;            // slot step within a butterfly
+  constexpr int NB = (NBF * NLMAX + kNT - 1) / kNT;
+  constexpr int PS = ROWS ? 1 : LD;
+  const int total = NBF * nl;
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int t = threadIdx.x + it * kNT;
+    if (t < total) {
+      int j, lo;
+      if constexpr (ROWS) {
+        const int l = t / NBF;
+        j = t - l * NBF;
+        lo = rows[l] * LD;
+      } else {
+        j = t / NLMAX;
+        lo = t - j * NLMAX;
+      }
+      C2* base = buf + (FIRST ? j : j * B) * PS + lo;
+      C2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = base[r * STEP * PS];
+      dft_any<C2, R, S>(v, tw, N);
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[r * STEP * PS] = v[r];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename C2, int S, int N, int NLMAX, bool ROWS, int LD>
 __device__ __forceinline__ void fixed_fft(C2* buf, const C2* tw, int nl, const int* rows) {
   constexpr int A = Split<N>::A, B = Split<N>::B;
   static_assert(A * B == N, "radix split");
-  fixed_pass<C2, S, N, A, 1, NLMAX, ROWS, LD>(buf, tw, nl, rows);
-  fixed_pass<C2, S, N, B, A, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+  if constexpr (Pfa<N>::on) {
+    fixed_pass_pfa<C2, S, N, true, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+    fixed_pass_pfa<C2, S, N, false, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+  } else {
+    fixed_pass<C2, S, N, A, 1, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+    fixed_pass<C2, S, N, B, A, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+  }
 }
 
 template <typename C2, typename R, int NX, int NY>
@@ -123,12 +181,20 @@ __global__ void __launch_bounds__(kNT)
     const int c = tid + q * kNT;
     if (c < ncol) {
       cv[q] = row[c];
-      co[q] = col_ix[c] * LD + col_iy[c];
+      if constexpr (Pfa<NX>::on)  // frequency indices in the gamma order
+        co[q] = Pfa<NX>::gamma(col_ix[c]) * LD + Pfa<NY>::gamma(col_iy[c]);
+      else
+        co[q] = col_ix[c] * LD + col_iy[c];
     }
   }
   for (int i = tid; i < NX; i += kNT) twx[i] = twx_g[i];
   for (int i = tid; i < NY; i += kNT) twy[i] = twy_g[i];
-  for (int i = tid; i < nxa; i += kNT) xl[i] = x_list[i];
+  for (int i = tid; i < nxa; i += kNT) {
+    if constexpr (Pfa<NX>::on)
+      xl[i] = Pfa<NX>::gamma(x_list[i]);
+    else
+      xl[i] = x_list[i];
+  }
   for (int i = tid; i < NX * LD; i += kNT) P[i] = C2{R(0), R(0)};
   __syncthreads();
 #pragma unroll
@@ -136,7 +202,9 @@ __global__ void __launch_bounds__(kNT)
     if (tid + q * kNT < ncol) P[co[q]] = cv[q];
   __syncthreads();
   fixed_fft<C2, +1, NY, NX, true, LD>(P, twy, nxa, xl);
-  // The potential of the plane, in registers across the inverse x transform.
+  // The potential of the plane, in registers across the inverse x transform
+  // (for prime-factor sizes the potential boxes come in the rho order of x
+  // and y, the order of the plane's slots after the inverse transforms).
   const R* Vz = Vt + ((int64_t)vslot[k] * nz + z) * NXY;
   constexpr int VQ = (NXY + kNT - 1) / kNT;
   R vv[VQ];
@@ -204,6 +272,31 @@ int launch_any(int n, dim3 grid, size_t lds, hipStream_t st, void* T1, const voi
 }  // namespace
 
 namespace qmx {
+
+bool plane_fixed_space_order(int n, int* A, int* B) {
+  int a = 0, b = 0;
+  switch (n) {
+#define QMX_PFA(N)                          \
+  case N:                                   \
+    if (Pfa<N>::on) {                       \
+      a = Pfa<N>::A;                        \
+      b = Pfa<N>::B;                        \
+    }                                       \
+    break;
+    QMX_PFA(24)
+    QMX_PFA(30)
+    QMX_PFA(32)
+    QMX_PFA(36)
+    QMX_PFA(40)
+    QMX_PFA(48)
+    QMX_PFA(60)
+    QMX_PFA(64)
+#undef QMX_PFA
+  }
+  *A = a;
+  *B = b;
+  return a > 0;
+}
 
 bool plane_fixed_available(int nx, int ny, int ncol) {
   if (nx != ny || ncol > kColPer * kNT) return false;

@@ -51,10 +51,10 @@ typedef __bf16 rot_bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kRotThreads = 256;  // four waves
 constexpr int kRotCols = 64;      // plane waves per workgroup (16 per wave)
-// k per slab (a multiple of 16). 32 was measured slower for complex64
-// (62 against 82 TFLOP/s on 128 x (144 -> 72) over 3456 plane waves): its
-// prefetched operands take the registers (121 against 56 VGPRs) that keep
-// eight waves per SIMD resident, and those waves hide the load latency.
+// k per slab. 32 was measured slower for complex64 (62 against 82 TFLOP/s
+// on 128 x (144 -> 72) over 3456 plane waves): its prefetched operands take
+// the registers that keep more waves per SIMD resident, and those waves
+// hide the load latency.
 constexpr int kRotSlab = 16;
 
 template <typename T>
@@ -83,14 +83,19 @@ struct RotMfma<double> {
 template <int MT>
 constexpr int rot_ldi() { return (16 * MT) % 32 == 0 ? 16 * MT + 16 : 16 * MT; }
 
-// Element (i, k) of op(M) for a batch, as the kernel's real type; M may be
-// complex64 or complex128 (MC2).
+// Element (i, k) of op(M) for a batch; M may be complex64 or complex128
+// (MC2).
 template <typename MC2>
 __device__ __forceinline__ MC2 op_m(const MC2* Mb, int64_t ldm, int transM, int i, int k) {
   return transM ? Mb[(int64_t)k * ldm + i] : Mb[(int64_t)i * ldm + k];
 }
 
-template <typename T, typename MC2, int MT, int KSL = kRotSlab>
+// One slab is 16 values of k. Staging map of op(M): thread tid takes k row
+// kk = tid / 16 of the slab and rows i = 16 u + tid % 16 (u < MT) of the
+// block, so its source addresses and LDS slots are fixed offsets from two
+// bases computed once. op(M) goes through two LDS buffers: slab s + 1 is
+// written while slab s is read, one barrier per slab.
+template <typename T, typename MC2, int MT>
 __global__ void __launch_bounds__(kRotThreads)
     k_rotate(const MC2* __restrict__ M, int64_t ldm, int64_t sm, int transM,
              const typename Cplx<T>::T* __restrict__ W, int64_t ldw, int64_t sw,
@@ -99,18 +104,15 @@ __global__ void __launch_bounds__(kRotThreads)
   using C2 = typename Cplx<T>::T;
   using MF = RotMfma<T>;
   constexpr int LDI = rot_ldi<MT>();
-  constexpr int ROWS = 16 * MT;
-  constexpr int KS = KSL;  // k per slab
-  constexpr int NQ = KS / 4;        // k rows of four per slab
-  constexpr int NU = MT * KS / 16;  // op(M) values a thread stages per slab
-  static_assert(KS % 16 == 0, "slab of whole 16-row groups");
-  __shared__ T sMr[KS * LDI], sMi[KS * LDI];
+  constexpr int KS = kRotSlab;
+  constexpr int NQ = KS / 4;  // k rows of four per slab
+  static_assert(KS == 16, "the staging map covers 16 k per slab");
+  __shared__ T sMr[2][KS * LDI], sMi[2][KS * LDI];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int64_t b = blockIdx.z;
-  const int i0 = blockIdx.y * ROWS;
+  const int i0 = blockIdx.y * 16 * MT;
   const int64_t g = (int64_t)blockIdx.x * kRotCols + 16 * wave + li;
-  const MC2* Mb = M + b * sm;
   const C2* Wb = W + b * sw;
   const int64_t gc = g < G ? g : G - 1;  // clamped: loads stay in bounds
   // Three real products per complex one (Gauss): p1 = Mr Wr, p2 = Mi Wi,
@@ -122,37 +124,57 @@ __global__ void __launch_bounds__(kRotThreads)
     p2[t] = p1[t];
     p3[t] = p1[t];
   }
-  // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk.
-  // Loads are clamped in bounds and masked where used. The next slab's W
-  // and op(M) values are issued before this slab's barriers, so their
-  // latency overlaps this slab's MFMAs; op(M) element u of a thread is
-  // (row i, k kk) with tid + kRotThreads u = kk ROWS + i.
+  // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk,
+  // rows clamped to K - 1 in the last slab and masked where used.
+  const int64_t ldw4 = 4 * ldw;
+  const C2* wrow = Wb + (int64_t)lk * ldw + gc;  // k = k0 + lk, advanced per slab
   C2 wn[NQ];
-  T mrn[NU], min_[NU];
   auto load_w = [&](int k0) {
+    if (k0 + KS <= K) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int k = k0 + 4 * q + lk;
-      wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
+      for (int q = 0; q < NQ; ++q) wn[q] = wrow[q * ldw4];
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int k = k0 + 4 * q + lk;
+        wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
+      }
     }
   };
+  // op(M) staging: k row kk, rows 16 u + il of the block.
+  const int kk = tid >> 4, il = tid & 15;
+  const int64_t m_kstep = transM ? ldm : 1;    // next k
+  const int64_t m_ustep = transM ? 16 : 16 * ldm;  // next u (16 rows)
+  const MC2* mrow = M + b * sm + (int64_t)(i0 + il) * (transM ? 1 : ldm) + kk * m_kstep;
+  const int rows_left = m - i0 - il;  // row 16 u + il exists while 16 u < rows_left
+  T mrn[MT], min_[MT];
   auto load_m = [&](int k0) {
+    const bool kok = k0 + kk < K;
+    const MC2* mp = mrow;
 #pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      const int e = tid + kRotThreads * u;  // < KS * ROWS
-      const int kk = e / ROWS, i = e - kk * ROWS;
-      const int gi = i0 + i, gk = k0 + kk;
+    for (int u = 0; u < MT; ++u) {
       mrn[u] = T(0);
       min_[u] = T(0);
-      if (gi < m && gk < K) {
-        const MC2 v = op_m(Mb, ldm, transM, gi, gk);
+      if (kok && 16 * u < rows_left) {
+        const MC2 v = *mp;
         mrn[u] = (T)v.x;
         min_[u] = (T)v.y;
       }
+      mp += m_ustep;
+    }
+  };
+  auto store_m = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      sMr[buf][kk * LDI + 16 * u + il] = mrn[u];
+      sMi[buf][kk * LDI + 16 * u + il] = min_[u];
     }
   };
   load_w(0);
   load_m(0);
+  store_m(0);
+  __syncthreads();
+  int buf = 0;
   for (int k0 = 0; k0 < K; k0 += KS) {
     C2 w[NQ];
 #pragma unroll
@@ -160,35 +182,24 @@ __global__ void __launch_bounds__(kRotThreads)
       const bool ok = k0 + 4 * q + lk < K;
       w[q] = ok ? wn[q] : C2{T(0), T(0)};
     }
-    T mr[NU], mi[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      mr[u] = mrn[u];
-      mi[u] = min_[u];
-    }
-    if (k0 + KS < K) {
+    wrow += KS * ldw;
+    mrow += KS * m_kstep;
+    if (k0 + KS < K) {  // next slab's operands, in flight during this slab's MFMAs
       load_w(k0 + KS);
       load_m(k0 + KS);
     }
-    __syncthreads();  // the previous slab's LDS reads are done
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      const int e = tid + kRotThreads * u;
-      const int kk = e / ROWS, i = e - kk * ROWS;
-      sMr[kk * LDI + i] = mr[u];
-      sMi[kk * LDI + i] = mi[u];
-    }
-    __syncthreads();
     // Each of the three real products runs over all tiles before the next,
     // so consecutive MFMAs write different accumulators.
+    const T* Mr = sMr[buf];
+    const T* Mi = sMi[buf];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int kr = (4 * q + lk) * LDI + li;
       T ar[MT], ai[MT];
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
-        ar[t] = sMr[kr + 16 * t];
-        ai[t] = sMi[kr + 16 * t];
+        ar[t] = Mr[kr + 16 * t];
+        ai[t] = Mi[kr + 16 * t];
       }
       const T ws = w[q].x + w[q].y;
 #pragma unroll
@@ -198,6 +209,11 @@ __global__ void __launch_bounds__(kRotThreads)
 #pragma unroll
       for (int t = 0; t < MT; ++t) p3[t] = MF::mac(ar[t] + ai[t], ws, p3[t]);
     }
+    // The other buffer was last read a slab ago, before the last barrier
+    // (after the final slab this stores unused values).
+    buf ^= 1;
+    store_m(buf);
+    __syncthreads();
   }
   if (g >= G) return;
   C2* ob = out + b * so;
