@@ -30,7 +30,7 @@ template <> struct ZSplit<24> { static constexpr int A = 8, B = 3; };
 template <> struct ZSplit<30> { static constexpr int A = 10, B = 3; };
 template <> struct ZSplit<32> { static constexpr int A = 8, B = 4; };
 template <> struct ZSplit<36> { static constexpr int A = 9, B = 4; };
-template <> struct ZSplit<40> { static constexpr int A = 10, B = 4; };
+template <> struct ZSplit<40> { static constexpr int A = 8, B = 5; };
 template <> struct ZSplit<48> { static constexpr int A = 16, B = 3; };
 template <> struct ZSplit<60> { static constexpr int A = 15, B = 4; };
 template <> struct ZSplit<64> { static constexpr int A = 16, B = 4; };
@@ -42,10 +42,42 @@ template <> struct ZSplit<64> { static constexpr int A = 16, B = 4; };
 // the twiddle), then A transforms of size B in place, so X[k1 + A k2] ends
 // in slot B k1 + k2 (out_slot). tw is the exact N-point table; S = +1
 // inverse (unscaled), -1 forward.
+//
+// Coprime A and B take the prime-factor form instead (no twiddles between
+// the stages): the size-A transforms run over the slots (B n1 + A n2) mod N
+// (n1 < A) and the size-B ones over (B k1 + A n2) mod N (n2 < B), each in
+// place, so X[k] ends in slot (B (k mod A) + A (k mod B)) mod N.
+template <int N>
+constexpr bool zsplit_pfa() {
+  return gcd_int(ZSplit<N>::A, ZSplit<N>::B) == 1;
+}
+
 template <typename C2, int N, int S>
 __device__ __forceinline__ void line_fft(C2 (&v)[N], const C2* __restrict__ tw) {
   constexpr int A = ZSplit<N>::A, B = ZSplit<N>::B;
   static_assert(A * B == N, "radix split");
+  if constexpr (zsplit_pfa<N>()) {
+    // In place: each transform writes its outputs back to the slots it read.
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      C2 a[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) a[n1] = v[(B * n1 + A * n2) % N];
+      dft_any<C2, A, S>(a, tw, N);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) v[(B * k1 + A * n2) % N] = a[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      C2 b[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) b[n2] = v[(B * k1 + A * n2) % N];
+      dft_any<C2, B, S>(b, tw, N);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[(B * k1 + A * k2) % N] = b[k2];
+    }
+    return;
+  }
 #pragma unroll
   for (int n2 = 0; n2 < B; ++n2) {
     C2 a[A];
@@ -76,7 +108,10 @@ __device__ __forceinline__ void line_fft(C2 (&v)[N], const C2* __restrict__ tw) 
 
 template <int N>
 __device__ constexpr int out_slot(int z) {
-  return ZSplit<N>::B * (z % ZSplit<N>::A) + z / ZSplit<N>::A;
+  if constexpr (zsplit_pfa<N>())
+    return (ZSplit<N>::B * (z % ZSplit<N>::A) + ZSplit<N>::A * (z % ZSplit<N>::B)) % N;
+  else
+    return ZSplit<N>::B * (z % ZSplit<N>::A) + z / ZSplit<N>::A;
 }
 
 // Position of point z in its column's run, -1 when z is not a point of it.
