@@ -33,6 +33,14 @@ FLAGS = [
 ]
 
 
+# Per-source additions. The rotations keep their MFMA accumulators in
+# ordinary VGPRs: left to its heuristics the compiler parks the FP64 ones in
+# AGPRs and copies them back and forth every k slab (and drains the MFMA
+# pipeline to do so), and the complex64 kernel needs one more register than
+# four waves per SIMD allow.
+EXTRA_FLAGS = {"qmx_rotate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every source (in parallel, one hipcc per file) and link."""
     from concurrent.futures import ThreadPoolExecutor
@@ -44,7 +52,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         if not path.exists():
             continue
         obj = LIBDIR / (path.stem + ".o")
-        cmds.append([HIPCC, *FLAGS, "-c", str(path), "-o", str(obj)])
+        cmds.append([HIPCC, *FLAGS, *EXTRA_FLAGS.get(src, []), "-c", str(path), "-o", str(obj)])
         objs.append(str(obj))
     if verbose:
         for cmd in cmds:

@@ -90,11 +90,43 @@ __device__ __forceinline__ MC2 op_m(const MC2* Mb, int64_t ldm, int transM, int 
   return transM ? Mb[(int64_t)k * ldm + i] : Mb[(int64_t)i * ldm + k];
 }
 
-// One slab is 16 values of k. Staging map of op(M): thread tid takes k row
-// kk = tid / 16 of the slab and rows i = 16 u + tid % 16 (u < MT) of the
-// block, so its source addresses and LDS slots are fixed offsets from two
-// bases computed once. op(M) goes through two LDS buffers: slab s + 1 is
-// written while slab s is read, one barrier per slab.
+// Raw buffer resource (fourth descriptor word of the gfx9 family) and
+// loads through it: 32-bit per-lane byte offsets against a base held in
+// scalar registers, with the hardware's range check (a load at or past
+// num_records bytes returns zero).
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+  const int n = bytes < 0x7FFFFFFF ? (int)bytes : 0x7FFFFFFF;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, kRsrcWord3);
+}
+template <typename V>
+__device__ __forceinline__ V rsrc_load(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ float2 rsrc_load<float2>(__amdgpu_buffer_rsrc_t r, int off) {
+  const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  float2 v;
+  __builtin_memcpy(&v, &x, sizeof(v));
+  return v;
+}
+template <>
+__device__ __forceinline__ double2 rsrc_load<double2>(__amdgpu_buffer_rsrc_t r, int off) {
+  const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  double2 v;
+  __builtin_memcpy(&v, &x, sizeof(v));
+  return v;
+}
+constexpr int kOutOfRange = 0x7FFFFFF0;  // a byte offset past every resource here
+
+// One slab is 16 values of k. W and op(M) are read through buffer
+// resources whose base advances by one slab per slab (scalar work): lane
+// offsets are computed once. W's resource ends at row K, so rows past K
+// read zero; op(M) entries past row m or column K get the out-of-range
+// offset. Staging map of op(M): thread tid takes k row kk = tid / 16 of the
+// slab and rows i = 16 u + tid % 16 (u < MT) of the block, LDS slots at
+// fixed offsets from one base. op(M) goes through two LDS buffers: slab
+// s + 1 is written while slab s is read, one barrier per slab. The host
+// keeps every resource below 2^31 bytes.
 template <typename T, typename MC2, int MT>
 __global__ void __launch_bounds__(kRotThreads)
     k_rotate(const MC2* __restrict__ M, int64_t ldm, int64_t sm, int transM,
@@ -113,8 +145,6 @@ __global__ void __launch_bounds__(kRotThreads)
   const int64_t b = blockIdx.z;
   const int i0 = blockIdx.y * 16 * MT;
   const int64_t g = (int64_t)blockIdx.x * kRotCols + 16 * wave + li;
-  const C2* Wb = W + b * sw;
-  const int64_t gc = g < G ? g : G - 1;  // clamped: loads stay in bounds
   // Three real products per complex one (Gauss): p1 = Mr Wr, p2 = Mi Wi,
   // p3 = (Mr + Mi)(Wr + Wi); then Re = p1 - p2, Im = p3 - p1 - p2.
   typename MF::Acc p1[MT], p2[MT], p3[MT];
@@ -124,43 +154,41 @@ __global__ void __launch_bounds__(kRotThreads)
     p2[t] = p1[t];
     p3[t] = p1[t];
   }
-  // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk,
-  // rows clamped to K - 1 in the last slab and masked where used.
-  const int64_t ldw4 = 4 * ldw;
-  const C2* wrow = Wb + (int64_t)lk * ldw + gc;  // k = k0 + lk, advanced per slab
+  // W: lane (column li, k row lk) takes k = k0 + 4 q + lk; columns past G
+  // read column G - 1 and are not stored.
+  const C2* wslab = W + b * sw;
+  int64_t wleft = (int64_t)K * ldw * (int64_t)sizeof(C2);  // bytes from wslab to row K
+  const int64_t gc = g < G ? g : G - 1;  // clamped: every load stays inside W's rows
+  int woff[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) woff[q] = (int)(((4 * q + lk) * ldw + gc) * (int64_t)sizeof(C2));
   C2 wn[NQ];
-  auto load_w = [&](int k0) {
-    if (k0 + KS <= K) {
+  auto load_w = [&]() {
+    const auto r = make_rsrc(wslab, wleft);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) wn[q] = wrow[q * ldw4];
-    } else {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int k = k0 + 4 * q + lk;
-        wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
-      }
-    }
+    for (int q = 0; q < NQ; ++q) wn[q] = rsrc_load<C2>(r, woff[q]);
   };
-  // op(M) staging: k row kk, rows 16 u + il of the block.
+  // op(M): k row kk of the slab, rows 16 u + il of the block.
   const int kk = tid >> 4, il = tid & 15;
-  const int64_t m_kstep = transM ? ldm : 1;    // next k
-  const int64_t m_ustep = transM ? 16 : 16 * ldm;  // next u (16 rows)
-  const MC2* mrow = M + b * sm + (int64_t)(i0 + il) * (transM ? 1 : ldm) + kk * m_kstep;
-  const int rows_left = m - i0 - il;  // row 16 u + il exists while 16 u < rows_left
+  const int64_t m_kstep = transM ? ldm : 1;        // next k
+  const int64_t m_istep = transM ? 1 : ldm;        // next row
+  const MC2* mslab = M + b * sm;
+  int64_t mleft = ((int64_t)(transM ? K : m) * ldm) * (int64_t)sizeof(MC2);
+  int moff[MT];
+#pragma unroll
+  for (int u = 0; u < MT; ++u) {
+    const int i = i0 + 16 * u + il;
+    moff[u] = i < m ? (int)((i * m_istep + kk * m_kstep) * (int64_t)sizeof(MC2)) : kOutOfRange;
+  }
   T mrn[MT], min_[MT];
   auto load_m = [&](int k0) {
+    const auto r = make_rsrc(mslab, mleft);
     const bool kok = k0 + kk < K;
-    const MC2* mp = mrow;
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
-      mrn[u] = T(0);
-      min_[u] = T(0);
-      if (kok && 16 * u < rows_left) {
-        const MC2 v = *mp;
-        mrn[u] = (T)v.x;
-        min_[u] = (T)v.y;
-      }
-      mp += m_ustep;
+      const MC2 v = rsrc_load<MC2>(r, kok ? moff[u] : kOutOfRange);
+      mrn[u] = (T)v.x;
+      min_[u] = (T)v.y;
     }
   };
   auto store_m = [&](int buf) {
@@ -170,7 +198,7 @@ __global__ void __launch_bounds__(kRotThreads)
       sMi[buf][kk * LDI + 16 * u + il] = min_[u];
     }
   };
-  load_w(0);
+  load_w();
   load_m(0);
   store_m(0);
   __syncthreads();
@@ -178,14 +206,14 @@ __global__ void __launch_bounds__(kRotThreads)
   for (int k0 = 0; k0 < K; k0 += KS) {
     C2 w[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const bool ok = k0 + 4 * q + lk < K;
-      w[q] = ok ? wn[q] : C2{T(0), T(0)};
-    }
-    wrow += KS * ldw;
-    mrow += KS * m_kstep;
+    for (int q = 0; q < NQ; ++q) w[q] = wn[q];
     if (k0 + KS < K) {  // next slab's operands, in flight during this slab's MFMAs
-      load_w(k0 + KS);
+      const int64_t wstep = (int64_t)KS * ldw * (int64_t)sizeof(C2);
+      wslab += (int64_t)KS * ldw;
+      wleft -= wstep;
+      mslab += KS * m_kstep;
+      mleft -= (int64_t)KS * m_kstep * (int64_t)sizeof(MC2);
+      load_w();
       load_m(k0 + KS);
     }
     // Each of the three real products runs over all tiles before the next,
@@ -214,6 +242,139 @@ __global__ void __launch_bounds__(kRotThreads)
     buf ^= 1;
     store_m(buf);
     __syncthreads();
+  }
+  if (g >= G) return;
+  C2* ob = out + b * so;
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + 16 * t + MF::row(lk, r);
+      if (i < m) {
+        C2* o = ob + (int64_t)i * ldo + g;
+        C2 v{alpha * (p1[t][r] - p2[t][r]), alpha * (p3[t][r] - p1[t][r] - p2[t][r])};
+        if (beta != T(0)) {
+          const C2 p = *o;
+          v.x += beta * p.x;
+          v.y += beta * p.y;
+        }
+        *o = v;
+      }
+    }
+}
+
+// The same product with 64-bit addressing throughout, for operands past
+// the buffer-resource range (one batch of W or op(M) of 2^30 bytes or more):
+// op(M) staged in one LDS buffer (two barriers per slab), W rows clamped to
+// K - 1 in the last slab and masked where used.
+template <typename T, typename MC2, int MT, int KSL = kRotSlab>
+__global__ void __launch_bounds__(kRotThreads)
+    k_rotate_flat(const MC2* __restrict__ M, int64_t ldm, int64_t sm, int transM,
+             const typename Cplx<T>::T* __restrict__ W, int64_t ldw, int64_t sw,
+             typename Cplx<T>::T* __restrict__ out, int64_t ldo, int64_t so, int m, int K,
+             int64_t G, T alpha, T beta) {
+  using C2 = typename Cplx<T>::T;
+  using MF = RotMfma<T>;
+  constexpr int LDI = rot_ldi<MT>();
+  constexpr int ROWS = 16 * MT;
+  constexpr int KS = KSL;  // k per slab
+  constexpr int NQ = KS / 4;        // k rows of four per slab
+  constexpr int NU = MT * KS / 16;  // op(M) values a thread stages per slab
+  static_assert(KS % 16 == 0, "slab of whole 16-row groups");
+  __shared__ T sMr[KS * LDI], sMi[KS * LDI];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t b = blockIdx.z;
+  const int i0 = blockIdx.y * ROWS;
+  const int64_t g = (int64_t)blockIdx.x * kRotCols + 16 * wave + li;
+  const MC2* Mb = M + b * sm;
+  const C2* Wb = W + b * sw;
+  const int64_t gc = g < G ? g : G - 1;  // clamped: loads stay in bounds
+  // Three real products per complex one (Gauss): p1 = Mr Wr, p2 = Mi Wi,
+  // p3 = (Mr + Mi)(Wr + Wi); then Re = p1 - p2, Im = p3 - p1 - p2.
+  typename MF::Acc p1[MT], p2[MT], p3[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    p1[t] = typename MF::Acc{T(0), T(0), T(0), T(0)};
+    p2[t] = p1[t];
+    p3[t] = p1[t];
+  }
+  // W operand of a slab: lane (column li, k row lk) takes k = k0 + 4 q + lk.
+  // Loads are clamped in bounds and masked where used. The next slab's W
+  // and op(M) values are issued before this slab's barriers, so their
+  // latency overlaps this slab's MFMAs; op(M) element u of a thread is
+  // (row i, k kk) with tid + kRotThreads u = kk ROWS + i.
+  C2 wn[NQ];
+  T mrn[NU], min_[NU];
+  auto load_w = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int k = k0 + 4 * q + lk;
+      wn[q] = Wb[(int64_t)(k < K ? k : K - 1) * ldw + gc];
+    }
+  };
+  auto load_m = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = tid + kRotThreads * u;  // < KS * ROWS
+      const int kk = e / ROWS, i = e - kk * ROWS;
+      const int gi = i0 + i, gk = k0 + kk;
+      mrn[u] = T(0);
+      min_[u] = T(0);
+      if (gi < m && gk < K) {
+        const MC2 v = op_m(Mb, ldm, transM, gi, gk);
+        mrn[u] = (T)v.x;
+        min_[u] = (T)v.y;
+      }
+    }
+  };
+  load_w(0);
+  load_m(0);
+  for (int k0 = 0; k0 < K; k0 += KS) {
+    C2 w[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const bool ok = k0 + 4 * q + lk < K;
+      w[q] = ok ? wn[q] : C2{T(0), T(0)};
+    }
+    T mr[NU], mi[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      mr[u] = mrn[u];
+      mi[u] = min_[u];
+    }
+    if (k0 + KS < K) {
+      load_w(k0 + KS);
+      load_m(k0 + KS);
+    }
+    __syncthreads();  // the previous slab's LDS reads are done
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = tid + kRotThreads * u;
+      const int kk = e / ROWS, i = e - kk * ROWS;
+      sMr[kk * LDI + i] = mr[u];
+      sMi[kk * LDI + i] = mi[u];
+    }
+    __syncthreads();
+    // Each of the three real products runs over all tiles before the next,
+    // so consecutive MFMAs write different accumulators.
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int kr = (4 * q + lk) * LDI + li;
+      T ar[MT], ai[MT];
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        ar[t] = sMr[kr + 16 * t];
+        ai[t] = sMi[kr + 16 * t];
+      }
+      const T ws = w[q].x + w[q].y;
+#pragma unroll
+      for (int t = 0; t < MT; ++t) p1[t] = MF::mac(ar[t], w[q].x, p1[t]);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) p2[t] = MF::mac(ai[t], w[q].y, p2[t]);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) p3[t] = MF::mac(ar[t] + ai[t], ws, p3[t]);
+    }
   }
   if (g >= G) return;
   C2* ob = out + b * so;
@@ -400,9 +561,18 @@ void launch_fp(dim3 grid, hipStream_t st, const void* M, int64_t ldm, int64_t sm
                const void* W, int64_t ldw, int64_t sw, void* out, int64_t ldo, int64_t so, int m,
                int K, int64_t G, double alpha, double beta) {
   using C2 = typename Cplx<T>::T;
-  hipLaunchKernelGGL((k_rotate<T, MC2, MT>), grid, dim3(kRotThreads), 0, st,
-                     static_cast<const MC2*>(M), ldm, sm, transM, static_cast<const C2*>(W), ldw,
-                     sw, static_cast<C2*>(out), ldo, so, m, K, G, (T)alpha, (T)beta);
+  // Buffer resources while one batch of W and of op(M), with the lanes'
+  // offsets past them, stays below 2^30 bytes.
+  const bool rsrc = (int64_t)std::max(K, 16) * ldw * (int64_t)sizeof(C2) < (1ll << 30) &&
+                    (int64_t)(K + m + 256) * ldm * (int64_t)sizeof(MC2) < (1ll << 30);
+  if (rsrc)
+    hipLaunchKernelGGL((k_rotate<T, MC2, MT>), grid, dim3(kRotThreads), 0, st,
+                       static_cast<const MC2*>(M), ldm, sm, transM, static_cast<const C2*>(W),
+                       ldw, sw, static_cast<C2*>(out), ldo, so, m, K, G, (T)alpha, (T)beta);
+  else
+    hipLaunchKernelGGL((k_rotate_flat<T, MC2, MT>), grid, dim3(kRotThreads), 0, st,
+                       static_cast<const MC2*>(M), ldm, sm, transM, static_cast<const C2*>(W),
+                       ldw, sw, static_cast<C2*>(out), ldo, so, m, K, G, (T)alpha, (T)beta);
 }
 
 template <typename MC2, int MT>
