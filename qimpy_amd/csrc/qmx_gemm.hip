@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "qmx_common.h"
+#include "qmx_rsrc.h"
 #include "qmx_tune.h"
 
 namespace qmx {
@@ -198,6 +199,10 @@ __global__ void __launch_bounds__(kDotThreads)
 constexpr int kBlkThreads = 512;
 constexpr int kBlkRows16 = 14;  // bm16 + bn16 at most (224 staged rows)
 constexpr int kBlkTiles = 48;   // bm16 * bn16 at most
+// complex128 blocks: at most 32 tiles (four per wave); FP64 accumulators of
+// six tiles a wave do not fit the registers of two waves per SIMD.
+constexpr int kBlkTiles64 = 32;
+constexpr int blk_tiles_max(int dtype) { return dtype == QMX_C128 ? kBlkTiles64 : kBlkTiles; }
 constexpr int kBlkQuads = (kBlkRows16 * 16 * (kSlab / 4) + kBlkThreads - 1) / kBlkThreads;
 
 // LDS row stride in elements: 16-byte aligned rows whose 16 lanes of one
@@ -256,49 +261,52 @@ __global__ void __launch_bounds__(kBlkThreads)
   const C2* Ab = A + (int64_t)b * sa;
   const C2* Bb = B + (int64_t)b * sb;
   // Staging: quad q -> (staged row q / 4, 4 consecutive g at (q % 4) * 4).
-  // Loads are unconditional and in bounds (rows past M or N read row 0 of A,
-  // basis points past the chunk its last point) and are masked to zero only
-  // when stored, so a slab's loads are issued back to back and waited for
-  // after the MFMAs of the previous slab.
-  const C2* src[kBlkQuads];
-  int sofs[kBlkQuads], gq[kBlkQuads];
-  bool live[kBlkQuads];
+  // Loads go through buffer resources (qmx_rsrc.h) whose base is basis
+  // point g0 of A or of B: a wave stages 16 consecutive rows and rowsA is a
+  // multiple of 16, so the matrix is uniform per wave. A resource ends at
+  // its matrix's last point, rows past M or N take the out-of-range
+  // offset, and points past the chunk (read from the next chunk or row, or
+  // zero) are masked when stored, so a slab's loads are issued back to back
+  // and waited for after the MFMAs of the previous slab.
+  int goff[kBlkQuads], sofs[kBlkQuads], gq[kBlkQuads];
+  bool fromA[kBlkQuads], staged[kBlkQuads];
+  const int64_t endA = ((int64_t)(M - 1) * lda + G) * (int64_t)sizeof(C2);
+  const int64_t endB = ((int64_t)(N - 1) * ldb + G) * (int64_t)sizeof(C2);
 #pragma unroll
   for (int u = 0; u < kBlkQuads; ++u) {
     const int q = tid + u * kBlkThreads;
     const int row = q >> 2;
-    src[u] = Ab;
-    live[u] = false;
     gq[u] = (q & 3) * 4;
     sofs[u] = row * LD + gq[u];
-    if (q < nquads) {
-      if (row < rowsA) {
-        if (m0 + row < M) {
-          src[u] = Ab + (int64_t)(m0 + row) * lda;
-          live[u] = true;
-        }
-      } else if (n0 + row - rowsA < N) {
-        src[u] = Bb + (int64_t)(n0 + row - rowsA) * ldb;
-        live[u] = true;
-      }
+    staged[u] = __builtin_amdgcn_readfirstlane(q < nquads);
+    fromA[u] = __builtin_amdgcn_readfirstlane(row < rowsA);
+    goff[u] = kOutOfRange;
+    if (fromA[u]) {
+      if (m0 + row < M) goff[u] = (int)(((int64_t)(m0 + row) * lda + gq[u]) * (int64_t)sizeof(C2));
+    } else if (n0 + row - rowsA < N) {
+      goff[u] = (int)(((int64_t)(n0 + row - rowsA) * ldb + gq[u]) * (int64_t)sizeof(C2));
     }
   }
   C2 reg[kBlkQuads][4];
   auto load = [&](int64_t g0) {
 #pragma unroll
-    for (int u = 0; u < kBlkQuads; ++u)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        reg[u][e] = src[u][std::min<int64_t>(g0 + gq[u] + e, g_end - 1)];
+    for (int u = 0; u < kBlkQuads; ++u) {
+      if (staged[u]) {
+        const int64_t skip = g0 * (int64_t)sizeof(C2);
+        const auto r = fromA[u] ? make_rsrc(Ab + g0, endA - skip) : make_rsrc(Bb + g0, endB - skip);
+        rsrc_load4(r, goff[u], reg[u]);
+      }
+    }
   };
   auto store = [&](int64_t g0) {
+    const bool whole = g0 + kSlab <= g_end;  // no point of the slab past the chunk
 #pragma unroll
     for (int u = 0; u < kBlkQuads; ++u) {
-      if (tid + u * kBlkThreads < nquads) {
+      if (staged[u]) {
         T r[4], i[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const bool ok = live[u] && g0 + gq[u] + e < g_end;
+          const bool ok = whole || g0 + gq[u] + e < g_end;
           r[e] = ok ? reg[u][e].x : T(0);
           i[e] = ok ? reg[u][e].y : T(0);
         }
@@ -432,7 +440,8 @@ bool dot_tuned(DotPlan& p, int dtype, int64_t batch, int64_t M, int64_t N, int64
   const int64_t M16 = (M + 15) / 16, N16 = (N + 15) / 16;
   if (ch[0] == 0) {
     const int64_t bm = ch[1], bn = ch[2];
-    if (bm < 1 || bn < 1 || bm > M16 || bn > N16 || bm + bn > kBlkRows16 || bm * bn > kBlkTiles)
+    if (bm < 1 || bn < 1 || bm > M16 || bn > N16 || bm + bn > kBlkRows16 ||
+        bm * bn > blk_tiles_max(dtype))
       return false;
     p.block = true;
     p.bm16 = (int)bm;
@@ -457,7 +466,7 @@ DotPlan dot_plan(int dtype, int64_t batch, int64_t M, int64_t N, int64_t G) {
   int64_t best = -1;
   for (int bm = 1; bm <= std::min<int64_t>(M16, kBlkRows16 - 1); ++bm)
     for (int bn = 1; bn <= std::min<int64_t>(N16, kBlkRows16 - bm); ++bn) {
-      if (bm * bn > kBlkTiles) continue;
+      if (bm * bn > blk_tiles_max(dtype)) continue;
       const int64_t mb = (M16 + bm - 1) / bm, nb = (N16 + bn - 1) / bn;
       const int64_t c = tiling_cost(mb * nb * bm * bn, mb * nb * (bm + bn));
       if (best < 0 || c < best) {

@@ -34,6 +34,7 @@
 #include <string>
 
 #include "qmx_common.h"
+#include "qmx_rsrc.h"
 #include "qmx_tune.h"
 
 namespace qmx {
@@ -90,36 +91,8 @@ __device__ __forceinline__ MC2 op_m(const MC2* Mb, int64_t ldm, int transM, int 
   return transM ? Mb[(int64_t)k * ldm + i] : Mb[(int64_t)i * ldm + k];
 }
 
-// Raw buffer resource (fourth descriptor word of the gfx9 family) and
-// loads through it: 32-bit per-lane byte offsets against a base held in
-// scalar registers, with the hardware's range check (a load at or past
-// num_records bytes returns zero).
-constexpr int kRsrcWord3 = 0x00020000;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
-  const int n = bytes < 0x7FFFFFFF ? (int)bytes : 0x7FFFFFFF;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, kRsrcWord3);
-}
-template <typename V>
-__device__ __forceinline__ V rsrc_load(__amdgpu_buffer_rsrc_t r, int off);
-template <>
-__device__ __forceinline__ float2 rsrc_load<float2>(__amdgpu_buffer_rsrc_t r, int off) {
-  const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-  float2 v;
-  __builtin_memcpy(&v, &x, sizeof(v));
-  return v;
-}
-template <>
-__device__ __forceinline__ double2 rsrc_load<double2>(__amdgpu_buffer_rsrc_t r, int off) {
-  const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-  double2 v;
-  __builtin_memcpy(&v, &x, sizeof(v));
-  return v;
-}
-constexpr int kOutOfRange = 0x7FFFFFF0;  // a byte offset past every resource here
-
 // One slab is 16 values of k. W and op(M) are read through buffer
-// resources whose base advances by one slab per slab (scalar work): lane
+// resources (qmx_rsrc.h) whose base advances one slab per slab: lane
 // offsets are computed once. W's resource ends at row K, so rows past K
 // read zero; op(M) entries past row m or column K get the out-of-range
 // offset. Staging map of op(M): thread tid takes k row kk = tid / 16 of the

@@ -145,3 +145,24 @@ def test_rotate_in_place(gpu, n, kind):
         assert torch.equal(X, ref) and torch.equal(X2, ref2)
     finally:
         gemm.set_bf16x3(False)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_rotate_past_resource_range(gpu, dtype):
+    """A rotation whose long operand passes 2^30 bytes per batch takes the
+    64-bit-addressed kernel; its columns equal those of the same rotation
+    over a compact slice (the buffer-resource kernel, same arithmetic
+    order), bit for bit, and a transposed M takes the same route."""
+    K, m = 16, 40
+    G = (1 << 30) // (K * torch.empty((), dtype=dtype).element_size()) + 1000
+    print(f"[rotate_range] {dtype} K {K} m {m} G {G}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    W = torch.randn((1, K, G), dtype=dtype, device="cuda", generator=g)
+    M = torch.randn((1, m, K), dtype=C128, device="cuda", generator=g)
+    out = gemm.rotate(M.transpose(-2, -1), W)
+    for a, b in ((0, 777), (G - 700, G)):
+        ref = gemm.rotate(M.transpose(-2, -1), W[:, :, a:b].contiguous())
+        assert torch.equal(out[:, :, a:b], ref)
+    outT = gemm.rotate(M.transpose(-2, -1).contiguous(), W)
+    assert torch.equal(outT[:, :, :500],
+                       gemm.rotate(M.transpose(-2, -1).contiguous(), W[:, :, :500].contiguous()))
