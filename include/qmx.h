@@ -234,7 +234,10 @@ int qmx_band_reduce(int dtype, const void* C, int64_t ldb, int64_t ldk, int64_t 
  * also writes norm2[k, b] = sum_g |out|^2 and, if rnorm2 != NULL, the
  * unpreconditioned residual norm rnorm2[k, b] = sum_g |HC - eig C|^2, both
  * accumulated in double. eig, KEref: double [nk * nb]; ke double
- * [nk * ke_ldk]. */
+ * [nk * ke_ldk]. KEref may be NULL: the kernel then forms each band's
+ * KEref = sum_g |C|^2 KE itself (Wavefunction.band_ke, _davidson.py:95).
+ * With complex64 data the factor 1 / (x + exp(-x)) is evaluated in FP32
+ * (its argument formed in FP64). */
 int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb,
                          int64_t ldk, const double* eig, const double* KEref,
                          const double* ke, int64_t ke_ldk, void* out, int64_t out_ldb,

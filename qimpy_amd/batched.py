@@ -359,15 +359,6 @@ def flush_retired_plans(upto: Optional[int] = None) -> None:
     _retired_plans[:] = keep
 
 
-def _band_ke(X: torch.Tensor, ke: torch.Tensor) -> torch.Tensor:
-    K, B, G = X.shape
-    norm2 = torch.empty((K, B), dtype=torch.float64, device=X.device)
-    kin = torch.empty_like(norm2)
-    _lib.call("qmx_band_reduce", DTYPE_CODE[X.dtype], _c(X), X.stride(1), X.stride(0), K, B, G,
-              _c(ke), ke.stride(0), _c(norm2), _c(kin), rc.stream_ptr())
-    return kin
-
-
 def _band_norm(X: torch.Tensor) -> torch.Tensor:
     K, B, G = X.shape
     norm2 = torch.empty((K, B), dtype=torch.float64, device=X.device)
@@ -649,12 +640,12 @@ class Bucket:
         into its small matrices."""
         nb = self.nb
         C, HC, R = self.X[:, :nb], self.HX[:, :nb], self.X[:, nb:]
-        KEref = _band_ke(C, self.ke)
         norm2 = torch.empty((self.Kb, nb), dtype=torch.float64, device=rc.device)
         rnorm2 = torch.empty_like(norm2)
         eig = self.eig.contiguous()
+        # KEref (band_ke of C) is formed inside the launch (null pointer).
         _lib.call("qmx_precond_residual", DTYPE_CODE[self.dtype], _c(C), _c(HC), C.stride(1),
-                  C.stride(0), _c(eig), _c(KEref), _c(self.ke), self.ke.stride(0), _c(R),
+                  C.stride(0), _c(eig), None, _c(self.ke), self.ke.stride(0), _c(R),
                   R.stride(1), R.stride(0), self.Kb, nb, self.Gb, _c(norm2), _c(rnorm2),
                   rc.stream_ptr())
         self.rnorm = rnorm2.sqrt()

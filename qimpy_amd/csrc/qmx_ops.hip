@@ -96,7 +96,22 @@ __global__ void __launch_bounds__(kRowThreads)
   C2* o = out + k * oldk + (int64_t)b * oldb;
   const double* kk = ke + k * ke_ldk;
   const double e = eig[ir];
-  const double inv_ref = 1.0 / KEref[ir];
+  double ref;
+  if (KEref) {
+    ref = KEref[ir];
+  } else {  // the band's kinetic energy sum_g |C|^2 KE first (the row stays in cache)
+    double kin = 0, unused = 0;
+    for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
+      const C2 cc = c[g];
+      kin += ((double)cc.x * cc.x + (double)cc.y * cc.y) * kk[g];
+    }
+    block_sum2(kin, unused);
+    __shared__ double kin_all;
+    if (threadIdx.x == 0) kin_all = kin;
+    __syncthreads();
+    ref = kin_all;
+  }
+  const double inv_ref = 1.0 / ref;
   double s = 0, sr = 0;
   for (int g = threadIdx.x; g < n_tot; g += blockDim.x) {
     const double inv_x = precond_factor<LOW>(kk[g], inv_ref);
@@ -356,7 +371,7 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, 
                          int64_t ke_ldk, void* out, int64_t out_ldb, int64_t out_ldk,
                          int64_t nk, int64_t nb, int64_t n_tot, double* norm2, double* rnorm2,
                          void* stream) {
-  QMX_CHECK_ARG(C && HC && eig && KEref && ke && out && norm2,
+  QMX_CHECK_ARG(C && HC && eig && ke && out && norm2,
                 "qmx_precond_residual: null argument");
   QMX_CHECK_ARG(nk >= 0 && nk < 65536 && nb >= 0 && nb < (1 << 30) && n_tot >= 0,
                 "qmx_precond_residual: bad sizes");
@@ -364,7 +379,7 @@ int qmx_precond_residual(int dtype, const void* C, const void* HC, int64_t ldb, 
   hipStream_t st = static_cast<hipStream_t>(stream);
   dim3 grid((unsigned)nb, (unsigned)nk);
   QMX_DISPATCH(dtype,
-               hipLaunchKernelGGL((k_precond_residual<float2, false, false>), grid,
+               hipLaunchKernelGGL((k_precond_residual<float2, true, false>), grid,
                                   dim3(kRowThreads), 0, st, (const float2*)C, (const float2*)HC,
                                   ldb, ldk, eig, KEref, ke, ke_ldk, (float2*)out, out_ldb,
                                   out_ldk, (int)nb, (int)n_tot, norm2, rnorm2, nullptr),

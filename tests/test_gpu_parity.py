@@ -351,6 +351,48 @@ def test_precond_residual_normalize(cfg0_el, dtype):
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
+def test_precond_residual(cfg0_el, dtype):
+    """qmx_precond_residual (the residual pass of both solvers): the oracle's
+    preconditioned residual with its norm and the unpreconditioned residual
+    norm. Complex64 evaluates the preconditioner factor in FP32 (tolerance
+    1e-5)."""
+    from oracle.davidson import precondition as o_precondition
+    from qimpy_amd import _lib, rc
+    from qimpy_amd.basis import DTYPE_CODE
+
+    spec, ob, el = cfg0_el
+    K, B, G = len(ob.k), 5, ob.n_tot
+    C = rand_coeff(ob, B, seed=18)
+    HC = rand_coeff(ob, B, seed=19)
+    eig = np.random.default_rng(20).standard_normal((K, B))
+    KEref = np.einsum("kbg,kg->kb", np.abs(C) ** 2, ob.KE)
+    ref = o_precondition(ob, HC - C * eig[..., None], KEref)
+    dev = "cuda"
+    Ct = torch.tensor(C, device=dev).to(dtype).contiguous()
+    HCt = torch.tensor(HC, device=dev).to(dtype).contiguous()
+    out = torch.empty_like(Ct)
+    norm2 = torch.empty((K, B), dtype=torch.float64, device=dev)
+    rnorm2 = torch.empty_like(norm2)
+    eig_t = torch.tensor(eig, device=dev)
+    KEref_t = torch.tensor(KEref, device=dev)
+    ke_t = torch.tensor(ob.KE, device=dev).contiguous()
+    _lib.call("qmx_precond_residual", DTYPE_CODE[dtype], Ct.data_ptr(), HCt.data_ptr(), G, B * G,
+              eig_t.data_ptr(), KEref_t.data_ptr(), ke_t.data_ptr(), G, out.data_ptr(), G, B * G,
+              K, B, G, norm2.data_ptr(), rnorm2.data_ptr(), rc.stream_ptr())
+    tol = 1e-13 if dtype == C128 else 1e-5
+    assert rel(out.cpu().numpy(), ref) < tol
+    assert rel(norm2.sqrt().cpu().numpy(), np.sqrt((np.abs(ref) ** 2).sum(-1))) < tol
+    rn = np.sqrt((np.abs(HC - C * eig[..., None]) ** 2).sum(-1))
+    assert rel(rnorm2.sqrt().cpu().numpy(), rn) < (1e-14 if dtype == C128 else 1e-6)
+    # KEref formed inside the launch (null pointer): the same residual
+    out2 = torch.empty_like(Ct)
+    _lib.call("qmx_precond_residual", DTYPE_CODE[dtype], Ct.data_ptr(), HCt.data_ptr(), G, B * G,
+              eig_t.data_ptr(), None, ke_t.data_ptr(), G, out2.data_ptr(), G, B * G,
+              K, B, G, norm2.data_ptr(), None, rc.stream_ptr())
+    assert rel(out2.cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
 def test_hamiltonian(dtype, small_el, cfg0_el):
     for spec, ob, el in (small_el, cfg0_el):
         C = rand_coeff(ob, 6, seed=8)
