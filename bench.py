@@ -414,9 +414,13 @@ def roofline(prof, wall_s: float) -> tuple[dict, dict]:
             e["traffic_source"] = traffic["source"]
             if "valu_busy" in traffic:
                 # HBM is not this kernel's limit: the counters put it on the
-                # VALU (issue-cycle fraction, packed FP32 disabled).
+                # VALU (issue-cycle fraction, packed FP32 disabled) and on
+                # LDS (waits and bank conflicts).
                 e["limiter"] = {"unit": "VALU issue fraction", "achieved": traffic["valu_busy"],
                                 "source": traffic["source"]}
+                for key in ("lds_wait_fraction", "lds_conflict_share"):
+                    if key in traffic:
+                        e["limiter"][key] = traffic[key]
         return e
 
     lib_ids = [i for i in prof if not names[i].startswith("local_apply")]
@@ -464,8 +468,9 @@ def measured_traffic(kernel: str):
         if e.get("kernel") == kernel:
             out = {"ratio": e["traffic_over_compulsory"],
                    "source": f"{e['profile']} via profiles/pmc_traffic.json"}
-            if "valu_busy" in e:
-                out["valu_busy"] = e["valu_busy"]
+            for key in ("valu_busy", "lds_wait_fraction", "lds_conflict_share"):
+                if key in e:
+                    out[key] = e[key]
             return out
     return None
 
