@@ -479,7 +479,7 @@ class Bucket:
         nb, dt = self.nb, self.dtype
         jobs = keep + new
         if not jobs:
-            self.jobs, self.k0, self.Kb = [], [], 0
+            self.jobs, self.k0, self.Kb, self.row0 = [], [], 0, 0
             for name in ("X", "HX", "X2", "HX2", "eig", "deig", "rnorm", "pnorm2", "beta", "D",
                          "ke", "ke_r", "n32", "V", "V64", "workspace", "_full", "_beta_full",
                          "_D_full", "_ke_full"):
@@ -488,50 +488,68 @@ class Bucket:
             self._destroy_plan()
             return
         Kb = sum(j.K for j in jobs)
-        Kkeep = sum(j.K for j in keep)
         old_X, old_HX = getattr(self, "X", None), getattr(self, "HX", None)
-        # Rows of the projector, coupling and kinetic-energy buffers that a
-        # job already holds at its new offset (same buffers, same place).
+        # Kept jobs' old rows, absolute in the full buffers (the stack sits at
+        # row0 of them).
+        old_row0 = getattr(self, "row0", 0)
         old_at = {id(j): a for j, a in zip(getattr(self, "jobs", []), getattr(self, "k0", []))}
         realloc = self._buffers(Kb, max(j.n_tot for j in jobs), max(j.P for j in jobs))
-        if not realloc:
-            # same buffers: the new state goes into the spare pair, which
-            # becomes current ([X, HX, X2, HX2] bases)
-            f = self._full
-            self._full = [f[2], f[3], f[0], f[1]]
         kcap, Gb, Pb = self._cap
-        Xn, HXn, X2n, HX2n = (b[:Kb] for b in self._full)
-        eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
-        deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
-        cp = _BlockCopies()
-        if keep:
-            # Kept jobs in runs of consecutive old rows, one copy per run.
-            runs = []  # (new row, old row, rows)
-            a_new = 0
-            for j in keep:
-                a_old = old_at[id(j)]
-                if runs and runs[-1][1] + runs[-1][2] == a_old:
-                    runs[-1][2] += j.K
-                else:
-                    runs.append([a_new, a_old, j.K])
-                a_new += j.K
-            for a, o, n_rows in runs:  # (zero past the old basis length)
-                cp.add(Xn[a : a + n_rows, :nb], old_X[o : o + n_rows, :nb])
-                cp.add(HXn[a : a + n_rows, :nb], old_HX[o : o + n_rows, :nb])
-                cp.add(eig[a : a + n_rows], self.eig[o : o + n_rows])
-                cp.add(deig[a : a + n_rows], self.deig[o : o + n_rows])
         k0, acc = [], 0
         for j in jobs:
             k0.append(acc)
             acc += j.K
-        ke = self._ke_full[:Kb]
-        beta = self._beta_full[:Kb]
-        D = self._D_full[:Kb]
+        # Where the new stack starts: the start that leaves the most kept rows
+        # where they are (jobs keep their order; a retirement near the front
+        # moves the few rows before it, one near the back the rows after it).
+        row0 = 0
+        if not realloc and keep:
+            def moved(r0):
+                return sum(j.K for j, a in zip(keep, k0) if old_row0 + old_at[id(j)] != r0 + a)
+            cands = {0} | {old_row0 + old_at[id(j)] - a for j, a in zip(keep, k0)}
+            row0 = min((r for r in cands if 0 <= r and r + Kb <= kcap), key=lambda r: (moved(r), r))
+        Xn, HXn, X2n, HX2n = (b[row0 : row0 + Kb] for b in self._full)
+        eig = torch.zeros((Kb, nb), dtype=torch.float64, device=dev)
+        deig = torch.full((Kb, nb), np.inf, dtype=torch.float64, device=dev)
+        cp, cp2 = _BlockCopies(), _BlockCopies()
+        if keep:
+            # Kept jobs in runs of consecutive old rows, one copy per run. With
+            # new buffers every kept row is copied across; in the same buffers
+            # only the rows that move, through the spare pair (first copy) and
+            # back (second launch), since a move can overlap its own rows.
+            runs = []  # (new row, old row, rows), relative to the new and old stacks
+            for j, a in zip(keep, k0):
+                o = old_at[id(j)]
+                if not realloc and old_row0 + o == row0 + a:
+                    continue
+                if runs and runs[-1][1] + runs[-1][2] == o and runs[-1][0] + runs[-1][2] == a:
+                    runs[-1][2] += j.K
+                else:
+                    runs.append([a, o, j.K])
+            for a, o, n_rows in runs:  # (zero past the old basis length)
+                if realloc:
+                    cp.add(Xn[a : a + n_rows, :nb], old_X[o : o + n_rows, :nb])
+                    cp.add(HXn[a : a + n_rows, :nb], old_HX[o : o + n_rows, :nb])
+                else:
+                    cp.add(X2n[a : a + n_rows, :nb], old_X[o : o + n_rows, :nb])
+                    cp.add(HX2n[a : a + n_rows, :nb], old_HX[o : o + n_rows, :nb])
+                    cp2.add(Xn[a : a + n_rows, :nb], X2n[a : a + n_rows, :nb])
+                    cp2.add(HXn[a : a + n_rows, :nb], HX2n[a : a + n_rows, :nb])
+            for j, a in zip(keep, k0):
+                o = old_at[id(j)]
+                cp.add(eig[a : a + j.K], self.eig[o : o + j.K])
+                cp.add(deig[a : a + j.K], self.deig[o : o + j.K])
+        # Rows of the projector, coupling and kinetic-energy buffers that a
+        # job already holds at its new place (same buffers, same rows).
+        old_abs = {i: old_row0 + a for i, a in old_at.items()}
+        ke = self._ke_full[row0 : row0 + Kb]
+        beta = self._beta_full[row0 : row0 + Kb]
+        D = self._D_full[row0 : row0 + Kb]
         V64 = torch.empty((len(jobs), self.N), dtype=torch.float64, device=dev)
         v_slot = np.zeros(Kb, dtype=np.int32)
         for i, (j, a) in enumerate(zip(jobs, k0)):
             t = j.device_tables()
-            if realloc or old_at.get(id(j)) != a:
+            if realloc or old_abs.get(id(j)) != row0 + a:
                 # The job's rows with their padding zeroed (entries past its
                 # basis and projectors), written only where they moved.
                 rows = slice(a, a + j.K)
@@ -545,13 +563,14 @@ class Bucket:
             cp.add(V64[i], t["V"])
             v_slot[a : a + j.K] = i
         cp.run()
+        cp2.run()
         n = np.concatenate([j.n for j in jobs])
         self._destroy_plan()
         geom, keepalive = _lib.make_geom_rows(self.shape, n, [j.fft_index for j in jobs], Gb,
                                               v_slot)
         self._plan = _lib.create_plan(geom)
         del keepalive
-        self.jobs, self.k0, self.Kb, self.Gb, self.Pb = jobs, k0, Kb, Gb, Pb
+        self.jobs, self.k0, self.Kb, self.Gb, self.Pb, self.row0 = jobs, k0, Kb, Gb, Pb, row0
         self.X, self.HX, self.X2, self.HX2 = Xn, HXn, X2n, HX2n
         self.eig, self.deig = eig, deig
         self.ke = ke
