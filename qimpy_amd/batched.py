@@ -229,6 +229,13 @@ class CellJob:
         self._host = None
 
     def release(self) -> None:
+        """Drop the job's device tables; their memory is reused only after
+        the work queued so far on the current stream (they may have been
+        made on another one)."""
+        st = torch.cuda.current_stream() if torch.cuda.is_available() else None
+        for t in list((self._dev or {}).values()) + [getattr(self, "_beta", None)]:
+            if st is not None and t is not None and t.is_cuda:
+                t.record_stream(st)
         self._dev = None
         self._beta = None
 
@@ -464,8 +471,12 @@ class Bucket:
                    * self.K_QUANTUM)
         dev, nb, dt = rc.device, self.nb, self.dtype
         self._cap = (kcap, g_need, p_need)
-        self._full = [torch.zeros((kcap, 2 * nb, g_need), dtype=dt, device=dev) for _ in range(4)]
-        self._beta_full = torch.zeros((kcap, p_need, g_need), dtype=dt, device=dev)
+        # Not cleared: every row a step reads is written first (kept rows and
+        # moved jobs' projector, coupling and kinetic rows by the restack copy
+        # with their padding zeroed, new jobs' bands by init_steps, rotations
+        # and the residual pass over the whole basis length).
+        self._full = [torch.empty((kcap, 2 * nb, g_need), dtype=dt, device=dev) for _ in range(4)]
+        self._beta_full = torch.empty((kcap, p_need, g_need), dtype=dt, device=dev)
         # FP64 couplings: they scale the complex128 projections directly.
         self._D_full = torch.zeros((kcap, p_need), dtype=torch.float64, device=dev)
         self._ke_full = torch.zeros((kcap, g_need), dtype=torch.float64, device=dev)
@@ -954,11 +965,10 @@ class BatchSolver:
     Davidson step per visit of its group and its results are those of a
     single-group solver. Two groups are the default: measured on the mixed
     served stream (1300 requests), two give 92 band solves/s against 84-86
-    for one, and three no more than two. Every launch goes to the caller's
-    current stream:
-    the library routines behind torch.linalg and the batched GEMMs share one
-    handle (and its device workspace) per thread across streams. ``run``
-    ticks until every submitted job is done.
+    for one, and three no more than two. Each group's launches go to a
+    stream of its own, so that one group's latency-bound phases (the grouped
+    eigensolves) overlap the other group's kernels; cached workspaces are
+    kept per stream. ``run`` ticks until every submitted job is done.
     """
 
     def __init__(self, dtype: torch.dtype = C64, *, eig_tolerance: Optional[float] = 1e-6,
@@ -997,6 +1007,13 @@ class BatchSolver:
         self._tail = [_GroupTail() for _ in range(self.n_groups)]
         self._turn = 0
         self.n_ticks = 0
+        # One stream per bucket group (default priority): a bucket stays in
+        # its group, so each group's work is ordered on its stream, and while
+        # one group is in a latency-bound phase (its grouped eigensolves above
+        # all, a few workgroups per matrix) the other group's kernels fill the
+        # device.
+        self._streams = ([torch.cuda.Stream() for _ in range(self.n_groups)]
+                         if self.n_groups > 1 else [torch.cuda.current_stream()])
         torch.cuda.synchronize()
         self._t0_event = torch.cuda.Event(enable_timing=True)
         self._t0_event.record()
@@ -1092,9 +1109,13 @@ class BatchSolver:
             self.results[job.job_id] = res
 
     def tick(self, now: Optional[float] = None) -> list[CellResult]:
-        """One iteration of the buckets of the next group; returns the
-        results that became available (that group's jobs finished in its
-        previous tick)."""
+        """One iteration of the buckets of the next group, on that group's
+        stream; returns the results that became available (that group's jobs
+        finished in its previous tick)."""
+        with torch.cuda.stream(self._streams[self._turn]):
+            return self._tick(now)
+
+    def _tick(self, now: Optional[float]) -> list[CellResult]:
         g = self._turn
         self._turn = (g + 1) % self.n_groups
         _stamp[0] += 1
@@ -1168,9 +1189,11 @@ class BatchSolver:
             nxt.fails = torch.stack([(i != 0).sum() for i in self._info]).sum().to(
                 "cpu", non_blocking=True)
             self._info = []
-        if nxt.packs or nxt.pending or nxt.fails is not None:
-            nxt.event = torch.cuda.Event()
-            nxt.event.record()
+        # Recorded every tick (also one that left nothing to read): the
+        # group's next tick waits on it before it frees retired plans, whose
+        # last launches may be on this group's stream.
+        nxt.event = torch.cuda.Event()
+        nxt.event.record()
         self.n_ticks += 1
         return new_results
 

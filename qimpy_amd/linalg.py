@@ -214,10 +214,11 @@ def eigh_large(H: torch.Tensor, n_want: Optional[int] = None):
     if Hf.stride(-1) != 1 or Hf.stride(-2) != n:
         Hf = Hf.contiguous()
     need = int(_lib.lib().qmx_heevx_large_work_bytes(n, nw))
-    ws = _large_ws.get(H.device)
+    key = (H.device, rc.stream_ptr())  # one workspace per stream: streams may overlap
+    ws = _large_ws.get(key)
     if ws is None or ws.numel() < need:
         ws = torch.empty(need, dtype=torch.uint8, device=H.device)
-        _large_ws[H.device] = ws
+        _large_ws[key] = ws
     B = Hf.shape[0]
     E = torch.empty((B, n), dtype=torch.float64, device=H.device)
     Vt = torch.empty((B, nw, n), dtype=torch.complex128, device=H.device)
@@ -287,10 +288,11 @@ def cholesky_inverse(O: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     if Of.stride(-1) != 1 or Of.stride(-2) != n:
         Of = Of.contiguous()
     need = int(_lib.lib().qmx_potrf_inv_work_bytes(n))
-    ws = _potrf_ws.get(O.device)
+    key = (O.device, rc.stream_ptr())  # one workspace per stream: streams may overlap
+    ws = _potrf_ws.get(key)
     if ws is None or ws.numel() < need:
         ws = torch.empty(need, dtype=torch.uint8, device=O.device)
-        _potrf_ws[O.device] = ws
+        _potrf_ws[key] = ws
     X = torch.empty_like(Of)
     info = torch.empty(Of.shape[0], dtype=torch.int32, device=O.device)
     for b in range(Of.shape[0]):
