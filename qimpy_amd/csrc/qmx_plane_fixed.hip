@@ -35,39 +35,101 @@ template <> struct Split<48> { static constexpr int A = 16, B = 3; };
 template <> struct Split<60> { static constexpr int A = 15, B = 4; };
 template <> struct Split<64> { static constexpr int A = 16, B = 4; };
 
+// Thread t's butterfly j and line l over nl row lines, rows fastest:
+// consecutive lanes take the same butterfly of consecutive rows, so their
+// LDS addresses differ by the odd row pitch LD and a wave's 8-byte reads and
+// writes fall on distinct banks (butterflies fastest would put them B slots
+// apart, on banks shared by several lanes). t / nl through a float
+// reciprocal: (t + 1/2) / nl is at least 1/(2 nl) >= 1/128 from an integer
+// and t < 2^13, so the rounding of the product (a few ulp) never reaches the
+// next integer.
+__device__ __forceinline__ void row_line(int t, int nl, float rinv, int& j, int& l) {
+  j = static_cast<int>((static_cast<float>(t) + 0.5f) * rinv);
+  l = t - j * nl;
+}
+
+__device__ __forceinline__ float row_rinv(int nl) {
+  return __builtin_amdgcn_rcpf(static_cast<float>(nl > 0 ? nl : 1));
+}
+
+// Lane map of each pass (first, second) of the row transforms (true: rows
+// fastest, row_line; false: butterflies fastest) and of the column
+// transforms (true: one butterfly per wave, lanes on the NY columns; false:
+// lanes packed over (butterfly, column), a wave's lanes wrapping to the next
+// butterfly mid-wave). gfx950's LDS serves an 8-byte access 32 lanes a
+// cycle over 64 banks of 4 bytes (an LDS probe: strides of 8, 24, 40 and
+// odd multiples of 8 bytes take 2 cycles a wave, 16 bytes 4, 32 bytes 8).
+// At 60 points the packed column map costs 3 cycles for 2 where a wave
+// wraps, and the radix-4 row pass reads 32-byte strided slots: the maps
+// below cut the plane's LDS cycles by a fifth there and its time by 2.5%.
+// At the other sizes the packed maps were as fast or faster when timed
+// (fewer partly filled waves), so they keep them.
+template <int N> struct LaneMap { static constexpr bool row0 = false, row1 = false, col0 = false, col1 = false; };
+template <> struct LaneMap<60> { static constexpr bool row0 = true, row1 = false, col0 = true, col1 = true; };
+
+// Thread t's (butterfly j, line lo offset) of a pass with NBF butterflies
+// per line over nl lines (the row offset through rows[] in ROWS mode); false
+// where t is past the pass's work.
+template <int NBF, int NLMAX, bool ROWS, bool MAP, int LD>
+__device__ __forceinline__ bool pass_item(int it, int nl, float rinv, const int* rows, int& j,
+                                          int& lo) {
+  if constexpr (ROWS) {
+    const int t = threadIdx.x + it * kNT;
+    int l;
+    if constexpr (MAP) {
+      row_line(t, nl, rinv, j, l);
+    } else {
+      l = t / NBF;
+      j = t - l * NBF;
+    }
+    const bool on = t < NBF * nl;
+    lo = on ? rows[l] * LD : 0;
+    return on;
+  } else if constexpr (MAP) {
+    const int lane = threadIdx.x & 63;
+    j = it * (kNT / 64) + (threadIdx.x >> 6);
+    lo = lane;
+    return lane < NLMAX && j < NBF;
+  } else {
+    const int t = threadIdx.x + it * kNT;
+    j = t / NLMAX;
+    lo = t - j * NLMAX;
+    return t < NBF * nl;
+  }
+}
+
+// Iterations of a pass over the block's threads.
+template <int NBF, int NLMAX, bool ROWS, bool MAP>
+constexpr int pass_iters() {
+  return (!ROWS && MAP) ? (NBF + kNT / 64 - 1) / (kNT / 64) : (NBF * NLMAX + kNT - 1) / kNT;
+}
+
 // One in-place Stockham pass of radix R (sub-transform length NS before it)
 // over nl lines of length N. ROWS: line l is row rows[l] of the plane, its
-// points contiguous (consecutive threads take consecutive butterflies of a
-// line); otherwise line l is column l (NL columns, stride LD between points;
-// consecutive threads take consecutive columns). Values are held in
-// registers across the barrier that separates the reads from the writes.
-template <typename C2, int S, int N, int R, int NS, int NLMAX, bool ROWS, int LD>
+// points contiguous; otherwise line l is column l (NL columns, stride LD
+// between points). Threads map to (butterfly, line) by pass_item under the
+// pass's LaneMap. Values are held in registers across the barrier that
+// separates the reads from the writes.
+template <typename C2, int S, int N, int R, int NS, int NLMAX, bool ROWS, bool MAP, int LD>
 __device__ __forceinline__ void fixed_pass(C2* buf, const C2* __restrict__ tw, int nl,
                                            const int* __restrict__ rows) {
   constexpr int STRIDE = N / R, TWSTEP = N / (NS * R);
-  constexpr int NB = (STRIDE * NLMAX + kNT - 1) / kNT;
+  constexpr int NB = pass_iters<STRIDE, NLMAX, ROWS, MAP>();
   constexpr int PS = ROWS ? 1 : LD;  // distance between the points of a line
-  const int total = STRIDE * nl;
+  const float rinv = ROWS ? row_rinv(nl) : 0.f;
   C2 v[NB][R];
-  int in0[NB], out0[NB];
+  int out0[NB];
+  bool on[NB];
 #pragma unroll
   for (int it = 0; it < NB; ++it) {
-    const int t = threadIdx.x + it * kNT;
     int j, lo;
-    if constexpr (ROWS) {
-      const int l = t / STRIDE;
-      j = t - l * STRIDE;
-      lo = t < total ? rows[l] * LD : 0;
-    } else {
-      j = t / NLMAX;
-      lo = t - j * NLMAX;
-    }
+    on[it] = pass_item<STRIDE, NLMAX, ROWS, MAP, LD>(it, nl, rinv, rows, j, lo);
     const int k = j % NS;
-    in0[it] = j * PS + lo;
+    const int in0 = j * PS + lo;
     out0[it] = ((j - k) * R + k) * PS + lo;
-    if (t < total) {
+    if (on[it]) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = buf[in0[it] + r * STRIDE * PS];
+      for (int r = 0; r < R; ++r) v[it][r] = buf[in0 + r * STRIDE * PS];
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -82,8 +144,7 @@ __device__ __forceinline__ void fixed_pass(C2* buf, const C2* __restrict__ tw, i
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < NB; ++it) {
-    const int t = threadIdx.x + it * kNT;
-    if (t < total) {
+    if (on[it]) {
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[out0[it] + r * NS * PS] = v[it][r];
     }
@@ -110,28 +171,19 @@ struct Pfa {
   __device__ static int gamma(int k) { return (k % A) * B + k % B; }
 };
 
-template <typename C2, int S, int N, bool FIRST, int NLMAX, bool ROWS, int LD>
+template <typename C2, int S, int N, bool FIRST, int NLMAX, bool ROWS, bool MAP, int LD>
 __device__ __forceinline__ void fixed_pass_pfa(C2* buf, const C2* __restrict__ tw, int nl,
                                                const int* __restrict__ rows) {
   constexpr int A = Pfa<N>::A, B = Pfa<N>::B;
   constexpr int R = FIRST ? A : B, NBF = N / R;  // radix, butterflies per line
   constexpr int STEP = FIRST ? B : 1;            // slot step within a butterfly
-  constexpr int NB = (NBF * NLMAX + kNT - 1) / kNT;
+  constexpr int NB = pass_iters<NBF, NLMAX, ROWS, MAP>();
   constexpr int PS = ROWS ? 1 : LD;
-  const int total = NBF * nl;
+  const float rinv = ROWS ? row_rinv(nl) : 0.f;
 #pragma unroll
   for (int it = 0; it < NB; ++it) {
-    const int t = threadIdx.x + it * kNT;
-    if (t < total) {
-      int j, lo;
-      if constexpr (ROWS) {
-        const int l = t / NBF;
-        j = t - l * NBF;
-        lo = rows[l] * LD;
-      } else {
-        j = t / NLMAX;
-        lo = t - j * NLMAX;
-      }
+    int j, lo;
+    if (pass_item<NBF, NLMAX, ROWS, MAP, LD>(it, nl, rinv, rows, j, lo)) {
       C2* base = buf + (FIRST ? j : j * B) * PS + lo;
       C2 v[R];
 #pragma unroll
@@ -148,12 +200,15 @@ template <typename C2, int S, int N, int NLMAX, bool ROWS, int LD>
 __device__ __forceinline__ void fixed_fft(C2* buf, const C2* tw, int nl, const int* rows) {
   constexpr int A = Split<N>::A, B = Split<N>::B;
   static_assert(A * B == N, "radix split");
+  constexpr bool M0 = ROWS ? LaneMap<N>::row0 : LaneMap<N>::col0;
+  constexpr bool M1 = ROWS ? LaneMap<N>::row1 : LaneMap<N>::col1;
+  static_assert(ROWS || !(M0 || M1) || NLMAX <= 64, "one butterfly per wave needs NY <= 64");
   if constexpr (Pfa<N>::on) {
-    fixed_pass_pfa<C2, S, N, true, NLMAX, ROWS, LD>(buf, tw, nl, rows);
-    fixed_pass_pfa<C2, S, N, false, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+    fixed_pass_pfa<C2, S, N, true, NLMAX, ROWS, M0, LD>(buf, tw, nl, rows);
+    fixed_pass_pfa<C2, S, N, false, NLMAX, ROWS, M1, LD>(buf, tw, nl, rows);
   } else {
-    fixed_pass<C2, S, N, A, 1, NLMAX, ROWS, LD>(buf, tw, nl, rows);
-    fixed_pass<C2, S, N, B, A, NLMAX, ROWS, LD>(buf, tw, nl, rows);
+    fixed_pass<C2, S, N, A, 1, NLMAX, ROWS, M0, LD>(buf, tw, nl, rows);
+    fixed_pass<C2, S, N, B, A, NLMAX, ROWS, M1, LD>(buf, tw, nl, rows);
   }
 }
 
