@@ -975,7 +975,8 @@ class BatchSolver:
                  residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
                  band_classes: Optional[Sequence[int]] = (32, 72), hbm_budget: float = 200e9,
-                 max_k_per_bucket: int = 2048, max_bucket_bytes: float = 16e9, groups: int = 2):
+                 max_k_per_bucket: int = 2048, max_bucket_bytes: float = 16e9, groups: int = 2,
+                 streams: bool = True):
         rc.require_gpu()
         from . import autotune
 
@@ -1012,8 +1013,9 @@ class BatchSolver:
         # one group is in a latency-bound phase (its grouped eigensolves above
         # all, a few workgroups per matrix) the other group's kernels fill the
         # device.
+        # streams=False keeps every launch on the caller's current stream.
         self._streams = ([torch.cuda.Stream() for _ in range(self.n_groups)]
-                         if self.n_groups > 1 else [torch.cuda.current_stream()])
+                         if streams and self.n_groups > 1 else None)
         torch.cuda.synchronize()
         self._t0_event = torch.cuda.Event(enable_timing=True)
         self._t0_event.record()
@@ -1112,6 +1114,8 @@ class BatchSolver:
         """One iteration of the buckets of the next group, on that group's
         stream; returns the results that became available (that group's jobs
         finished in its previous tick)."""
+        if self._streams is None:
+            return self._tick(now)
         with torch.cuda.stream(self._streams[self._turn]):
             return self._tick(now)
 
