@@ -42,6 +42,27 @@ def _solver(dtype, **kw):
     return BatchSolver(dtype, **args)
 
 
+@pytest.mark.parametrize("dtype", [C128, C64], ids=["plain", "mixed"])
+def test_group_streams_match_one_stream(gpu, dtype):
+    """Two bucket groups on streams of their own give the results of the
+    same groups on one stream: the same launches per group in the same
+    order, only overlapped across groups (eigenvalues and iteration counts
+    equal, bit for bit)."""
+    specs = _specs()
+    runs = []
+    for streams in (True, False):
+        solver = _solver(dtype, groups=2, streams=streams)
+        solver.submit([CellJob(s, job_id=i) for i, s in enumerate(specs)])
+        runs.append(solver.run())
+    a, b = runs
+    assert sorted(a) == sorted(b) == list(range(len(specs)))
+    for i in a:
+        print(f"[group streams {dtype}] cell {i}: {a[i].iterations} / {b[i].iterations} iterations",
+              flush=True)
+        assert a[i].iterations == b[i].iterations
+        assert np.array_equal(a[i].eig, b[i].eig)
+
+
 @pytest.mark.parametrize("groups", [1, 2])
 @pytest.mark.parametrize("dtype", [C128, C64], ids=["plain", "mixed"])
 def test_batched_cells_match_dense(gpu, dense_refs, dtype, groups):
