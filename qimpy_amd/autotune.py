@@ -310,7 +310,11 @@ def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--cells", type=int, default=96, help="cells of the served distribution")
     ap.add_argument("--out", default=None, help="cache file (default: the shipped one)")
+    ap.add_argument("--band-classes", default=None,
+                    help="comma-separated band classes of the solver (default: its own)")
     args = ap.parse_args(argv)
+    classes = ({"band_classes": tuple(int(c) for c in args.band_classes.split(",") if c)}
+               if args.band_classes else {})
     rc.init()
     rc.require_gpu()
     global _loaded_default
@@ -320,7 +324,7 @@ def main(argv=None) -> None:
     specs = [systems.make_cfg2_cell(i, cfg=3) for i in range(args.cells)]
     with record() as shapes:
         for dt in (torch.complex64, torch.complex128):
-            solver = BatchSolver(dt, max_iterations=60)
+            solver = BatchSolver(dt, max_iterations=60, **classes)
             solver.submit([CellJob(s, job_id=i) for i, s in enumerate(specs)])
             solver.run()
             del solver
