@@ -56,7 +56,7 @@ RATE = 200.0  # requests per second
 EIG_TOLERANCE = 1e-6  # mixed form: per-band eigenvalue error target (Ha)
 PLAIN_THRESHOLD = 1e-8  # plain form: the reference's |deig| stop
 #: Band classes of the batched solver (jobs pad to the smallest that holds them).
-BAND_CLASSES = (32, 72)
+BAND_CLASSES = (32, 48, 72)
 #: Bucket groups the served solver deals its buckets to (served in turn).
 GROUPS = 2
 CHEFSI_RESIDUAL = 3e-5
