@@ -511,13 +511,17 @@ __global__ void __launch_bounds__(NT)
   const int z = blockIdx.x, ib = blockIdx.y, k = blockIdx.z;
   const int tid = threadIdx.x;
   C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
-  C2 cv[kPlaneColPer];
+  // Staged columns as separate real and imaginary parts (an array of
+  // double2 held across the zero fill is kept in scratch memory).
+  R cvx[kPlaneColPer], cvy[kPlaneColPer];
   int co[kPlaneColPer];
 #pragma unroll
   for (int q = 0; q < kPlaneColPer; ++q) {
     const int c = tid + q * NT;
     if (c < ncol) {
-      cv[q] = row[c];
+      const C2 t = row[c];
+      cvx[q] = t.x;
+      cvy[q] = t.y;
       co[q] = col_ix[c] * ldp + col_iy[c];
     }
   }
@@ -528,7 +532,7 @@ __global__ void __launch_bounds__(NT)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPlaneColPer; ++q)
-    if (tid + q * NT < ncol) P[co[q]] = cv[q];
+    if (tid + q * NT < ncol) P[co[q]] = C2{cvx[q], cvy[q]};
   __syncthreads();
   fft_ip<C2, +1, NT, MAXE, false, true>(P, twy, ry, nxa, fnxa, 1, ldp, xl);
   const R* Vz = Vt + ((int64_t)vslot[k] * nz + z) * nx * ny;
@@ -611,14 +615,20 @@ __global__ void __launch_bounds__(NT)
     const double w = prefac[(int64_t)k * ldp + b0 + ib];
     if (w == 0.0) continue;  // empty band (uniform over the workgroup)
     const int64_t roff = (((int64_t)k * nb + ib) * nz + z) * ncol;
-    C2 cv[kNp][kPlaneColPer];
+    R cvx[kNp][kPlaneColPer], cvy[kNp][kPlaneColPer];  // real, imaginary (no scratch)
     int co[kPlaneColPer];
 #pragma unroll
     for (int q = 0; q < kPlaneColPer; ++q) {
       const int c = tid + q * NT;
       if (c < ncol) {
-        cv[0][q] = T1[roff + c];
-        if constexpr (CROSS) cv[1][q] = T1b[roff + c];
+        const C2 t = T1[roff + c];
+        cvx[0][q] = t.x;
+        cvy[0][q] = t.y;
+        if constexpr (CROSS) {
+          const C2 u = T1b[roff + c];
+          cvx[1][q] = u.x;
+          cvy[1][q] = u.y;
+        }
         co[q] = col_ix[c] * ldp_ + col_iy[c];
       }
     }
@@ -627,8 +637,8 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int q = 0; q < kPlaneColPer; ++q)
       if (tid + q * NT < ncol) {
-        P[co[q]] = cv[0][q];
-        if constexpr (CROSS) P[plane + co[q]] = cv[1][q];
+        P[co[q]] = C2{cvx[0][q], cvy[0][q]};
+        if constexpr (CROSS) P[plane + co[q]] = C2{cvx[1][q], cvy[1][q]};
       }
     __syncthreads();
 #pragma unroll

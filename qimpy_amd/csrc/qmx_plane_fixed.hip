@@ -229,13 +229,17 @@ __global__ void __launch_bounds__(kNT)
   const int z = blockIdx.x, ib = blockIdx.y, k = blockIdx.z;
   const int tid = threadIdx.x;
   C2* row = T1 + (((int64_t)k * nb + ib) * nz + z) * ncol;
-  C2 cv[kColPer];
+  // The staged columns as separate real and imaginary parts: held in an
+  // array of double2 across the zero fill, they were kept in scratch memory.
+  R cvx[kColPer], cvy[kColPer];
   int co[kColPer];
 #pragma unroll
   for (int q = 0; q < kColPer; ++q) {
     const int c = tid + q * kNT;
     if (c < ncol) {
-      cv[q] = row[c];
+      const C2 t = row[c];
+      cvx[q] = t.x;
+      cvy[q] = t.y;
       if constexpr (Pfa<NX>::on)  // frequency indices in the gamma order
         co[q] = Pfa<NX>::gamma(col_ix[c]) * LD + Pfa<NY>::gamma(col_iy[c]);
       else
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(kNT)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kColPer; ++q)
-    if (tid + q * kNT < ncol) P[co[q]] = cv[q];
+    if (tid + q * kNT < ncol) P[co[q]] = C2{cvx[q], cvy[q]};
   __syncthreads();
   fixed_fft<C2, +1, NY, NX, true, LD>(P, twy, nxa, xl);
   // The potential of the plane, in registers across the inverse x transform
