@@ -8,7 +8,7 @@ workload (cubic boxes 24..60^3, 8-64 bands, 1-16 k-points, 2-16 atoms with 8
 projectors each, 20 Ha). Every request's inputs (sphere tables, potential
 box, k-points, positions, couplings) are staged in HBM before the stream
 starts; a request enters the server at its arrival time, which admits it
-against a 200 GB HBM budget and iterates every resident request in lockstep
+against a 180 GB HBM budget and iterates every resident request in lockstep
 batches (continuous batching) until its FP64 result is ready. A step is a
 window of `window` consecutive completed requests (in completion order):
 the first `warmup` windows are untimed, the next `steps` are timed, from
@@ -195,7 +195,7 @@ def served(specs, arrivals, dtype, staged=None, profile=False, streams=True):
 
     jobs = None if staged is None else [j.fresh(a) for j, a in zip(staged, arrivals)]
     solver = BatchSolver(dtype, eig_tolerance=EIG_TOLERANCE, eig_threshold=PLAIN_THRESHOLD,
-                         max_iterations=200, hbm_budget=200e9, band_classes=BAND_CLASSES,
+                         max_iterations=200, hbm_budget=180e9, band_classes=BAND_CLASSES,
                          groups=GROUPS, streams=streams)
     if profile:
         _lib.profile_enable(True)
@@ -567,7 +567,7 @@ def main():
         "config": {"workload": (f"served stream: {n_req} requests, Poisson {args.rate:.0f} req/s, "
                                 "cells of the uneven batched distribution (boxes 24..60^3, 8-64 "
                                 "bands, 1-16 k, 2-16 atoms x 8 projectors, 20 Ha), inputs staged "
-                                "in HBM, HBM budget 200 GB; a step is a window of "
+                                "in HBM, HBM budget 180 GB; a step is a window of "
                                 f"{args.window} completed requests"),
                    "requests": n_req, "rate_per_s": args.rate, "window": args.window,
                    "inputs": "host arrays" if args.host_inputs else "staged in HBM",

@@ -974,7 +974,7 @@ class BatchSolver:
     def __init__(self, dtype: torch.dtype = C64, *, eig_tolerance: Optional[float] = 1e-6,
                  residual_threshold: float = 3e-4,
                  eig_threshold: float = 1e-8, max_iterations: int = 200,
-                 band_classes: Optional[Sequence[int]] = (32, 48, 72), hbm_budget: float = 200e9,
+                 band_classes: Optional[Sequence[int]] = (32, 48, 72), hbm_budget: float = 180e9,
                  max_k_per_bucket: int = 2048, max_bucket_bytes: float = 16e9, groups: int = 2,
                  streams: bool = True):
         rc.require_gpu()
